@@ -1,0 +1,140 @@
+/*
+ * mosaic_hip.h -- C ABI of the gfx950 (MI355X) USFFT Radon transform and
+ * Fresnel propagator that replace model_mosaic's cupy operators.
+ *
+ * Conventions
+ *   - Every entry point that can fail returns an int status: MM_OK (0) or a
+ *     negative MM_ERR_* code.  mm_last_error() returns the calling thread's
+ *     message for the last failure.  No C++ exception crosses this ABI.
+ *   - Device pointers are caller-owned, contiguous, on the device that is
+ *     current when the plan was created.  Host pointers are read during the
+ *     call only.
+ *   - Work is enqueued on the HIP stream passed as `stream` (NULL = default
+ *     stream) and is asynchronous with respect to the host.
+ *   - A plan owns its workspace and its per-angle tables.  A plan is not
+ *     thread-safe: one in-flight stream per plan.
+ *   - float2 = complex64 (interleaved re, im), row-major layouts as stated.
+ */
+#ifndef MOSAIC_HIP_H
+#define MOSAIC_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_ERR_ARG (-1)         /* bad argument, shape or size */
+#define MM_ERR_OOM (-2)         /* device allocation failed */
+#define MM_ERR_HIP (-3)         /* HIP runtime error */
+#define MM_ERR_UNSUPPORTED (-4) /* size the kernels do not cover */
+
+typedef struct mm_radon mm_radon;
+typedef struct mm_fresnel mm_fresnel;
+typedef struct mm_model mm_model;
+
+/* Message for the last failure on the calling thread ("" if none). */
+const char* mm_last_error(void);
+/* ABI version (major*100 + minor). */
+int mm_version(void);
+
+/* ---------------------------------------------------------------------------
+ * USFFT Radon transform.
+ * Replaces processing/tomo.py:17 TomoReal (constructor :30, R :121, RT :153,
+ * free :109) and the device work of processing/tomo_large.py:46 TomoLargeReal
+ * (R :344, RT :582).  eps = 1e-3, mu = -ln(eps)/(2 n^2), m = 4 taps each side
+ * (processing/tomo.py:39-42), 2x oversampled Fourier grid.
+ *
+ * mm_radon_create: n a power of two in [16, 8192], nz_max >= 1,
+ *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
+ *   the reference stores theta as float32 -- pass those values).
+ *   flags: 0, or MM_RADON_FP64_TILES (scatter tiles accumulate in FP64).
+ */
+#define MM_RADON_FP64_TILES 1u
+int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags);
+
+/* R: obj (nz, n, n) float32 -> sino (ntheta, nz, n) float32, 1 <= nz <= nz_max.
+ * Same operator as processing/tomo.py:121-151 (TomoReal.R). */
+int mm_radon_fwd(mm_radon* plan, const float* obj_dev, float* sino_dev, int nz, void* stream);
+
+/* RT: sino (ntheta, nz, n) float32 (sino_is_c64 = 0) or complex64 (= 1)
+ *  -> obj (nz, n, n) of the same element type.
+ * Same operator as processing/tomo.py:153-188 (TomoReal.RT): real part for a
+ * float32 sinogram, the full complex result for a complex64 one. */
+int mm_radon_adj(mm_radon* plan, const void* sino_dev, int sino_is_c64, void* obj_dev, int nz, void* stream);
+
+/* Stage timing (no counterpart in the reference; for benchmarks).
+ * mm_radon_profile(plan, 1) makes every later R / RT call record HIP events
+ * on its stream around each kernel stage; mm_radon_stage_ms waits for the
+ * events of the last R and RT calls and writes MM_RADON_NSTAGES durations
+ * (ms, -1 if the stage has not run): R row rfft, R column FFT, R gather,
+ * R line IFFT, RT line FFT, RT scatter (+ partial-tile sum), RT column IFFT,
+ * RT row C2R. */
+#define MM_RADON_NSTAGES 8
+int mm_radon_profile(mm_radon* plan, int enable);
+int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
+
+/* Device bytes the plan holds (workspace + tables). */
+size_t mm_radon_workspace_bytes(const mm_radon* plan);
+void mm_radon_destroy(mm_radon* plan);
+
+/* ---------------------------------------------------------------------------
+ * Fresnel propagation.
+ * Replaces processing/propagation.py:15 Propagation (constructor :18, D :47)
+ * and processing/propagation_large.py:35 PropagationLarge.D (:202); DT is the
+ * exact adjoint of D (the reference has no DT, propagation_large.py:15).
+ *
+ * mm_fresnel_create: frame (nz, n), nz and n even powers of two in [8, 8192];
+ *   ndist distances (dist_host, metres), wavelength and voxelsize in metres.
+ *   pad_mode 0 = symmetric mirror padding (processing/kernels.py:530-550, the
+ *   reference), 1 = zero padding.  The chirp exp(-i pi lambda z f^2) is built in
+ *   FP64 with the phase reduced mod 2 pi, then rounded to complex64.
+ */
+int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavelength, double voxelsize,
+                      const double* dist_host, int pad_mode);
+
+/* D: psi (nframes, nz, n) complex64 -> out (nframes, nz, n) complex64 with
+ * distance index j.  out may equal psi (in place). */
+int mm_fresnel_fwd(mm_fresnel* plan, const void* psi_dev, void* out_dev, int nframes, int j, void* stream);
+
+/* DT: adjoint of D, same shapes.  out may equal in. */
+int mm_fresnel_adj(mm_fresnel* plan, const void* in_dev, void* out_dev, int nframes, int j, void* stream);
+
+size_t mm_fresnel_workspace_bytes(const mm_fresnel* plan);
+void mm_fresnel_destroy(mm_fresnel* plan);
+
+/* ---------------------------------------------------------------------------
+ * Element-wise kernels of step3_propagation.py:94-110.
+ * psi = exp(-p*inv_norm*inv_beta_ratio) * (cos(p*inv_norm) + i sin(p*inv_norm)).
+ */
+int mm_psi_from_proj(const float* proj_dev, void* psi_dev, long long count, float inv_norm, float inv_beta_ratio,
+                     void* stream);
+/* out = re^2 + im^2 (step3_propagation.py:105-110). */
+int mm_abs2(const void* z_dev, float* out_dev, long long count, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Differentiable forward model (not in the reference):
+ *   p = R delta, b = R beta  (ntheta, nz, n),
+ *   psi = exp(i k p - k b), u = D_j psi, I = |u|^2,
+ *   L = sum (I - data)^2  (accumulated in FP64).
+ * Gradient: G = DT_j(4 (I - data) u), dL/dp = -k Im(conj(G) psi),
+ *   dL/db = -k Re(conj(G) psi), dL/ddelta = RT(dL/dp), dL/dbeta = RT(dL/db).
+ * psi and u are recomputed in the backward pass, never stored.
+ * The Fresnel plan's frame (nz, n) must equal (nz_call, radon n).
+ */
+int mm_model_create(mm_model** model, mm_radon* radon, mm_fresnel* fresnel, int nz, float k);
+/* loss_dev: one double on the device. */
+int mm_model_fwd(mm_model* model, const float* delta_dev, const float* beta_dev, const float* data_dev, int j,
+                 double* loss_dev, void* stream);
+/* grad_* (nz, n, n) float32, multiplied by gscale (the upstream gradient). */
+int mm_model_bwd(mm_model* model, const float* delta_dev, const float* beta_dev, const float* data_dev, int j,
+                 float gscale, float* grad_delta_dev, float* grad_beta_dev, void* stream);
+size_t mm_model_workspace_bytes(const mm_model* model);
+void mm_model_destroy(mm_model* model);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOSAIC_HIP_H */

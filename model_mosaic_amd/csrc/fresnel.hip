@@ -1,0 +1,298 @@
+// Fresnel propagation D and its adjoint DT for gfx950.
+//
+// Operator (processing/propagation.py:47-64): pad (symmetric mirror,
+// processing/kernels.py:530-550, or zero) the (nz, n) frame to (2nz, 2n),
+// FFT2, multiply by exp(-i pi lambda z (fx^2 + fy^2)) / (4 n nz), unnormalised
+// IFFT2, centre crop.  The kernel and the padding are separable
+// (processing/propagation_large.py:7-13), so D = Dx o Dy with
+//   Dy = crop_y IFFT_y K_y FFT_y pad_y   (one column pass, both FFTs in LDS)
+//   Dx = crop_x IFFT_x K_x FFT_x pad_x   (one row pass)
+// i.e. two HBM round trips of the complex64 frame instead of the reference's
+// pad / fft2 / multiply / ifft2 / crop passes over a 4x larger buffer.
+// DT = pad^T IFFT conj(K) FFT crop^T per axis (crop^T = zero embedding,
+// pad^T = fold-add of the mirrored borders).
+// The per-axis chirps K_x (2n) and K_y (2nz) are built on the GPU: the phase
+// pi lambda z f^2 is formed and reduced mod 2 pi in FP64, then the FP32 sincos
+// of the reduced phase is taken and 1/(2n) resp. 1/(2nz) is folded in.
+#include <cmath>
+#include <vector>
+
+#include "mm_fft.h"
+
+namespace mm {
+namespace fresnel {
+
+// table[j * L + k] = exp(-i (pi lambda z_j f_k^2 mod 2 pi)) / L, f_k = fftfreq(L, vox)[k]
+__global__ void chirp_table(float2* __restrict__ table, const double* __restrict__ dist, int ndist, int L,
+                            double wavelength, double voxelsize) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ndist * L) return;
+  const int j = idx / L, k = idx - j * L;
+  const double f = (double)(k < L / 2 ? k : k - L) / ((double)L * voxelsize);
+  const double twopi = 6.283185307179586476925286766559;
+  double ph = M_PI * wavelength * dist[j] * f * f;
+  ph = fmod(ph, twopi);
+  float s, c;
+  sincosf((float)ph, &s, &c);
+  const float inv = 1.0f / (float)L;
+  table[idx] = make_float2(c * inv, -s * inv);
+}
+
+// Source row of padded row i for a frame of `len` rows (len/2 on each side).
+__device__ __forceinline__ int pad_src(int i, int len, bool mirror) {
+  const int h = len / 2;
+  if (i < h) return mirror ? h - i - 1 : -1;
+  if (i >= len + h) return mirror ? 2 * len - i + h - 1 : -1;
+  return i - h;
+}
+
+// Column pass: frames (b, nz, n); blockIdx.y = frame, C columns per block.
+// ADJ = false: out = crop IFFT K FFT pad (in);  ADJ = true: out = pad^T IFFT conj(K) FFT crop^T (in).
+template <int L, bool ADJ>
+__global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
+                                                     int nz, bool mirror, const float2* __restrict__ K,
+                                                     const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int col0 = blockIdx.x * Sh::C;
+  const long long fo = (long long)blockIdx.y * nz * n + col0;
+  const float2* src = in + fo;
+  float2* dst = out + fo;
+  const int h = nz / 2;
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it) {
+    const int idx = threadIdx.x + it * kThreads;
+    const int c = idx % Sh::C, i = idx / Sh::C;
+    int sr;
+    if (ADJ)
+      sr = (i >= h && i < h + nz) ? i - h : -1;
+    else
+      sr = pad_src(i, nz, mirror);
+    float2 v = make_float2(0.f, 0.f);
+    if (sr >= 0 && col0 + c < n) v = src[(long long)sr * n + c];
+    lds[c * Sh::S + i] = v;
+  }
+  __syncthreads();
+  const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  float2* x = lds + cc * Sh::S;
+  fft_lds<L, Sh::TPC, false>(x, t, tw);
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int k = t + e * Sh::TPC;
+    float2 kk = K[k];
+    if (ADJ) kk.y = -kk.y;
+    x[k] = cmul(x[k], kk);
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, true>(x, t, tw);
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it) {
+    const int idx = threadIdx.x + it * kThreads;
+    const int c = idx % Sh::C, r = idx / Sh::C;
+    if (r >= nz || col0 + c >= n) continue;
+    float2 v = lds[c * Sh::S + r + h];
+    if (ADJ && mirror) {
+      const int pr = (r < h) ? (h - 1 - r) : (2 * nz + h - 1 - r);
+      v = cadd(v, lds[c * Sh::S + pr]);
+    }
+    dst[(long long)r * n + c] = v;
+  }
+}
+
+// Row pass: rows of length n, C rows per block, nrows = b * nz.
+template <int L, bool ADJ>
+__global__ __launch_bounds__(kThreads) void row_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
+                                                     long long nrows, bool mirror, const float2* __restrict__ K,
+                                                     const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  const long long row = (long long)blockIdx.x * Sh::C + c;
+  const bool valid = row < nrows;
+  const float2* src = in + (valid ? row : 0) * n;
+  float2* x = lds + c * Sh::S;
+  const int h = n / 2;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int i = t + e * Sh::TPC;
+    int sx;
+    if (ADJ)
+      sx = (i >= h && i < h + n) ? i - h : -1;
+    else
+      sx = pad_src(i, n, mirror);
+    x[i] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, false>(x, t, tw);
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int k = t + e * Sh::TPC;
+    float2 kk = K[k];
+    if (ADJ) kk.y = -kk.y;
+    x[k] = cmul(x[k], kk);
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  if (!valid) return;
+  float2* dst = out + row * n;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int r = t + e * Sh::TPC;
+    if (r >= n) continue;
+    float2 v = x[r + h];
+    if (ADJ && mirror) {
+      const int pr = (r < h) ? (h - 1 - r) : (2 * n + h - 1 - r);
+      v = cadd(v, x[pr]);
+    }
+    dst[r] = v;
+  }
+}
+
+}  // namespace fresnel
+}  // namespace mm
+
+using namespace mm;
+using namespace mm::fresnel;
+
+struct mm_fresnel {
+  int dev = 0;
+  int n = 0, nz = 0, ndist = 0, pad_mode = 0;
+  float2* Kx = nullptr;  // ndist x 2n
+  float2* Ky = nullptr;  // ndist x 2nz
+  const float2* tw = nullptr;
+  size_t bytes = 0;
+};
+
+static void fresnel_free(mm_fresnel* p) {
+  if (!p) return;
+  if (p->Kx) (void)hipFree(p->Kx);
+  if (p->Ky) (void)hipFree(p->Ky);
+  delete p;
+}
+
+template <int L, bool ADJ>
+static void launch_col_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
+  using Sh = FftShape<L>;
+  hipLaunchKernelGGL((col_pass<L, ADJ>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
+                     dim3(kThreads), 0, st, in, out, p->n, p->nz, p->pad_mode == 0, p->Ky + (size_t)j * L, p->tw + L);
+}
+
+template <int L, bool ADJ>
+static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const long long rows = (long long)nframes * p->nz;
+  hipLaunchKernelGGL((row_pass<L, ADJ>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, in, out,
+                     p->n, rows, p->pad_mode == 0, p->Kx + (size_t)j * L, p->tw + L);
+}
+
+#define MM_DISPATCH_F(L_RUNTIME, CALL)                     \
+  switch (L_RUNTIME) {                                     \
+    case 16: { constexpr int LL = 16; CALL; } break;       \
+    case 32: { constexpr int LL = 32; CALL; } break;       \
+    case 64: { constexpr int LL = 64; CALL; } break;       \
+    case 128: { constexpr int LL = 128; CALL; } break;     \
+    case 256: { constexpr int LL = 256; CALL; } break;     \
+    case 512: { constexpr int LL = 512; CALL; } break;     \
+    case 1024: { constexpr int LL = 1024; CALL; } break;   \
+    case 2048: { constexpr int LL = 2048; CALL; } break;   \
+    case 4096: { constexpr int LL = 4096; CALL; } break;   \
+    case 8192: { constexpr int LL = 8192; CALL; } break;   \
+    case 16384: { constexpr int LL = 16384; CALL; } break; \
+    default: return fail(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(L_RUNTIME) + " not built"); \
+  }
+
+static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, int j, void* stream, bool adj) {
+  MM_ARG_CHECK(p != nullptr, "mm_fresnel: plan is NULL");
+  MM_ARG_CHECK(nframes >= 0, "mm_fresnel: nframes < 0");
+  MM_ARG_CHECK(j >= 0 && j < p->ndist,
+               "mm_fresnel: distance index " + std::to_string(j) + " outside [0, " + std::to_string(p->ndist) + ")");
+  if (nframes == 0) return MM_OK;
+  MM_ARG_CHECK(in && out, "mm_fresnel: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const float2* src = (const float2*)in;
+  float2* dst = (float2*)out;
+  const int Ly = 2 * p->nz, Lx = 2 * p->n;
+  if (adj) {
+    MM_DISPATCH_F(Ly, (launch_col_pass<LL, true>(p, src, dst, nframes, j, st)));
+  } else {
+    MM_DISPATCH_F(Ly, (launch_col_pass<LL, false>(p, src, dst, nframes, j, st)));
+  }
+  int rc = check_launch("fresnel col_pass");
+  if (rc) return rc;
+  if (adj) {
+    MM_DISPATCH_F(Lx, (launch_row_pass<LL, true>(p, dst, dst, nframes, j, st)));
+  } else {
+    MM_DISPATCH_F(Lx, (launch_row_pass<LL, false>(p, dst, dst, nframes, j, st)));
+  }
+  return check_launch("fresnel row_pass");
+}
+
+namespace mm {
+void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist) {
+  *n = p->n;
+  *nz = p->nz;
+  *ndist = p->ndist;
+}
+}  // namespace mm
+
+extern "C" {
+
+int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavelength, double voxelsize,
+                      const double* dist_host, int pad_mode) {
+  MM_ARG_CHECK(plan != nullptr, "mm_fresnel_create: plan is NULL");
+  *plan = nullptr;
+  MM_ARG_CHECK(is_pow2(n) && n >= 8 && 2 * n <= kMaxFFT,
+               "mm_fresnel_create: n must be a power of two in [8, 8192], got " + std::to_string(n));
+  MM_ARG_CHECK(is_pow2(nz) && nz >= 8 && 2 * nz <= kMaxFFT,
+               "mm_fresnel_create: nz must be a power of two in [8, 8192], got " + std::to_string(nz));
+  MM_ARG_CHECK(ndist >= 1 && dist_host != nullptr, "mm_fresnel_create: need ndist >= 1 distances");
+  MM_ARG_CHECK(pad_mode == 0 || pad_mode == 1, "mm_fresnel_create: pad_mode must be 0 (symmetric) or 1 (zero)");
+  MM_ARG_CHECK(voxelsize > 0 && wavelength >= 0, "mm_fresnel_create: bad wavelength / voxelsize");
+  mm_fresnel* p = new mm_fresnel();
+  auto bail = [&](int code) {
+    fresnel_free(p);
+    return code;
+  };
+  int rc = twiddle_bank(&p->tw);
+  if (rc) return bail(rc);
+  if (hipGetDevice(&p->dev) != hipSuccess) return bail(fail(MM_ERR_HIP, "hipGetDevice"));
+  p->n = n;
+  p->nz = nz;
+  p->ndist = ndist;
+  p->pad_mode = pad_mode;
+  double* ddist = nullptr;
+  if (hipMalloc(&ddist, sizeof(double) * ndist) != hipSuccess) return bail(fail(MM_ERR_OOM, "fresnel distances"));
+  if (hipMalloc(&p->Kx, sizeof(float2) * (size_t)ndist * 2 * n) != hipSuccess ||
+      hipMalloc(&p->Ky, sizeof(float2) * (size_t)ndist * 2 * nz) != hipSuccess) {
+    (void)hipFree(ddist);
+    return bail(fail(MM_ERR_OOM, "fresnel chirp tables"));
+  }
+  p->bytes = sizeof(float2) * (size_t)ndist * 2 * (n + nz);
+  hipError_t e = hipMemcpy(ddist, dist_host, sizeof(double) * ndist, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(chirp_table, dim3((ndist * 2 * n + 255) / 256), dim3(256), 0, 0, p->Kx, ddist, ndist, 2 * n,
+                       wavelength, voxelsize);
+    hipLaunchKernelGGL(chirp_table, dim3((ndist * 2 * nz + 255) / 256), dim3(256), 0, 0, p->Ky, ddist, ndist, 2 * nz,
+                       wavelength, voxelsize);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
+  (void)hipFree(ddist);
+  if (e != hipSuccess) return bail(fail(MM_ERR_HIP, std::string("fresnel chirp build: ") + hipGetErrorString(e)));
+  *plan = p;
+  return MM_OK;
+}
+
+int mm_fresnel_fwd(mm_fresnel* p, const void* psi, void* out, int nframes, int j, void* stream) {
+  return fresnel_apply(p, psi, out, nframes, j, stream, false);
+}
+
+int mm_fresnel_adj(mm_fresnel* p, const void* in, void* out, int nframes, int j, void* stream) {
+  return fresnel_apply(p, in, out, nframes, j, stream, true);
+}
+
+size_t mm_fresnel_workspace_bytes(const mm_fresnel* p) { return p ? p->bytes : 0; }
+
+void mm_fresnel_destroy(mm_fresnel* p) { fresnel_free(p); }
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Shared helpers for the gfx950 USFFT Radon / Fresnel library: status codes,
+// the thread-local error message, complex arithmetic and the twiddle bank.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/mosaic_hip.h"
+
+namespace mm {
+
+// ---- error state -----------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define MM_HIP_TRY(expr)                                                             \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return ::mm::fail(MM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define MM_ARG_CHECK(cond, msg) \
+  do {                          \
+    if (!(cond)) return ::mm::fail(MM_ERR_ARG, msg); \
+  } while (0)
+
+// Checks the launch that was just issued.
+int check_launch(const char* what);
+
+// ---- complex helpers -------------------------------------------------------
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__host__ __device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---- twiddle bank ----------------------------------------------------------
+// For every power of two L in [2, kMaxFFT] the bank holds exp(-2 pi i t / L),
+// t in [0, L), at element offset L (computed in FP64 on the host).
+constexpr int kMaxFFT = 16384;
+// Returns the device pointer of the bank on the current device (built on first use).
+int twiddle_bank(const float2** out);
+
+// Plan dimensions for the modules that compose plans (model.hip).
+void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max);
+void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);
+
+inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+inline int ilog2(long v) { int r = 0; while ((1L << r) < v) ++r; return r; }
+
+}  // namespace mm

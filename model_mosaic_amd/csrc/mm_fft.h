@@ -1,0 +1,132 @@
+// LDS-resident Stockham FFT for gfx950.
+//
+// A workgroup of kThreads = 1024 threads holds C = kBatchElems / L transforms of
+// length L in LDS (column-major: transform c occupies x[c*S .. c*S + L), S = L + 1
+// so that the C transforms start in different banks).  TPC = kThreads / C
+// threads work on one transform; every thread owns E = L / TPC = 16 points per
+// stage.  Stages are radix-8 (radix-4/2 for the leftover factor), each one a
+// read-all / barrier / twiddle + DFT_R in registers / write / barrier sweep
+// (Stockham autosort: natural order in and out, no bit reversal).  Twiddles come
+// from the FP64-built bank exp(-2 pi i t / L), so a length-L transform has the
+// accuracy of its radix arithmetic only.
+#pragma once
+
+#include "mm_common.h"
+
+namespace mm {
+
+constexpr int kThreads = 1024;      // threads per FFT workgroup
+constexpr int kBatchElems = 16384;  // complex points per FFT workgroup (128 KiB of LDS)
+constexpr int kPerThread = 16;      // points per thread per stage
+
+template <int L>
+struct FftShape {
+  static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
+  static constexpr int C = kBatchElems / L;  // transforms per workgroup
+  static constexpr int TPC = L / kPerThread; // threads per transform
+  static constexpr int S = L + 1;            // LDS stride between transforms
+  static constexpr int LDS = C * S;          // float2 elements of LDS
+  static_assert(C * TPC == kThreads, "shape");
+};
+
+constexpr int pick_radix(int rem) { return (rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2); }
+
+template <bool INV>
+__host__ __device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__host__ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_mi<INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <bool INV>
+__host__ __device__ __forceinline__ void dft8(float2* a) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const float c = 0.70710678118654752440f;
+  // W8^k o_k (forward W8 = exp(-i pi/4), inverse its conjugate)
+  float2 w1 = INV ? make_float2(c * (o1.x - o1.y), c * (o1.x + o1.y)) : make_float2(c * (o1.x + o1.y), c * (o1.y - o1.x));
+  float2 w2 = mul_mi<INV>(o2);
+  float2 w3 = INV ? make_float2(-c * (o3.x + o3.y), c * (o3.x - o3.y)) : make_float2(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+  a[0] = cadd(e0, o0);
+  a[4] = csub(e0, o0);
+  a[1] = cadd(e1, w1);
+  a[5] = csub(e1, w1);
+  a[2] = cadd(e2, w2);
+  a[6] = csub(e2, w2);
+  a[3] = cadd(e3, w3);
+  a[7] = csub(e3, w3);
+}
+
+template <int R, bool INV>
+__host__ __device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 2) {
+    dft2<INV>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+  } else {
+    dft8<INV>(v);
+  }
+}
+
+// One Stockham radix-R stage over a length-L transform at x (LDS), NS = product
+// of the radices already applied.  Thread t in [0, TPC) handles butterflies
+// j = t + b*TPC.
+template <int L, int R, int NS, int TPC, bool INV>
+__device__ __forceinline__ void fft_stage(float2* __restrict__ x, int t, const float2* __restrict__ tw) {
+  constexpr int NB = L / R;
+  constexpr int BPT = NB / TPC;
+  static_assert(BPT >= 1 && NB % TPC == 0, "butterflies per thread");
+  float2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPC;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = x[j + r * NB];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPC;
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+      constexpr int STEP = L / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[k * r * STEP];
+        if (INV) w.y = -w.y;
+        v[b][r] = cmul(v[b][r], w);
+      }
+    }
+    dft<R, INV>(v[b]);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[base + r * NS] = v[b][r];
+  }
+  __syncthreads();
+}
+
+// Full in-place transform of x[0..L) by the TPC threads with index t.
+// Must be reached by every thread of the workgroup (it contains barriers).
+// Caller synchronises before the first stage reads x.  Unnormalised in both
+// directions (INV only conjugates the kernel).
+template <int L, int TPC, bool INV, int NS = 1>
+__device__ __forceinline__ void fft_lds(float2* __restrict__ x, int t, const float2* __restrict__ tw) {
+  if constexpr (NS < L) {
+    constexpr int R = pick_radix(L / NS);
+    fft_stage<L, R, NS, TPC, INV>(x, t, tw);
+    fft_lds<L, TPC, INV, NS * R>(x, t, tw);
+  }
+}
+
+}  // namespace mm

@@ -1,0 +1,1098 @@
+// USFFT Radon transform R and its adjoint RT for gfx950.
+//
+// Operator (processing/tomo.py:121-188, processing/kernels.py:16-93,274-349):
+//   R x  = Re( c1d . IFFT_n( c1d . G( FFT2_2n( pad(phi . x) ) ) ) ) / 4
+//   RT y = Re or complex( phi . crop( IFFT2_2n( G^T( c1d . FFT_n(c1d . y) ) ) ) )
+// G samples the 2n x 2n Fourier grid at the polar points
+//   X = 2n x0, Y = 2n y0,  x0 = (r - n/2)/n cos(theta),  y0 = -(r - n/2)/n sin(theta)
+// with the (2m+1)^2-tap Gaussian window (pi/mu) exp(-pi^2/mu |xi - l/2n|^2), m = 4.
+//
+// Design (all FP32 arithmetic except the FP64 polar coordinates):
+//  * Real input => Hermitian spectrum.  Only the half plane l0 in [0, n] of the
+//    rfft2 is ever stored (fde: nz x 2n x FS complex, raw fftfreq rows).
+//  * The samples r and n - r of one angle sit at mirrored frequencies, so for
+//    real data g(n - r) = conj(g(r)).  Each angle is therefore gathered at the
+//    H = n/2 + 1 "representative" samples of its half line in the right half
+//    plane (X >= 0); the other half of the line is their conjugate.  The
+//    unpaired r = 0 sample is represented by its mirror.
+//  * The adjoint scatter folds the Hermitian part of the full grid into the
+//    half plane: a contribution c at tap k lands at k when k0 in [0, n] and as
+//    conj(c) at -k when k0 in {0} U [n, 2n); the self-mirrored (r = n/2) and
+//    unpaired (r = 0) samples carry weight 1/2.  A C2R row transform then
+//    returns Re(IFFT2(full grid)), which is what the reference keeps.
+//    A complex sinogram is split into the two Hermitian channels of its real
+//    and imaginary parts (one complex row FFT gives both).
+//  * Gather and scatter are tiled over the Fourier plane: samples are binned
+//    (host, FP64, at plan time) into quadtree tiles of at most 1024 samples.
+//    A gather workgroup stages its tile + 4-point halo of the half spectrum in
+//    LDS (applying the conj reflection and the (-1)^(k0+k1) centring sign while
+//    loading) and each thread sums its 81 taps from LDS.  A scatter workgroup
+//    owns a tile of the output half plane, accumulates the taps of every
+//    sample that touches it with LDS atomics (FP32, or FP64 with
+//    MM_RADON_FP64_TILES) and writes the tile once; only tiles at the 8 x 8
+//    quadtree floor with more than 1024 samples (around the origin, where
+//    every angle passes) are split into partial tiles that a second kernel
+//    sums.  Per-sample weights stay in registers across the z loop.
+//  * FFTs are the LDS Stockham transforms of mm_fft.h: row transforms over
+//    contiguous rows, column transforms over C = 16384/L adjacent columns.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mm_fft.h"
+
+namespace mm {
+namespace radon {
+
+constexpr int M = 4;             // window half width (m of processing/tomo.py:41 for every n)
+constexpr int TAPS = 2 * M + 1;  // 9
+constexpr int ROOT = 64;         // quadtree root tile
+constexpr int FLOOR = 8;         // quadtree floor tile
+constexpr int TILE_THREADS = 1024;
+constexpr int SMAX = TILE_THREADS;  // samples per tile work item
+constexpr int PATCH = ROOT + 2 * M;  // gather LDS patch edge
+
+struct Item {
+  int x0, y0;        // tile origin (l0, l1), centred coordinates
+  int w, h;          // tile extent
+  int start, count;  // samples[start, start + count)
+  int slot;          // scatter: -1 = own the tile, >= 0 = partial slot
+  int pad;
+};
+
+struct SplitTile {
+  int x0, y0, w, h, slot0, nslots, pad0, pad1;
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Row pass of R: two object rows per complex FFT.  Row y of slice z, times
+// phi_y[y] phi_x[x], zero-padded to 2n (object centred), is transformed and
+// the half spectrum kx in [0, n] is written to fde row n/2 + y.
+template <int L>
+__global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
+                                                         const float* __restrict__ phx, const float* __restrict__ phy,
+                                                         int n, int nz, int FS, long long fzs,
+                                                         const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  const int half = n / 2;
+  const long long pair = (long long)blockIdx.x * Sh::C + c;
+  const bool valid = pair < (long long)nz * half;
+  int z = 0, y0 = 0;
+  if (valid) {
+    z = (int)(pair / half);
+    y0 = 2 * (int)(pair - (long long)z * half);
+  }
+  float2* x = lds + c * Sh::S;
+  const float* ra = obj + ((long long)z * n + y0) * n;
+  const float* rb = ra + n;
+  const float pya = valid ? phy[y0] : 0.f, pyb = valid ? phy[y0 + 1] : 0.f;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int i = t + e * Sh::TPC;
+    const int xi = i - half;
+    float2 v = make_float2(0.f, 0.f);
+    if (valid && xi >= 0 && xi < n) {
+      const float p = phx[xi];
+      v = make_float2(ra[xi] * pya * p, rb[xi] * pyb * p);
+    }
+    x[i] = v;
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, false>(x, t, tw);
+  if (valid) {
+    float2* oa = fde + (long long)z * fzs + (long long)(half + y0) * FS;
+    float2* ob = oa + FS;
+    for (int k = t; k <= n; k += Sh::TPC) {
+      const float2 zk = x[k], zm = x[(L - k) & (L - 1)];
+      oa[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      ob[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+  }
+}
+
+// Column transform over C adjacent columns of a (rows x FS) complex plane,
+// z-plane = blockIdx.y.  Rows [in_lo, in_hi) are read (others are zero), rows
+// [out_lo, out_hi) are written back in place, times `scale`.
+template <int L, bool INV>
+__global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, int ncols, int FS, long long zs,
+                                                    int in_lo, int in_hi, int out_lo, int out_hi, float scale,
+                                                    const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int col0 = blockIdx.x * Sh::C;
+  float2* b = buf + (long long)blockIdx.y * zs + col0;
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it) {
+    const int idx = threadIdx.x + it * kThreads;
+    const int c = idx % Sh::C, i = idx / Sh::C;
+    float2 v = make_float2(0.f, 0.f);
+    if (i >= in_lo && i < in_hi && col0 + c < ncols) v = b[(long long)i * FS + c];
+    lds[c * Sh::S + i] = v;
+  }
+  __syncthreads();
+  const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  fft_lds<L, Sh::TPC, INV>(lds + cc * Sh::S, t, tw);
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it) {
+    const int idx = threadIdx.x + it * kThreads;
+    const int c = idx % Sh::C, i = idx / Sh::C;
+    if (i >= out_lo && i < out_hi && col0 + c < ncols) b[(long long)i * FS + c] = cscale(lds[c * Sh::S + i], scale);
+  }
+}
+
+// Polar coordinates of representative sample id = theta * H + k: X = 2k|cos|,
+// Y = -2k sigma sin (FP64), split into the nearest grid point and the FP32
+// fractional offset.
+struct SampleGeom {
+  int th, k, rx, ry;
+  float fx, fy;
+};
+
+__device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* __restrict__ dirs) {
+  SampleGeom g;
+  g.th = id / H;
+  g.k = id - g.th * H;
+  const double2 d = dirs[g.th];
+  const double X = (2.0 * g.k) * d.x, Y = (2.0 * g.k) * d.y;
+  const double rx = rint(X), ry = rint(Y);
+  g.rx = (int)rx;
+  g.ry = (int)ry;
+  g.fx = (float)(X - rx);
+  g.fy = (float)(Y - ry);
+  return g;
+}
+
+// Tiled polar gather (forward R): gh[theta, z, k] = C0 sum ey sum ex F_c.
+__global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh,
+                                                            const Item* __restrict__ items,
+                                                            const int* __restrict__ samples,
+                                                            const double2* __restrict__ dirs, int n, int H, int nz,
+                                                            int FS, long long fzs, int zchunk, float a, float C0) {
+  __shared__ float2 P[PATCH * PATCH];
+  const Item it = items[blockIdx.x];
+  const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
+  const int pw = it.w + 2 * M, ph = it.h + 2 * M, np = pw * ph;
+  const int px0 = it.x0 - M, py0 = it.y0 - M;
+  const int s = threadIdx.x;
+  const bool active = s < it.count;
+  float ex[TAPS], ey[TAPS];
+  int base = 0, th = 0, k = 0;
+  if (active) {
+    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
+    th = g.th;
+    k = g.k;
+    base = (g.ry - it.y0) * pw + (g.rx - it.x0);
+#pragma unroll
+    for (int i = 0; i < TAPS; ++i) {
+      const float dx = (float)(i - M) - g.fx, dy = (float)(i - M) - g.fy;
+      ex[i] = __expf(-a * dx * dx);
+      ey[i] = C0 * __expf(-a * dy * dy);
+    }
+  }
+  const int mask = 2 * n - 1;
+  for (int z = z0; z < z1; ++z) {
+    const float2* f = fde + (long long)z * fzs;
+    for (int idx = threadIdx.x; idx < np; idx += TILE_THREADS) {
+      const int pr = idx / pw, pc = idx - pr * pw;
+      const int k0 = (px0 + pc) & mask, k1 = (py0 + pr) & mask;
+      float2 v;
+      if (k0 <= n) {
+        v = f[(long long)k1 * FS + k0];
+      } else {
+        v = f[(long long)((2 * n - k1) & mask) * FS + (2 * n - k0)];
+        v.y = -v.y;
+      }
+      if ((k0 + k1) & 1) v = make_float2(-v.x, -v.y);
+      P[idx] = v;
+    }
+    __syncthreads();
+    if (active) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int i1 = 0; i1 < TAPS; ++i1) {
+        const float2* pp = P + base + i1 * pw;
+        float2 row = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i0 = 0; i0 < TAPS; ++i0) {
+          const float2 v = pp[i0];
+          row.x = fmaf(ex[i0], v.x, row.x);
+          row.y = fmaf(ex[i0], v.y, row.y);
+        }
+        acc.x = fmaf(ey[i1], row.x, acc.x);
+        acc.y = fmaf(ey[i1], row.y, acc.y);
+      }
+      gh[((long long)th * nz + z) * H + k] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Line pass of R: rebuild the full line g[r] from its representatives,
+// sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).
+template <int L>
+__global__ __launch_bounds__(kThreads) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
+                                                         const int* __restrict__ sig, int H, long long nrows, int nz,
+                                                         float scale, const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  const long long row = (long long)blockIdx.x * Sh::C + c;
+  const bool valid = row < nrows;
+  const int sg = valid ? sig[row / nz] : 1;
+  const float2* src = gh + (valid ? row : 0) * H;
+  float2* x = lds + c * Sh::S;
+  constexpr int half = L / 2;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int r = t + e * Sh::TPC;
+    const int d = r - half;
+    int kk;
+    bool cj;
+    if (sg > 0) {
+      kk = d >= 0 ? d : -d;
+      cj = d < 0;
+    } else {
+      kk = d <= 0 ? -d : d;
+      cj = d > 0;
+    }
+    float2 v = valid ? src[kk] : make_float2(0.f, 0.f);
+    if (cj) v.y = -v.y;
+    if (!(r & 1)) v = make_float2(-v.x, -v.y);
+    x[r] = v;
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  if (valid) {
+    float* dst = sino + row * L;
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+      const int s = t + e * Sh::TPC;
+      const float v = x[s].x * scale;
+      dst[s] = (s & 1) ? v : -v;
+    }
+  }
+}
+
+// Line pass of RT: yhat = c1d . FFT(c1d . y), written at the representatives
+// with the fold weights (1/2 at k = 0 and k = n/2) times `scale`.  A complex
+// sinogram yields the two channels of its real and imaginary parts.
+template <int L, bool CPLX>
+__global__ __launch_bounds__(kThreads) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
+                                                         float2* __restrict__ yim, const int* __restrict__ sig, int H,
+                                                         long long nrows, int nz, float scale,
+                                                         const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  const long long row = (long long)blockIdx.x * Sh::C + c;
+  const bool valid = row < nrows;
+  const long long rr = valid ? row : 0;
+  const int sg = valid ? sig[rr / nz] : 1;
+  float2* x = lds + c * Sh::S;
+  constexpr int half = L / 2;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int r = t + e * Sh::TPC;
+    float2 v;
+    if (CPLX)
+      v = valid ? ((const float2*)sino_v)[rr * L + r] : make_float2(0.f, 0.f);
+    else
+      v = make_float2(valid ? ((const float*)sino_v)[rr * L + r] : 0.f, 0.f);
+    if (!(r & 1)) v = make_float2(-v.x, -v.y);
+    x[r] = v;
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, false>(x, t, tw);
+  if (!valid) return;  // no barrier below
+  auto yhat = [&](int r) {
+    float2 v = x[r];
+    return (r & 1) ? v : make_float2(-v.x, -v.y);
+  };
+  for (int k = t; k < H; k += Sh::TPC) {
+    const float f = ((k == 0 || k == half) ? 0.5f : 1.0f) * scale;
+    if (!CPLX) {
+      float2 v;
+      if (sg > 0)
+        v = (k < half) ? yhat(half + k) : cconj(yhat(0));
+      else
+        v = yhat(half - k);
+      yre[rr * H + k] = cscale(v, f);
+    } else {
+      float2 vr, vi;
+      if (k == half) {
+        const float2 v0 = yhat(0);
+        if (sg > 0) {
+          vr = cconj(v0);
+          vi = make_float2(v0.y, v0.x);  // conj(-i v0)
+        } else {
+          vr = v0;
+          vi = make_float2(v0.y, -v0.x);  // -i v0
+        }
+      } else {
+        const int r = sg > 0 ? half + k : half - k;
+        const float2 p = yhat(r), q = yhat((L - r) & (L - 1));
+        vr = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+        vi = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      }
+      yre[rr * H + k] = cscale(vr, f);
+      yim[rr * H + k] = cscale(vi, f);
+    }
+  }
+}
+
+template <typename Acc>
+__device__ __forceinline__ void lds_add(Acc* T, int idx, float vx, float vy) {
+  atomicAdd(&T[2 * idx], (Acc)vx);
+  atomicAdd(&T[2 * idx + 1], (Acc)vy);
+}
+
+// Tiled adjoint scatter with Hermitian fold into the half plane (see header).
+template <typename Acc>
+__global__ __launch_bounds__(TILE_THREADS) void scatter_tiles(const float2* __restrict__ yh, float2* __restrict__ Hb,
+                                                             float2* __restrict__ partial,
+                                                             const Item* __restrict__ items,
+                                                             const int* __restrict__ samples,
+                                                             const double2* __restrict__ dirs, int n, int H, int nz,
+                                                             int FS, long long hzs, int zchunk, float a) {
+  __shared__ Acc T[2 * ROOT * ROOT];
+  const Item it = items[blockIdx.x];
+  const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
+  const int tw_ = it.w, tsz = it.w * it.h;
+  const int s = threadIdx.x;
+  const bool active = s < it.count;
+  const int twon = 2 * n, mask = twon - 1;
+  float ex[TAPS], ey[TAPS];
+  unsigned colD = 0, colR = 0, okD1 = 0, okR1 = 0;
+  int bx = 0, by = 0, th = 0, k = 0;
+  if (active) {
+    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
+    th = g.th;
+    k = g.k;
+    bx = g.rx - M;
+    by = g.ry - M;
+#pragma unroll
+    for (int i = 0; i < TAPS; ++i) {
+      const float dx = (float)(i - M) - g.fx, dy = (float)(i - M) - g.fy;
+      const float sx = ((bx + i) & 1) ? -1.f : 1.f, sy = ((by + i) & 1) ? -1.f : 1.f;
+      ex[i] = sx * __expf(-a * dx * dx);
+      ey[i] = sy * __expf(-a * dy * dy);
+      // direct target: l0 = bx + i in [0, n] inside the tile
+      const int l0 = bx + i;
+      if (l0 >= 0 && l0 <= n && l0 >= it.x0 && l0 < it.x0 + it.w) colD |= 1u << i;
+      const int wr = ((by + i + n) & mask) - n;
+      if (wr >= it.y0 && wr < it.y0 + it.h) okD1 |= 1u << i;
+      // folded target of k0 in {0} U [n, 2n): column -l0 (l0 <= 0) or 2n - l0 (l0 >= n)
+      if (l0 <= 0 || l0 >= n) {
+        const int c2 = (l0 <= 0) ? -l0 : twon - l0;
+        if (c2 >= it.x0 && c2 < it.x0 + it.w) colR |= 1u << i;
+      }
+      const int wr2 = ((n - (by + i)) & mask) - n;
+      if (wr2 >= it.y0 && wr2 < it.y0 + it.h) okR1 |= 1u << i;
+    }
+  }
+  for (int idx = threadIdx.x; idx < tsz; idx += TILE_THREADS) {
+    T[2 * idx] = (Acc)0;
+    T[2 * idx + 1] = (Acc)0;
+  }
+  __syncthreads();
+  const bool anyD = (colD != 0) && (okD1 != 0);
+  const bool anyR = (colR != 0) && (okR1 != 0);
+  const int cD0 = bx - it.x0;
+  for (int z = z0; z < z1; ++z) {
+    if (active && (anyD || anyR)) {
+      const float2 v = yh[((long long)th * nz + z) * H + k];
+      if (anyD) {
+#pragma unroll
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          if (!((okD1 >> i1) & 1u)) continue;
+          const float wx = ey[i1] * v.x, wy = ey[i1] * v.y;
+          const int rb = ((((by + i1 + n) & mask) - n) - it.y0) * tw_ + cD0;
+#pragma unroll
+          for (int i0 = 0; i0 < TAPS; ++i0)
+            if ((colD >> i0) & 1u) lds_add<Acc>(T, rb + i0, ex[i0] * wx, ex[i0] * wy);
+        }
+      }
+      if (anyR) {
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          if (!((okR1 >> i1) & 1u)) continue;
+          const float wx = ey[i1] * v.x, wy = -ey[i1] * v.y;
+          const int rb = ((((n - (by + i1)) & mask) - n) - it.y0) * tw_ - it.x0;
+#pragma unroll
+          for (int i0 = 0; i0 < TAPS; ++i0) {
+            if (!((colR >> i0) & 1u)) continue;
+            const int l0 = bx + i0;
+            const int c2 = (l0 <= 0) ? -l0 : twon - l0;
+            lds_add<Acc>(T, rb + c2, ex[i0] * wx, ex[i0] * wy);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < tsz; idx += TILE_THREADS) {
+      const int lr = idx / tw_, lc = idx - lr * tw_;
+      const int l0 = it.x0 + lc;
+      const float2 val = make_float2((float)T[2 * idx], (float)T[2 * idx + 1]);
+      T[2 * idx] = (Acc)0;
+      T[2 * idx + 1] = (Acc)0;
+      if (l0 > n) continue;
+      if (it.slot < 0) {
+        const int k1 = (it.y0 + lr) & mask;
+        Hb[(long long)z * hzs + (long long)k1 * FS + l0] = val;
+      } else {
+        partial[((long long)it.slot * nz + z) * (FLOOR * FLOOR) + idx] = val;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Sum of the partial tiles of the split floor tiles.
+__global__ __launch_bounds__(FLOOR* FLOOR) void scatter_reduce(const float2* __restrict__ partial,
+                                                               float2* __restrict__ Hb,
+                                                               const SplitTile* __restrict__ splits, int n, int nz,
+                                                               int FS, long long hzs) {
+  const SplitTile st = splits[blockIdx.x];
+  const int z = blockIdx.y;
+  const int e = threadIdx.x;
+  const int lr = e / st.w, lc = e - lr * st.w;
+  const int l0 = st.x0 + lc;
+  if (lr >= st.h || l0 > n) return;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int sl = st.slot0; sl < st.slot0 + st.nslots; ++sl) {
+    const float2 v = partial[((long long)sl * nz + z) * (FLOOR * FLOOR) + e];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  const int k1 = (st.y0 + lr) & (2 * n - 1);
+  Hb[(long long)z * hzs + (long long)k1 * FS + l0] = acc;
+}
+
+// Row pass of RT: C2R inverse transform of the half-spectrum rows n/2 + y
+// (after the column IFFT), centre crop, times phi.  Real output packs two rows
+// per complex FFT; complex output packs the real/imaginary channels of one row.
+template <int L, bool CPLX>
+__global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restrict__ Hre,
+                                                         const float2* __restrict__ Him, void* __restrict__ out_v,
+                                                         const float* __restrict__ phx, const float* __restrict__ phy,
+                                                         int n, int nz, int FS, long long hzs, float scale,
+                                                         const float2* __restrict__ tw) {
+  using Sh = FftShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  const int half = n / 2;
+  const long long unit = (long long)blockIdx.x * Sh::C + c;
+  const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
+  const bool valid = unit < nunits;
+  int z = 0, y = 0;
+  if (valid) {
+    if (CPLX) {
+      z = (int)(unit / n);
+      y = (int)(unit - (long long)z * n);
+    } else {
+      z = (int)(unit / half);
+      y = 2 * (int)(unit - (long long)z * half);
+    }
+  }
+  const float2* A = Hre + (long long)z * hzs + (long long)(half + y) * FS;
+  const float2* B = CPLX ? (Him + (long long)z * hzs + (long long)(half + y) * FS) : (A + FS);
+  float2* x = lds + c * Sh::S;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int kx = t + e * Sh::TPC;
+    float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+    if (valid) {
+      if (kx <= n) {
+        a = A[kx];
+        b = B[kx];
+        if (kx == 0 || kx == n) {
+          a.y = 0.f;
+          b.y = 0.f;
+        }
+      } else {
+        a = cconj(A[L - kx]);
+        b = cconj(B[L - kx]);
+      }
+    }
+    x[kx] = make_float2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  if (!valid) return;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int xx = t + e * Sh::TPC;
+    if (xx >= n) continue;
+    const float2 v = cscale(x[xx + half], scale);
+    const float px = phx[xx];
+    if (CPLX) {
+      const float p = phy[y] * px;
+      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(v.x * p, v.y * p);
+    } else {
+      float* o = (float*)out_v + ((long long)z * n + y) * n;
+      o[xx] = v.x * phy[y] * px;
+      o[n + xx] = v.y * phy[y + 1] * px;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: geometry, binning, launches
+// ---------------------------------------------------------------------------
+
+struct Rect {
+  int c0, c1, r0, r1;  // inclusive, centred coordinates inside the half plane domain
+};
+
+static inline int wrapc(int v, int n) { return ((v + n) & (2 * n - 1)) - n; }
+
+// Splits rows [r0, r1] (centred, may leave [-n, n)) into wrapped pieces.
+static void push_wrapped(std::vector<Rect>& out, int c0, int c1, int r0, int r1, int n) {
+  if (c0 > c1) return;
+  if (r0 < -n) {
+    if (r1 < -n) {
+      out.push_back({c0, c1, r0 + 2 * n, r1 + 2 * n});
+    } else {
+      out.push_back({c0, c1, r0 + 2 * n, n - 1});
+      out.push_back({c0, c1, -n, r1});
+    }
+  } else if (r1 >= n) {
+    if (r0 >= n) {
+      out.push_back({c0, c1, r0 - 2 * n, r1 - 2 * n});
+    } else {
+      out.push_back({c0, c1, r0, n - 1});
+      out.push_back({c0, c1, -n, r1 - 2 * n});
+    }
+  } else {
+    out.push_back({c0, c1, r0, r1});
+  }
+}
+
+struct HostGeom {
+  int n, H, ntheta;
+  std::vector<double> dx, dy;  // per theta: |cos|, -sigma sin
+  void centre(int id, int& rx, int& ry) const {
+    const int th = id / H, k = id - th * H;
+    const double X = (2.0 * k) * dx[th], Y = (2.0 * k) * dy[th];
+    rx = (int)std::rint(X);
+    ry = (int)std::rint(Y);
+  }
+  // Output-plane rectangles touched by the taps of sample id (direct + folded).
+  void rects(int id, std::vector<Rect>& out) const {
+    out.clear();
+    int rx, ry;
+    centre(id, rx, ry);
+    const int bx = rx - M, by = ry - M;
+    push_wrapped(out, std::max(bx, 0), std::min(bx + 2 * M, n), by, by + 2 * M, n);
+    if (bx <= 0) push_wrapped(out, 0, -bx, -(by + 2 * M), -by, n);
+    if (bx + 2 * M >= n) push_wrapped(out, 2 * n - (bx + 2 * M), n, -(by + 2 * M), -by, n);
+  }
+};
+
+struct Binning {
+  std::vector<Item> items;
+  std::vector<int> samples;
+  std::vector<SplitTile> splits;
+  int nslots = 0;
+};
+
+static bool rect_hits(const Rect& r, int x0, int y0, int w, int h) {
+  return r.c0 <= x0 + w - 1 && r.c1 >= x0 && r.r0 <= y0 + h - 1 && r.r1 >= y0;
+}
+
+// Gather: samples binned by their centre.
+static void gather_node(const HostGeom& G, int x0, int y0, int sz, std::vector<int>& ids, Binning& B) {
+  if (ids.empty()) return;
+  if ((int)ids.size() <= SMAX || sz <= FLOOR) {
+    for (size_t s = 0; s < ids.size(); s += SMAX) {
+      Item it{};
+      it.x0 = x0;
+      it.y0 = y0;
+      it.w = sz;
+      it.h = sz;
+      it.start = (int)B.samples.size();
+      it.count = (int)std::min<size_t>(SMAX, ids.size() - s);
+      it.slot = -1;
+      B.samples.insert(B.samples.end(), ids.begin() + s, ids.begin() + s + it.count);
+      B.items.push_back(it);
+    }
+    return;
+  }
+  const int h = sz / 2;
+  std::vector<int> q[4];
+  for (int id : ids) {
+    int rx, ry;
+    G.centre(id, rx, ry);
+    const int qx = (rx - x0) >= h, qy = (ry - y0) >= h;
+    q[qy * 2 + qx].push_back(id);
+  }
+  std::vector<int>().swap(ids);
+  for (int c = 0; c < 4; ++c) gather_node(G, x0 + (c & 1) * h, y0 + (c >> 1) * h, h, q[c], B);
+}
+
+static void build_gather(const HostGeom& G, int root, Binning& B) {
+  const int n = G.n;
+  const int nrx = n / root + 1, nry = 2 * n / root + 1;
+  std::vector<std::vector<int>> roots((size_t)nrx * nry);
+  const int total = G.ntheta * G.H;
+  for (int id = 0; id < total; ++id) {
+    int rx, ry;
+    G.centre(id, rx, ry);
+    roots[(size_t)((ry + n) / root) * nrx + rx / root].push_back(id);
+  }
+  for (int ty = 0; ty < nry; ++ty)
+    for (int tx = 0; tx < nrx; ++tx) gather_node(G, tx * root, ty * root - n, root, roots[(size_t)ty * nrx + tx], B);
+}
+
+// Scatter: samples binned by every output tile their taps touch.
+static void scatter_node(const HostGeom& G, int x0, int y0, int sz, std::vector<int>& ids, Binning& B,
+                         std::vector<Rect>& tmp) {
+  if (x0 > G.n) return;  // entirely right of the half plane
+  if ((int)ids.size() <= SMAX || sz <= FLOOR) {
+    if ((int)ids.size() <= SMAX) {
+      Item it{};
+      it.x0 = x0;
+      it.y0 = y0;
+      it.w = sz;
+      it.h = sz;
+      it.start = (int)B.samples.size();
+      it.count = (int)ids.size();
+      it.slot = -1;
+      B.samples.insert(B.samples.end(), ids.begin(), ids.end());
+      B.items.push_back(it);
+    } else {
+      SplitTile st{};
+      st.x0 = x0;
+      st.y0 = y0;
+      st.w = sz;
+      st.h = sz;
+      st.slot0 = B.nslots;
+      st.nslots = (int)((ids.size() + SMAX - 1) / SMAX);
+      for (int p = 0; p < st.nslots; ++p) {
+        Item it{};
+        it.x0 = x0;
+        it.y0 = y0;
+        it.w = sz;
+        it.h = sz;
+        it.start = (int)B.samples.size();
+        it.count = (int)std::min<size_t>(SMAX, ids.size() - (size_t)p * SMAX);
+        it.slot = B.nslots + p;
+        B.samples.insert(B.samples.end(), ids.begin() + (size_t)p * SMAX, ids.begin() + (size_t)p * SMAX + it.count);
+        B.items.push_back(it);
+      }
+      B.nslots += st.nslots;
+      B.splits.push_back(st);
+    }
+    return;
+  }
+  const int h = sz / 2;
+  for (int c = 0; c < 4; ++c) {
+    const int cx = x0 + (c & 1) * h, cy = y0 + (c >> 1) * h;
+    if (cx > G.n) continue;
+    std::vector<int> sub;
+    for (int id : ids) {
+      G.rects(id, tmp);
+      for (const Rect& r : tmp)
+        if (rect_hits(r, cx, cy, h, h)) {
+          sub.push_back(id);
+          break;
+        }
+    }
+    scatter_node(G, cx, cy, h, sub, B, tmp);
+  }
+}
+
+static void build_scatter(const HostGeom& G, int root, Binning& B) {
+  const int n = G.n;
+  const int nrx = n / root + 1, nry = 2 * n / root;
+  std::vector<std::vector<int>> roots((size_t)nrx * nry);
+  std::vector<Rect> tmp;
+  std::vector<int> hit;
+  const int total = G.ntheta * G.H;
+  for (int id = 0; id < total; ++id) {
+    G.rects(id, tmp);
+    hit.clear();
+    for (const Rect& r : tmp)
+      for (int ty = (r.r0 + n) / root; ty <= (r.r1 + n) / root; ++ty)
+        for (int tx = r.c0 / root; tx <= r.c1 / root; ++tx) hit.push_back(ty * nrx + tx);
+    std::sort(hit.begin(), hit.end());
+    hit.erase(std::unique(hit.begin(), hit.end()), hit.end());
+    for (int h : hit) roots[h].push_back(id);
+  }
+  for (int ty = 0; ty < nry; ++ty)
+    for (int tx = 0; tx < nrx; ++tx) scatter_node(G, tx * root, ty * root - n, root, roots[(size_t)ty * nrx + tx], B, tmp);
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src, size_t& bytes) {
+  *dst = nullptr;
+  if (src.empty()) return MM_OK;
+  const size_t nb = src.size() * sizeof(T);
+  hipError_t e = hipMalloc(dst, nb);
+  if (e != hipSuccess) return fail(MM_ERR_OOM, std::string("radon plan upload: ") + hipGetErrorString(e));
+  bytes += nb;
+  MM_HIP_TRY(hipMemcpy(*dst, src.data(), nb, hipMemcpyHostToDevice));
+  return MM_OK;
+}
+
+}  // namespace radon
+}  // namespace mm
+
+using namespace mm;
+using namespace mm::radon;
+
+struct mm_radon {
+  int dev = 0;
+  int n = 0, nz_max = 0, ntheta = 0, H = 0, FS = 0;
+  unsigned flags = 0;
+  double mu = 0;
+  float a = 0, C0g = 0, C0s = 0;
+  float2* fde = nullptr;   // nz_max x 2n x FS: R half spectrum / RT channel "re" half plane
+  float2* fde2 = nullptr;  // RT channel "im" (complex sinograms, allocated on first use)
+  float2* gh = nullptr;    // ntheta x nz_max x H: representatives
+  float2* gh2 = nullptr;   // RT channel "im" representatives
+  float2* partial = nullptr;
+  float* phx = nullptr;
+  float* phy = nullptr;
+  double2* dirs = nullptr;
+  int* sig = nullptr;
+  Item* gitems = nullptr;
+  int* gsamples = nullptr;
+  int n_gitems = 0;
+  Item* sitems = nullptr;
+  int* ssamples = nullptr;
+  int n_sitems = 0;
+  SplitTile* splits = nullptr;
+  int n_splits = 0, n_slots = 0;
+  const float2* tw = nullptr;
+  size_t bytes = 0;
+  bool profile = false;
+  bool ran[MM_RADON_NSTAGES] = {};
+  hipEvent_t ev[MM_RADON_NSTAGES + 1][2] = {};
+};
+
+// Records the start (which = 0) or end (which = 1) of `stage` when profiling.
+static int mark(mm_radon* p, int stage, int which, hipStream_t st) {
+  if (!p->profile) return MM_OK;
+  MM_HIP_TRY(hipEventRecord(p->ev[stage][which], st));
+  if (which == 1) p->ran[stage] = true;
+  return MM_OK;
+}
+
+static void radon_free(mm_radon* p) {
+  if (!p) return;
+  for (auto& e : p->ev)
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
+                  p->gitems, p->gsamples, p->sitems, p->ssamples, p->splits};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete p;
+}
+
+static int dmalloc(void** p, size_t nb, size_t& bytes, const char* what) {
+  hipError_t e = hipMalloc(p, nb);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(MM_ERR_OOM, std::string("radon ") + what + " (" + std::to_string(nb) + " bytes): " + hipGetErrorString(e));
+  }
+  bytes += nb;
+  return MM_OK;
+}
+
+// ---- launch helpers (templated on the FFT length) --------------------------
+
+template <int L>
+static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const long long units = (long long)nz * (p->n / 2);
+  const long long fzs = (long long)2 * p->n * p->FS;
+  hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, obj,
+                     p->fde, p->phx, p->phy, p->n, nz, p->FS, fzs, p->tw + L);
+}
+
+template <int L, bool INV>
+static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in_hi, int out_lo, int out_hi,
+                       float scale, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const int ncols = p->n + 1;
+  const long long zs = (long long)2 * p->n * p->FS;
+  hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(kThreads),
+                     0, st, buf, ncols, p->FS, zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + L);
+}
+
+template <int L>
+static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const long long rows = (long long)p->ntheta * nz;
+  hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, p->gh,
+                     sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + L);
+}
+
+template <int L>
+static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const long long rows = (long long)p->ntheta * nz;
+  const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
+  if (cplx)
+    hipLaunchKernelGGL((rt1_line_fft<L, true>), grid, dim3(kThreads), 0, st, sino, p->gh, p->gh2, p->sig, p->H, rows,
+                       nz, p->C0s, p->tw + L);
+  else
+    hipLaunchKernelGGL((rt1_line_fft<L, false>), grid, dim3(kThreads), 0, st, sino, p->gh, p->gh2, p->sig, p->H,
+                       rows, nz, p->C0s, p->tw + L);
+}
+
+template <int L>
+static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
+  using Sh = FftShape<L>;
+  const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
+  const long long hzs = (long long)2 * p->n * p->FS;
+  const dim3 grid((unsigned)((units + Sh::C - 1) / Sh::C));
+  const float scale = 1.0f / (2.0f * p->n);
+  if (cplx)
+    hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
+                       p->n, nz, p->FS, hzs, scale, p->tw + L);
+  else
+    hipLaunchKernelGGL((rt4_rows_c2r<L, false>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
+                       p->n, nz, p->FS, hzs, scale, p->tw + L);
+}
+
+#define MM_DISPATCH_L(L_RUNTIME, CALL)              \
+  switch (L_RUNTIME) {                              \
+    case 16: { constexpr int LL = 16; CALL; } break;     \
+    case 32: { constexpr int LL = 32; CALL; } break;     \
+    case 64: { constexpr int LL = 64; CALL; } break;     \
+    case 128: { constexpr int LL = 128; CALL; } break;   \
+    case 256: { constexpr int LL = 256; CALL; } break;   \
+    case 512: { constexpr int LL = 512; CALL; } break;   \
+    case 1024: { constexpr int LL = 1024; CALL; } break; \
+    case 2048: { constexpr int LL = 2048; CALL; } break; \
+    case 4096: { constexpr int LL = 4096; CALL; } break; \
+    case 8192: { constexpr int LL = 8192; CALL; } break; \
+    case 16384: { constexpr int LL = 16384; CALL; } break; \
+    default: return fail(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(L_RUNTIME) + " not built"); \
+  }
+
+static int zsplit_for(int items, int nz) {
+  if (items <= 0) return 1;
+  int zs = (2048 + items - 1) / items;
+  return std::max(1, std::min(zs, nz));
+}
+
+static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
+  if (p->n_gitems == 0) return MM_OK;
+  const int zsplit = zsplit_for(p->n_gitems, nz);
+  const int zchunk = (nz + zsplit - 1) / zsplit;
+  const int zblocks = (nz + zchunk - 1) / zchunk;
+  const long long fzs = (long long)2 * p->n * p->FS;
+  hipLaunchKernelGGL(gather_tiles, dim3((unsigned)p->n_gitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st, p->fde,
+                     p->gh, p->gitems, p->gsamples, p->dirs, p->n, p->H, nz, p->FS, fzs, zchunk, p->a, p->C0g);
+  return check_launch("gather_tiles");
+}
+
+static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int nz, hipStream_t st) {
+  const int zsplit = zsplit_for(p->n_sitems, nz);
+  const int zchunk = (nz + zsplit - 1) / zsplit;
+  const int zblocks = (nz + zchunk - 1) / zchunk;
+  const long long hzs = (long long)2 * p->n * p->FS;
+  if (p->flags & MM_RADON_FP64_TILES)
+    hipLaunchKernelGGL(scatter_tiles<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(TILE_THREADS), 0,
+                       st, yh, Hb, p->partial, p->sitems, p->ssamples, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk,
+                       p->a);
+  else
+    hipLaunchKernelGGL(scatter_tiles<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st,
+                       yh, Hb, p->partial, p->sitems, p->ssamples, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
+  int rc = check_launch("scatter_tiles");
+  if (rc) return rc;
+  if (p->n_splits > 0) {
+    hipLaunchKernelGGL(scatter_reduce, dim3((unsigned)p->n_splits, (unsigned)nz), dim3(FLOOR * FLOOR), 0, st,
+                       p->partial, Hb, p->splits, p->n, nz, p->FS, hzs);
+    rc = check_launch("scatter_reduce");
+  }
+  return rc;
+}
+
+namespace mm {
+void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max) {
+  *n = p->n;
+  *ntheta = p->ntheta;
+  *nz_max = p->nz_max;
+}
+}  // namespace mm
+
+extern "C" {
+
+int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags) {
+  MM_ARG_CHECK(plan != nullptr, "mm_radon_create: plan is NULL");
+  *plan = nullptr;
+  MM_ARG_CHECK(is_pow2(n) && n >= 16 && 2 * n <= kMaxFFT,
+               "mm_radon_create: n must be a power of two in [16, 8192], got " + std::to_string(n));
+  MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
+  MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
+  MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
+  MM_ARG_CHECK((flags & ~MM_RADON_FP64_TILES) == 0, "mm_radon_create: unknown flags");
+  const double eps = 1e-3;
+  const double mu = -std::log(eps) / (2.0 * n * n);
+  const int m = (int)std::ceil(2.0 * n / M_PI * std::sqrt(-mu * std::log(eps) + (mu * n) * (mu * n) / 4.0));
+  if (m != M) return fail(MM_ERR_UNSUPPORTED, "mm_radon_create: window half width m=" + std::to_string(m) + " (kernels built for 4)");
+
+  mm_radon* p = new mm_radon();
+  int rc = MM_OK;
+  auto bail = [&](int code) {
+    radon_free(p);
+    return code;
+  };
+  if ((rc = twiddle_bank(&p->tw))) return bail(rc);
+  if (hipGetDevice(&p->dev) != hipSuccess) return bail(fail(MM_ERR_HIP, "hipGetDevice"));
+  p->n = n;
+  p->nz_max = nz_max;
+  p->ntheta = ntheta;
+  p->H = n / 2 + 1;
+  p->FS = n + 2;
+  p->flags = flags;
+  p->mu = mu;
+  p->a = (float)(M_PI * M_PI / (mu * 4.0 * n * n));
+  p->C0g = (float)(M_PI / mu / n);
+  p->C0s = (float)(M_PI / mu);
+
+  // separable phi (processing/tomo.py:46-49,59): phi(y,x) = phy[y] * phx[x]
+  std::vector<float> phx(n), phy(n);
+  const double sgn_n = (double)(1 - n % 4);
+  const double cst = 1.0 / (n * std::sqrt((double)n * ntheta));
+  for (int i = 0; i < n; ++i) {
+    const double t = (double)i / n - 0.5;
+    const double e = std::exp(mu * (double)n * n * t * t);
+    phy[i] = (float)e;
+    phx[i] = (float)(e * sgn_n * cst);
+  }
+  HostGeom G;
+  G.n = n;
+  G.H = p->H;
+  G.ntheta = ntheta;
+  G.dx.resize(ntheta);
+  G.dy.resize(ntheta);
+  std::vector<double2> dirs(ntheta);
+  std::vector<int> sig(ntheta);
+  for (int t = 0; t < ntheta; ++t) {
+    const double c = std::cos(theta_host[t]), s = std::sin(theta_host[t]);
+    const int sg = c >= 0 ? 1 : -1;
+    G.dx[t] = std::fabs(c);
+    G.dy[t] = -sg * s;
+    dirs[t] = make_double2(G.dx[t], G.dy[t]);
+    sig[t] = sg;
+  }
+  Binning BG, BS;
+  const int root = std::min(ROOT, n);
+  build_gather(G, root, BG);
+  build_scatter(G, root, BS);
+
+  size_t& bytes = p->bytes;
+  if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
+      (rc = upload(&p->sig, sig, bytes)) || (rc = upload(&p->gitems, BG.items, bytes)) ||
+      (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->sitems, BS.items, bytes)) ||
+      (rc = upload(&p->ssamples, BS.samples, bytes)) || (rc = upload(&p->splits, BS.splits, bytes)))
+    return bail(rc);
+  p->n_gitems = (int)BG.items.size();
+  p->n_sitems = (int)BS.items.size();
+  p->n_splits = (int)BS.splits.size();
+  p->n_slots = BS.nslots;
+  if ((rc = dmalloc((void**)&p->fde, sizeof(float2) * (size_t)nz_max * 2 * n * p->FS, bytes, "fde")) ||
+      (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives")))
+    return bail(rc);
+  if (p->n_slots > 0 &&
+      (rc = dmalloc((void**)&p->partial, sizeof(float2) * (size_t)p->n_slots * nz_max * FLOOR * FLOOR, bytes, "partial tiles")))
+    return bail(rc);
+  *plan = p;
+  return MM_OK;
+}
+
+int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_fwd: plan is NULL");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
+               "mm_radon_fwd: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
+  MM_ARG_CHECK(obj && sino, "mm_radon_fwd: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int L2 = 2 * p->n, L1 = p->n;
+  int rc;
+  if ((rc = mark(p, 0, 0, st))) return rc;
+  MM_DISPATCH_L(L2, (launch_r1<LL>(p, obj, nz, st)));
+  if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
+  MM_DISPATCH_L(L2, (launch_col<LL, false>(p, p->fde, nz, p->n / 2, 3 * p->n / 2, 0, 2 * p->n, 1.0f, st)));
+  if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
+  if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
+  MM_DISPATCH_L(L1, (launch_r4<LL>(p, sino, nz, st)));
+  if ((rc = check_launch("r4_line_ifft"))) return rc;
+  return mark(p, 3, 1, st);
+}
+
+int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_adj: plan is NULL");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
+               "mm_radon_adj: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
+  MM_ARG_CHECK(sino && obj, "mm_radon_adj: null pointer");
+  MM_ARG_CHECK(sino_is_c64 == 0 || sino_is_c64 == 1, "mm_radon_adj: sino_is_c64 must be 0 or 1");
+  const bool cplx = sino_is_c64 != 0;
+  if (cplx) {
+    int rc;
+    if (!p->fde2 &&
+        (rc = dmalloc((void**)&p->fde2, sizeof(float2) * (size_t)p->nz_max * 2 * p->n * p->FS, p->bytes, "fde (im)")))
+      return rc;
+    if (!p->gh2 &&
+        (rc = dmalloc((void**)&p->gh2, sizeof(float2) * (size_t)p->ntheta * p->nz_max * p->H, p->bytes, "representatives (im)")))
+      return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int L2 = 2 * p->n, L1 = p->n;
+  int rc;
+  if ((rc = mark(p, 4, 0, st))) return rc;
+  MM_DISPATCH_L(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
+  if ((rc = check_launch("rt1_line_fft")) || (rc = mark(p, 4, 1, st)) || (rc = mark(p, 5, 0, st))) return rc;
+  if ((rc = launch_scatter(p, p->gh, p->fde, nz, st))) return rc;
+  if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
+  if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
+  const float s = 1.0f / (2.0f * p->n);
+  MM_DISPATCH_L(L2, (launch_col<LL, true>(p, p->fde, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+  if ((rc = check_launch("col_fft (RT)"))) return rc;
+  if (cplx) {
+    MM_DISPATCH_L(L2, (launch_col<LL, true>(p, p->fde2, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+    if ((rc = check_launch("col_fft (RT im)"))) return rc;
+  }
+  if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
+  MM_DISPATCH_L(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
+  if ((rc = check_launch("rt4_rows_c2r"))) return rc;
+  return mark(p, 7, 1, st);
+}
+
+int mm_radon_profile(mm_radon* p, int enable) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_profile: plan is NULL");
+  if (enable && !p->ev[0][0]) {
+    for (auto& e : p->ev)
+      for (auto& x : e) MM_HIP_TRY(hipEventCreate(&x));
+  }
+  p->profile = enable != 0;
+  for (bool& r : p->ran) r = false;
+  return MM_OK;
+}
+
+int mm_radon_stage_ms(mm_radon* p, float* ms, int max_stages) {
+  MM_ARG_CHECK(p != nullptr && ms != nullptr, "mm_radon_stage_ms: null pointer");
+  for (int s = 0; s < max_stages && s < MM_RADON_NSTAGES; ++s) {
+    ms[s] = -1.f;
+    if (!p->ran[s]) continue;
+    MM_HIP_TRY(hipEventSynchronize(p->ev[s][1]));
+    MM_HIP_TRY(hipEventElapsedTime(&ms[s], p->ev[s][0], p->ev[s][1]));
+  }
+  return MM_OK;
+}
+
+size_t mm_radon_workspace_bytes(const mm_radon* p) { return p ? p->bytes : 0; }
+
+void mm_radon_destroy(mm_radon* p) { radon_free(p); }
+
+}  // extern "C"

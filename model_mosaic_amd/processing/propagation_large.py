@@ -1,0 +1,109 @@
+"""PropagationLarge: the host-array interface of the Fresnel propagator.
+
+Mirrors ``PropagationLarge`` of processing/propagation_large.py:35-348:
+constructor ``PropagationLarge(n, nz, wavelength, voxelsize, distance)``,
+``D(psi, j, chunks)`` with psi a host (ntheta, nz, n) or (nz, n) complex64
+array and ``chunks = [CHUNK_NZ, CHUNK_2N]``, ``psi_buffer(ntheta)`` and
+``free()``.  Same validation (AssertionError with the reference's messages,
+processing/propagation_large.py:216-223) and ownership: ``D`` returns the
+internal cached pinned output, overwritten by the next call.  ``DT`` (new)
+applies the exact adjoint with the same interface.
+
+The reference streams strips through a 40 GB card; here the frames move to
+HBM in batches bounded by ``gpu_budget_bytes`` and the separable two-pass
+kernels of libmosaic_hip.so run on them.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from model_mosaic_amd._carriers import pinned_empty, require_gpu
+from model_mosaic_amd.processing.propagation import Propagation
+
+DEFAULT_GPU_BUDGET = 96 << 30
+
+
+def check_prop_chunks(nz: int, n: int, psi_shape, chunks) -> None:
+    """Shape and [CHUNK_NZ, CHUNK_2N] validation of processing/propagation_large.py:215-223."""
+    _, nz_in, n_in = psi_shape
+    assert nz_in == nz and n_in == n, f"psi shape (…, {nz_in}, {n_in}) mismatches propagator ({nz}, {n})"
+    chunk_nz, chunk_2n = (int(c) for c in chunks)
+    assert chunk_nz >= 1 and nz % chunk_nz == 0, f"CHUNK_NZ={chunk_nz} must divide nz={nz}"
+    assert chunk_2n >= 1 and (2 * n) % chunk_2n == 0, f"CHUNK_2N={chunk_2n} must divide 2n={2 * n}"
+
+
+class PropagationLarge:
+    """Fresnel forward propagator for host-resident frame stacks."""
+
+    def __init__(self, n, nz, wavelength, voxelsize, distance, gpu_budget_bytes: int = DEFAULT_GPU_BUDGET,
+                 pad: str = "symmetric"):
+        require_gpu()
+        self.n = int(n)
+        self.nz = int(nz)
+        self.wavelength = float(wavelength)
+        self.voxelsize = float(voxelsize)
+        self.distance = np.atleast_1d(np.asarray(distance, dtype=np.float64))
+        self.gpu_budget_bytes = int(gpu_budget_bytes)
+        self.pad = pad
+        self._prop: Propagation | None = None
+        self._out = None
+        self._psi = None
+
+    def _batch(self, ntheta: int) -> int:
+        per = 2 * self.nz * self.n * 8
+        return int(max(1, min(ntheta, self.gpu_budget_bytes // per)))
+
+    def _get_prop(self, b: int) -> Propagation:
+        if self._prop is None or self._prop.ntheta < b:
+            if self._prop is not None:
+                self._prop.free()
+            self._prop = Propagation(self.n, self.nz, b, len(self.distance), self.wavelength, self.voxelsize,
+                                     self.distance, pad=self.pad)
+        return self._prop
+
+    def psi_buffer(self, ntheta):
+        """Pinned (ntheta, nz, n) complex64 buffer callers may fill before ``D`` (propagation_large.py:177)."""
+        shape = (int(ntheta), self.nz, self.n)
+        if self._psi is None or tuple(self._psi.shape) != shape:
+            self._psi = pinned_empty(shape, torch.complex64)
+        return self._psi.numpy()
+
+    def _get_out(self, ntheta):
+        shape = (ntheta, self.nz, self.n)
+        if self._out is None or tuple(self._out.shape) != shape:
+            self._out = pinned_empty(shape, torch.complex64)
+        return self._out
+
+    def free(self):
+        if self._prop is not None:
+            self._prop.free()
+        self._prop = None
+        self._out = self._psi = None
+
+    def _run(self, psi, j, chunks, adj: bool):
+        psi_t = psi if isinstance(psi, torch.Tensor) else torch.as_tensor(np.asarray(psi))
+        added_dim = psi_t.ndim == 2
+        if added_dim:
+            psi_t = psi_t[None]
+        ntheta = psi_t.shape[0]
+        check_prop_chunks(self.nz, self.n, tuple(psi_t.shape), chunks)
+        b = self._batch(ntheta)
+        prop = self._get_prop(b)
+        out = self._get_out(ntheta)
+        for t0 in range(0, ntheta, b):
+            t1 = min(ntheta, t0 + b)
+            x = psi_t[t0:t1].to(prop.device, dtype=torch.complex64).contiguous()
+            y = prop.DT(x, j) if adj else prop.D(x, j)
+            out[t0:t1].copy_(y)
+        torch.cuda.synchronize(prop.device)
+        res = out.numpy()
+        return res[0] if added_dim else res
+
+    def D(self, psi, j, chunks):
+        """Forward Fresnel propagation with the j-th distance (propagation_large.py:202)."""
+        return self._run(psi, j, chunks, False)
+
+    def DT(self, psi, j, chunks):
+        """Adjoint of ``D`` (new; the reference provides only D)."""
+        return self._run(psi, j, chunks, True)

@@ -1,0 +1,101 @@
+"""TomoReal: USFFT Radon transform on one MI355X, device-resident.
+
+Mirrors ``TomoReal`` of processing/tomo.py:17-188 (same constructor
+``TomoReal(n, nz, theta)``, ``R``, ``RT``, ``free``; same layouts: obj
+(nz, n, n) float32, sinogram (ntheta, nz, n)).  Carriers are torch ROCm
+tensors instead of cupy arrays; numpy inputs are copied to the device.
+All arithmetic happens in libmosaic_hip.so (mm_radon_*).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from model_mosaic_amd import _lib
+from model_mosaic_amd._carriers import as_device_tensor, require_gpu
+
+
+class TomoReal:
+    """Radon transform R and adjoint RT via the USFFT (eps = 1e-3, m = 4, 2x oversampling).
+
+    Parameters mirror processing/tomo.py:30: ``n`` detector width (= object
+    width), ``nz`` the maximum number of slices per call, ``theta`` the
+    projection angles in radians (stored as float32, like the reference).
+    ``fp64_tiles`` accumulates the adjoint scatter tiles in FP64.
+    """
+
+    def __init__(self, n, nz, theta, fp64_tiles: bool = False):
+        require_gpu()
+        self._lib = _lib.load()
+        self.n = int(n)
+        self._nz = int(nz)
+        self.theta = np.asarray(theta).astype(np.float32)
+        self.ntheta = len(self.theta)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        th = np.ascontiguousarray(self.theta.astype(np.float64))
+        plan = C.c_void_p()
+        flags = _lib.MM_RADON_FP64_TILES if fp64_tiles else 0
+        _lib.check(self._lib.mm_radon_create(C.byref(plan), self.n, self._nz, self.ntheta,
+                                             th.ctypes.data_as(C.POINTER(C.c_double)), flags),
+                   "TomoReal")
+        self._plan = plan
+
+    # -- helpers -------------------------------------------------------------
+    def _alive(self):
+        if not self._plan:
+            raise RuntimeError("TomoReal: plan was freed")
+
+    def workspace_bytes(self) -> int:
+        return int(self._lib.mm_radon_workspace_bytes(self._plan)) if self._plan else 0
+
+    def free(self):
+        """Releases the device plan (processing/tomo.py:109)."""
+        if getattr(self, "_plan", None):
+            torch.cuda.synchronize(self.device)
+            self._lib.mm_radon_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # -- operators -----------------------------------------------------------
+    def R(self, obj, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(nz, n, n) float32 object -> (ntheta, nz, n) float32 sinogram (processing/tomo.py:121)."""
+        self._alive()
+        x = as_device_tensor(obj, torch.float32, self.device)
+        if x.ndim != 3 or x.shape[1] != self.n or x.shape[2] != self.n:
+            raise ValueError(f"TomoReal.R: obj shape {tuple(x.shape)} != (nz, {self.n}, {self.n})")
+        nz = x.shape[0]
+        if not 1 <= nz <= self._nz:
+            raise ValueError(f"TomoReal.R: nz={nz} outside [1, {self._nz}]")
+        if out is None:
+            out = torch.empty((self.ntheta, nz, self.n), dtype=torch.float32, device=self.device)
+        _lib.check(self._lib.mm_radon_fwd(self._plan, x.data_ptr(), out.data_ptr(), nz, _lib.stream_handle()),
+                   "TomoReal.R")
+        return out
+
+    def RT(self, data, out: torch.Tensor | None = None) -> torch.Tensor:
+        """(ntheta, nz, n) sinogram -> (nz, n, n) object (processing/tomo.py:153).
+
+        float32 input returns the real part (float32); any other dtype is taken
+        as complex64 and returns the complex result, like the reference.
+        """
+        self._alive()
+        is_real = (data.dtype == torch.float32) if isinstance(data, torch.Tensor) else (np.asarray(data).dtype == np.float32)
+        dt = torch.float32 if is_real else torch.complex64
+        y = as_device_tensor(data, dt, self.device)
+        if y.ndim != 3 or y.shape[0] != self.ntheta or y.shape[2] != self.n:
+            raise ValueError(f"TomoReal.RT: data shape {tuple(y.shape)} != ({self.ntheta}, nz, {self.n})")
+        nz = y.shape[1]
+        if not 1 <= nz <= self._nz:
+            raise ValueError(f"TomoReal.RT: nz={nz} outside [1, {self._nz}]")
+        if out is None:
+            out = torch.empty((nz, self.n, self.n), dtype=dt, device=self.device)
+        _lib.check(self._lib.mm_radon_adj(self._plan, y.data_ptr(), 0 if is_real else 1, out.data_ptr(), nz,
+                                          _lib.stream_handle()), "TomoReal.RT")
+        return out

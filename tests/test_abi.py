@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every entry point declared in include/mosaic_hip.h.
+
+No compute call is made here: only entry points that return before touching a
+device (argument validation, version, error string)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from model_mosaic_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mosaic_hip.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b(mm_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not _lib.LIB_PATH.exists():
+        from model_mosaic_amd import build
+        build.build()
+    return _lib.load()
+
+
+def test_header_matches_binding_table():
+    assert header_functions() == set(_lib.EXPORTED)
+
+
+def test_library_exports_every_symbol(lib):
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_error_string(lib):
+    assert lib.mm_version() >= 100
+    assert isinstance(_lib.last_error(), str)
+
+
+def test_null_plan_is_an_argument_error(lib):
+    rc = lib.mm_radon_fwd(None, None, None, 1, None)
+    assert rc == _lib.MM_ERR_ARG
+    assert "plan is NULL" in _lib.last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc, "mm_radon_fwd")
+    assert lib.mm_fresnel_fwd(None, None, None, 1, 0, None) == _lib.MM_ERR_ARG
+    assert lib.mm_model_fwd(None, None, None, None, 0, None, None) == _lib.MM_ERR_ARG
+
+
+def test_bad_sizes_rejected_before_device_work(lib):
+    p = C.c_void_p()
+    th = (C.c_double * 4)(0, 0.1, 0.2, 0.3)
+    rc = lib.mm_radon_create(C.byref(p), 100, 1, 4, th, 0)    # not a power of two
+    assert rc == _lib.MM_ERR_ARG and not p
+    rc = lib.mm_fresnel_create(C.byref(p), 64, 48, 1, 1e-10, 1e-6, (C.c_double * 1)(0.1), 0)
+    assert rc == _lib.MM_ERR_ARG and not p
+    rc = lib.mm_psi_from_proj(None, None, -1, 1.0, 1.0, None)
+    assert rc == _lib.MM_ERR_ARG
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(_lib.MosaicLibraryError):
+        _lib.load(tmp_path / "nope.so")

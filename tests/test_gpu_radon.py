@@ -1,0 +1,191 @@
+"""HIP Radon R / RT parity against the oracle and its golden fixtures (needs a GPU)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import usfft
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def tomo_cls():
+    from model_mosaic_amd.processing.tomo import TomoReal
+    return TomoReal
+
+
+def test_R_RT_golden_small(tomo_cls):
+    g = gold("radon_small")
+    nz = g["obj"].shape[0]
+    t = tomo_cls(64, nz, g["theta"])
+    r = t.R(torch.from_numpy(g["obj"]).cuda()).cpu().numpy()
+    assert r.dtype == np.float32 and r.shape == g["R"].shape
+    b = t.RT(torch.from_numpy(g["y"]).cuda()).cpu().numpy()
+    c = t.RT(torch.from_numpy(g["yc"]).cuda()).cpu().numpy()
+    print(f"small: R {rel(r, g['R']):.2e}  RT {rel(b, g['RT']):.2e}  RT complex {rel(c, g['RTc']):.2e}")
+    assert rel(r, g["R"]) < 1e-5
+    assert b.dtype == np.float32 and rel(b, g["RT"]) < 1e-5
+    assert c.dtype == np.complex64 and rel(c, g["RTc"]) < 1e-5
+
+
+def test_R_RT_golden_c1(tomo_cls):
+    g = gold("radon_c1")
+    t = tomo_cls(256, 2, g["theta"])
+    r = t.R(torch.from_numpy(g["obj"]).cuda()).cpu().numpy()
+    err64 = rel(r, g["R"])
+    err_nudft = rel(r[:, :1], g["R_nudft"])
+    print(f"C1 sizes: R rel-L2 vs FP64 USFFT {err64:.2e}, vs exact NUDFT {err_nudft:.2e}")
+    assert err64 < 1e-5
+    assert err_nudft < 2e-3
+    b = t.RT(torch.from_numpy(g["R"]).cuda()).cpu().numpy()
+    assert rel(b, g["RT"]) < 1e-5
+
+
+def test_R_matches_fp32_reference_restatement(tomo_cls):
+    n, nz, nt = 64, 3, 40
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    x = np.random.default_rng(11).standard_normal((nz, n, n)).astype(np.float32)
+    r = tomo_cls(n, nz, th).R(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert rel(r, usfft.tomo_R(x, th, np.float32)) < 1e-5
+    assert rel(r, usfft.tomo_R(x, th, np.float64)) < 1e-5
+
+
+@pytest.mark.parametrize("n,nt,span", [(16, 7, np.pi), (32, 33, np.pi), (128, 64, 2 * np.pi), (512, 96, np.pi)])
+def test_sizes_vs_fp64(tomo_cls, n, nt, span):
+    nz = 2
+    th = np.linspace(0, span, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(n + nt)
+    x = rng.standard_normal((nz, n, n)).astype(np.float32)
+    y = rng.standard_normal((nt, nz, n)).astype(np.float32)
+    t = tomo_cls(n, nz, th)
+    assert rel(t.R(torch.from_numpy(x).cuda()).cpu().numpy(), usfft.tomo_R(x, th, np.float64)) < 1e-5
+    assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
+
+
+def test_random_angles_and_partial_nz(tomo_cls):
+    n, nt = 64, 29
+    th = np.sort(np.random.default_rng(5).uniform(0, 2 * np.pi, nt)).astype(np.float32)
+    x = np.random.default_rng(6).standard_normal((3, n, n)).astype(np.float32)
+    t = tomo_cls(n, 8, th)   # plan for 8 slices, call with 3
+    assert rel(t.R(torch.from_numpy(x).cuda()).cpu().numpy(), usfft.tomo_R(x, th, np.float64)) < 1e-5
+
+
+@pytest.mark.parametrize("fp64_tiles", [False, True])
+def test_adjoint_identity(tomo_cls, fp64_tiles):
+    """<R x, y> = <x, RT y> (tests/test_large_variants.py:122-172), inner products in FP64.
+
+    Random x, y make <Rx, y> cancel down to ~sqrt(N) of its terms, which
+    magnifies FP32 rounding; so besides the reference's |lhs-rhs|/max(|lhs|,|rhs|)
+    (at its own tolerance) the error is also bounded by the Cauchy-Schwarz scale
+    ||Rx|| ||y|| and on the non-cancelling pair y = Rx, both at 1e-6."""
+    n, nz, nt = 64, 8, 24
+    th = np.linspace(0, 2 * np.pi, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(rng.standard_normal((nz, n, n)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.standard_normal((nt, nz, n)).astype(np.float32)).cuda()
+    t = tomo_cls(n, nz, th, fp64_tiles=fp64_tiles)
+    rx = t.R(x).double()
+    lhs = torch.sum(rx * y.double()).item()
+    rhs = torch.sum(x.double() * t.RT(y).double()).item()
+    ref_err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    cs_err = abs(lhs - rhs) / (rx.norm().item() * y.double().norm().item())
+    lhs2 = torch.sum(rx * rx).item()
+    rhs2 = torch.sum(x.double() * t.RT(rx.float()).double()).item()
+    self_err = abs(lhs2 - rhs2) / lhs2
+    print(f"adjoint: ref-metric {ref_err:.2e}  cauchy-schwarz {cs_err:.2e}  y=Rx {self_err:.2e}")
+    assert ref_err < 1e-4
+    assert cs_err < 1e-6
+    assert self_err < 1e-6
+
+
+def test_R_deterministic(tomo_cls):
+    n, nz, nt = 128, 2, 64
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    x = torch.randn((nz, n, n), device="cuda")
+    t = tomo_cls(n, nz, th)
+    assert torch.equal(t.R(x), t.R(x))
+
+
+def test_shape_errors(tomo_cls):
+    t = tomo_cls(32, 2, np.linspace(0, np.pi, 8, endpoint=False))
+    with pytest.raises(ValueError):
+        t.R(torch.zeros((3, 32, 32), device="cuda"))
+    with pytest.raises(ValueError):
+        t.R(torch.zeros((2, 32, 16), device="cuda"))
+    with pytest.raises(ValueError):
+        t.RT(torch.zeros((7, 2, 32), device="cuda"))
+
+
+# ---- TomoLargeReal: tests/test_large_variants.py semantics ----------------------------------
+
+def test_large_R_chunk_invariance_and_parity(tomo_cls):
+    from model_mosaic_amd.processing.tomo_large import TomoLargeReal
+    n, nz, nt = 64, 16, 24
+    th = np.linspace(0, 2 * np.pi, nt, endpoint=False).astype(np.float32)
+    np.random.seed(0)
+    obj = np.random.randn(nz, n, n).astype(np.float32)
+    ref = tomo_cls(n, nz, th).R(torch.from_numpy(obj).cuda()).cpu().numpy()
+    tl = TomoLargeReal(n, th, n)
+    coarse = tl.R(obj, [n, nt, n]).copy()
+    for chunks in [(32, 12, 32), (16, 6, 16), (8, 6, 8)]:
+        fine = TomoLargeReal(n, th, chunks[2]).R(obj, list(chunks)).copy()
+        assert np.max(np.abs(fine - coarse)) / np.max(np.abs(coarse)) < 1e-4
+    assert np.max(np.abs(coarse - ref)) / np.max(np.abs(ref)) < 1e-4
+    assert rel(coarse, usfft.tomo_large_R(obj, th, np.float64)) < 1e-5
+
+
+def test_large_R_returns_view_until_next_call():
+    from model_mosaic_amd.processing.tomo_large import TomoLargeReal
+    n, nt = 32, 8
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    tl = TomoLargeReal(n, th, n)
+    a = tl.R(np.ones((2, n, n), np.float32), [n, nt, n])
+    keep = a.copy()
+    tl.R(np.zeros((2, n, n), np.float32), [n, nt, n])
+    assert np.all(a == 0) and np.any(keep != 0)   # same buffer, overwritten (tomo_large.py:545-553)
+
+
+@pytest.mark.parametrize("cplx", [True, False])
+def test_large_RT_parity(tomo_cls, cplx):
+    from model_mosaic_amd.processing.tomo_large import TomoLargeReal
+    n, nz, nt = 64, 8, 24
+    th = np.linspace(0, 2 * np.pi, nt, endpoint=False).astype(np.float32)
+    np.random.seed(0)
+    if cplx:
+        sino = (np.random.randn(nt, nz, n) + 1j * np.random.randn(nt, nz, n)).astype(np.complex64)
+    else:
+        sino = np.random.randn(nt, nz, n).astype(np.float32)
+    ref = tomo_cls(n, nz, th).RT(torch.from_numpy(sino).cuda()).cpu().numpy()
+    for chunks in [(n, nt, 2 * n), (32, 12, 32), (16, 6, 16), (8, 6, 8)]:
+        got = np.asarray(TomoLargeReal(n, th, chunks[2]).RT(sino, list(chunks))).copy()
+        assert got.dtype == sino.dtype
+        assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-4
+    assert rel(got, usfft.tomo_large_RT(sino, th, np.float64)) < 1e-5
+
+
+def test_large_adjoint_and_validation():
+    from model_mosaic_amd.processing.tomo_large import TomoLargeReal
+    n, nz, nt = 64, 8, 24
+    th = np.linspace(0, 2 * np.pi, nt, endpoint=False).astype(np.float32)
+    np.random.seed(0)
+    x = np.random.randn(nz, n, n).astype(np.float32)
+    y = np.random.randn(nt, nz, n).astype(np.float32)
+    tl = TomoLargeReal(n, th, n)
+    rx = np.asarray(tl.R(x, [n, nt, n])).copy()
+    rty = np.asarray(tl.RT(y, [n, nt, n])).copy()
+    lhs, rhs = float(np.sum(rx.astype(np.float64) * y)), float(np.sum(x.astype(np.float64) * rty))
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) < 1e-6
+    with pytest.raises(AssertionError):
+        tl.R(x, [24, nt, n])

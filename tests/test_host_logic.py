@@ -1,0 +1,50 @@
+"""Host-side logic that needs no GPU: chunk validation, status mapping, phantoms."""
+import numpy as np
+import pytest
+
+from model_mosaic_amd import _lib, phantom
+from model_mosaic_amd.processing.propagation_large import check_prop_chunks
+from model_mosaic_amd.processing.tomo_large import check_tomo_chunks
+
+
+def test_prop_chunk_validation_matches_reference():
+    # tests/test_large_variants.py:205-223: non-dividing chunks raise AssertionError
+    check_prop_chunks(16, 32, (1, 16, 32), (8, 16))
+    with pytest.raises(AssertionError, match="CHUNK_NZ=5"):
+        check_prop_chunks(16, 32, (1, 16, 32), (5, 16))
+    with pytest.raises(AssertionError, match="CHUNK_2N=9"):
+        check_prop_chunks(16, 32, (1, 16, 32), (8, 9))
+    with pytest.raises(AssertionError, match="mismatches propagator"):
+        check_prop_chunks(16, 32, (1, 16, 30), (8, 16))
+
+
+def test_tomo_chunk_validation():
+    for chunks in [(64, 24, 64), (32, 12, 32), (16, 6, 16), (8, 6, 8), (64, 24, 128)]:
+        check_tomo_chunks(64, 24, chunks)
+    with pytest.raises(AssertionError, match="CHUNK_N=24"):
+        check_tomo_chunks(64, 24, (24, 12, 32))
+    with pytest.raises(AssertionError, match="CHUNK_THETA=5"):
+        check_tomo_chunks(64, 24, (32, 5, 32))
+    with pytest.raises(AssertionError):
+        check_tomo_chunks(64, 24, (32, 12))
+
+
+def test_status_mapping():
+    assert _lib.check(_lib.MM_OK, "x") is None
+    with pytest.raises(ValueError):
+        _lib.check(_lib.MM_ERR_ARG, "x")
+    with pytest.raises(MemoryError):
+        _lib.check(_lib.MM_ERR_OOM, "x")
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.MM_ERR_UNSUPPORTED, "x")
+    with pytest.raises(_lib.MosaicLibraryError):
+        _lib.check(_lib.MM_ERR_HIP, "x")
+
+
+def test_phantom_seeded_and_bounded():
+    a = phantom.ellipsoids(8, 64, seed=3)
+    b = phantom.ellipsoids(8, 64, seed=3)
+    assert a.shape == (8, 64, 64) and bool((a == b).all())
+    assert float(a.max()) <= 12 * 1e-6 and float(a.min()) >= 0 and float(a.max()) > 0
+    th = phantom.angles(384)
+    assert th.dtype == np.float32 and th[0] == 0 and th[-1] < np.pi
