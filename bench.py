@@ -1,0 +1,267 @@
+"""Benchmark of the USFFT Radon R+RT hot path (and the forward-model iteration) on one MI355X.
+
+Workload (batched FP32 USFFT R and RT on one GPU): a float32 delta volume
+nz=64 x 2048 x 2048 of 12 seeded random ellipsoids plus a 1% Gaussian-noise
+texture, ntheta=2048 angles uniform on [0, pi), detector 2048, complex64 padded
+Fourier grid 4096^2.  One step = R of all 64 slices followed by RT of the
+resulting sinogram.  value = slices / s (R+RT) over the timed steps.
+
+Also reported in the same JSON line:
+  * roofline of the dominant kernel (HIP events on the launch stream, live over
+    the timed region) and of the whole step against the algorithmic bytes
+    B_dir(n, ntheta) = 4 n^2 + 32 n (n+1) + 4 ntheta n per slice per direction;
+  * per-stage milliseconds;
+  * the plain one-slice-at-a-time path (same kernels, nz = 1 per call) and the
+    batched/plain ratio;
+  * the forward+backward model iteration (nz=128 x 1024^2, ntheta=1024, 30 keV,
+    z = 0.2 m, pixel 1.38 um, Poisson data at 1e4 photons/pixel) in s/iter;
+  * cpu_baseline: the NumPy FP32 restatement (oracle) of R+RT on the host,
+    one n=2048 slice at 256 and 512 angles, extrapolated linearly in ntheta
+    to the 2048-angle slice, reported in slices/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def b_dir(n, nt):
+    return 4 * n * n + 32 * n * (n + 1) + 4 * nt * n
+
+
+def stage_bytes(n, nt):
+    """Algorithmic bytes per slice of each stage: its own input read once + output written once."""
+    h = n // 2 + 1
+    half = 8 * n * (n + 1)       # n rows (or columns) of the half spectrum
+    full = 16 * n * (n + 1)      # the 2n x (n+1) half spectrum
+    reps = 8 * nt * h            # representative samples
+    return {
+        "r_rows_rfft": 4 * n * n + half,
+        "r_col_fft": half + full,
+        "r_gather": full + reps,
+        "r_line_ifft": reps + 4 * nt * n,
+        "rt_line_fft": 4 * nt * n + reps,
+        "rt_scatter": reps + full,
+        "rt_col_ifft": full + half,
+        "rt_rows_c2r": half + 4 * n * n,
+    }
+
+
+def cpu_baseline():
+    """Oracle (NumPy FP32 restatement) R+RT of one n=2048 slice, single thread."""
+    import numpy as np
+
+    from oracle import usfft
+
+    n = 2048
+    x = np.random.default_rng(0).standard_normal((1, n, n)).astype(np.float32)
+    times = {}
+    for nt in (256, 512):
+        th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+        t0 = time.perf_counter()
+        r = usfft.tomo_R(x, th, np.float32)
+        usfft.tomo_RT(r, th, dtype=np.float32)
+        times[nt] = time.perf_counter() - t0
+    per_slice = times[512] + (times[512] - times[256]) * (2048 - 512) / 256
+    return {
+        "value": 1.0 / per_slice,
+        "unit": "slices/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle.usfft FP32 R+RT of 1 slice n=2048 at 256 angles ({times[256]:.1f} s) and 512 angles "
+                   f"({times[512]:.1f} s), linearly extrapolated in ntheta to 2048 angles ({per_slice:.1f} s/slice)"),
+    }
+
+
+def run_model(torch, steps=3, warmup=1):
+    import numpy as np
+
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.model import ForwardModel, misfit
+    from model_mosaic_amd.step_kernels import abs2, psi_from_proj
+
+    nz, n, nt = 128, 1024, 1024
+    wl, vox, z = 1.24e-9 / 30.0, 1.38e-6, 0.2
+    theta = phantom.angles(nt)
+    dev = torch.device("cuda", 0)
+    delta_true = phantom.ellipsoids(nz, n, seed=4, device=dev)
+    model = ForwardModel(n, nz, theta, wl, vox, [z])
+    with torch.no_grad():
+        p = model.tomo.R(delta_true)
+        psi = psi_from_proj(p, model.k, 1e-3)          # beta = 1e-3 delta  =>  R beta = 1e-3 R delta
+        inten = abs2(model.prop.D(psi, 0))
+        g = torch.Generator(device=dev).manual_seed(5)
+        data = torch.poisson(inten * 1e4, generator=g) / 1e4
+        del p, psi, inten
+    delta = (0.9 * delta_true).requires_grad_(True)
+    beta = (0.9e-3 * delta_true).requires_grad_(True)
+
+    def it():
+        loss = misfit(delta, beta, model, data)
+        loss.backward()
+        delta.grad = None
+        beta.grad = None
+        return loss
+
+    for _ in range(warmup):
+        it()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = it()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    bdir = b_dir(n, nt)
+    frame = nz * n
+    # 4 R + 2 RT directions, 2 D + 1 DT separable passes (4 x 8 bytes per pixel), 5 element-wise passes
+    alg = 6 * nz * bdir + 3 * nt * 4 * frame * 8 + nt * frame * (8 + 12 + 16 + 16 + 16)
+    model.free()
+    return {"s_per_iter": dt, "loss": float(loss), "config": f"nz={nz} n={n} ntheta={nt} 30keV z=0.2m pixel=1.38um",
+            "algorithmic_GB": alg / 1e9, "hbm_frac": alg / dt / 1e9 / PEAK_HBM_GBS}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--ntheta", type=int, default=2048)
+    ap.add_argument("--no-model", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (R/RT shard over z with no "
+              "communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from model_mosaic_amd import _lib, phantom
+    from model_mosaic_amd.processing.tomo import TomoReal
+
+    lib = _lib.load()
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    nz, n, nt = a.nz, a.n, a.ntheta
+    theta = phantom.angles(nt)
+    print(f"[bench] building phantom {nz}x{n}x{n}, ntheta={nt}", file=sys.stderr, flush=True)
+    obj = phantom.ellipsoids(nz, n, seed=0, noise=0.01, device=dev)
+    t0 = time.perf_counter()
+    tomo = TomoReal(n, nz, theta)
+    plan_s = time.perf_counter() - t0
+    sino = torch.empty((nt, nz, n), dtype=torch.float32, device=dev)
+    back = torch.empty((nz, n, n), dtype=torch.float32, device=dev)
+    print(f"[bench] plan {plan_s:.2f} s, workspace {tomo.workspace_bytes() / 2**30:.2f} GiB; warmup",
+          file=sys.stderr, flush=True)
+
+    def step():
+        tomo.R(obj, out=sino)
+        tomo.RT(sino, out=back)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: per-stage HIP events recorded on the launch stream for every step
+    _lib.check(lib.mm_radon_profile(tomo._plan, 1), "profile")
+    nst = _lib.MM_RADON_NSTAGES
+    acc = np.zeros(nst)
+    buf = (np.ctypeslib.ctypes.c_float * nst)()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+        _lib.check(lib.mm_radon_stage_ms(tomo._plan, buf, nst), "stage_ms")
+        acc += np.array(buf[:])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.check(lib.mm_radon_profile(tomo._plan, 0), "profile")
+    ms_step = elapsed / a.steps * 1e3
+    value = nz * a.steps / elapsed
+    stage_ms = {name: acc[i] / a.steps for i, name in enumerate(_lib.RADON_STAGES)}
+    sb = stage_bytes(n, nt)
+    dom = max(stage_ms, key=stage_ms.get)
+    dom_bytes = sb[dom] * nz
+    dom_ach = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    step_bytes = 2 * nz * b_dir(n, nt)
+    step_ach = step_bytes / (ms_step * 1e-3) / 1e9
+    print(f"[bench] {value:.1f} slices/s, {ms_step:.2f} ms/step; stages " +
+          ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()), file=sys.stderr, flush=True)
+
+    plain = None
+    if not a.no_plain:
+        t1 = TomoReal(n, 1, theta)
+        s1 = torch.empty((nt, 1, n), dtype=torch.float32, device=dev)
+        o1 = torch.empty((1, n, n), dtype=torch.float32, device=dev)
+        k = min(8, nz)
+        for z in range(2):
+            t1.R(obj[z:z + 1], out=s1)
+            t1.RT(s1, out=o1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for z in range(k):
+            t1.R(obj[z:z + 1], out=s1)
+            t1.RT(s1, out=o1)
+        torch.cuda.synchronize()
+        plain = k / (time.perf_counter() - t0)
+        t1.free()
+        print(f"[bench] plain one-slice path {plain:.1f} slices/s", file=sys.stderr, flush=True)
+    tomo.free()
+    del obj, sino, back
+    torch.cuda.empty_cache()
+
+    model = None
+    if not a.no_model:
+        print("[bench] forward+backward model iteration", file=sys.stderr, flush=True)
+        model = run_model(torch)
+        print(f"[bench] model {model['s_per_iter'] * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": "USFFT R+RT slices/s @n=2048 & fwd+bwd model s/iter; % HBM roofline",
+        "value": round(value, 3),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched FP32 USFFT R+RT, seeded ellipsoids + 1% noise",
+                   "nz": nz, "n": n, "ntheta": nt, "oversampling": 2, "m": 4, "eps": 1e-3},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_ach, 2), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(dom_ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": dom_bytes, "avg_ms": round(stage_ms[dom], 4)},
+        "step_roofline": {"algorithmic_bytes": step_bytes, "achieved": round(step_ach, 2), "unit": "GB/s",
+                          "frac": round(step_ach / PEAK_HBM_GBS, 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "plain_one_slice_slices_per_s": None if plain is None else round(plain, 3),
+        "batched_over_plain": None if plain is None else round(value / plain, 3),
+        "model_fwd_bwd": model,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
