@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "mm_fft.h"
@@ -59,10 +60,6 @@ struct Item {
   int start, count;  // samples[start, start + count)
   int slot;          // scatter: -1 = own the tile, >= 0 = partial slot
   int pad;
-};
-
-struct SplitTile {
-  int x0, y0, w, h, slot0, nslots, pad0, pad1;
 };
 
 // ---------------------------------------------------------------------------
@@ -346,131 +343,124 @@ __global__ __launch_bounds__(kThreads) void rt1_line_fft(const void* __restrict_
   }
 }
 
-template <typename Acc>
-__device__ __forceinline__ void lds_add(Acc* T, int idx, float vx, float vy) {
-  atomicAdd(&T[2 * idx], (Acc)vx);
-  atomicAdd(&T[2 * idx + 1], (Acc)vy);
-}
+// ---- adjoint scatter: per-output-point contribution lists (owner computes) --
+// The folded scatter of a representative sample s is rewritten as the direct
+// scatter of two "entries": s itself and its mirror (position -X, -Y, value
+// conj v).  A tap contributes to the half plane iff its raw column
+// (l0 mod 2n) lies in [0, n]; for the mirror entry this selects exactly the
+// folded taps of k0 in {0} U [n, 2n).  At plan time every 16 x 16 output tile
+// gets the list of entries touching it and, for each of its points, the
+// (entry, i0, i1) taps landing there.  A workgroup owns one tile: thread p
+// keeps the list of point p in LDS for the whole z loop, the tile's entry
+// values are staged SZC slices at a time, and each thread accumulates its own
+// point in registers -- no atomics, fixed summation order (bitwise
+// deterministic), one write per output point.  Tiles whose lists exceed the
+// LDS budget (around the origin) are split over entry chunks into partial
+// tiles that scatter_reduce sums.
+constexpr int STS = 16;          // output tile edge
+constexpr int SNP = STS * STS;   // points = threads per work item
+constexpr int SE_MAX = 512;      // entries per work item
+constexpr int SL_MAX = 80;       // list slots per point per work item
+constexpr int SZC = 8;           // z slices per staging round
+constexpr unsigned short SL_EMPTY = 0xFFFF;  // slot = entry | (9 * i1 + i0) << 9
 
-// Tiled adjoint scatter with Hermitian fold into the half plane (see header).
+struct SItem {
+  long long loff;  // lists[loff + j * SNP + p], j < nl: entry | (9 i1 + i0) << 9
+  int eoff, ne;    // entries[eoff, eoff + ne): (sample id << 1) | mirror
+  int x0, y0;      // tile origin (l0, l1), centred coordinates
+  int nl, slot;    // list slots; -1: the item owns the tile, >= 0: partial slot
+};
+
+struct SplitTile {
+  int x0, y0, slot0, nslots;
+};
+
 template <typename Acc>
-__global__ __launch_bounds__(TILE_THREADS) void scatter_tiles(const float2* __restrict__ yh, float2* __restrict__ Hb,
-                                                             float2* __restrict__ partial,
-                                                             const Item* __restrict__ items,
-                                                             const int* __restrict__ samples,
-                                                             const double2* __restrict__ dirs, int n, int H, int nz,
-                                                             int FS, long long hzs, int zchunk, float a) {
-  __shared__ Acc T[2 * ROOT * ROOT];
-  const Item it = items[blockIdx.x];
+__global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ yh, float2* __restrict__ Hb,
+                                                     float2* __restrict__ partial, const SItem* __restrict__ items,
+                                                     const int* __restrict__ entries,
+                                                     const unsigned short* __restrict__ lists,
+                                                     const double2* __restrict__ dirs, int n, int H, int nz, int FS,
+                                                     long long hzs, int zchunk, float a) {
+  __shared__ float2 V[SZC][SE_MAX];
+  __shared__ unsigned short Ls[SL_MAX][SNP];
+  __shared__ float2 F[SE_MAX];
+  __shared__ int2 TK[SE_MAX];
+  const SItem it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
-  const int tw_ = it.w, tsz = it.w * it.h;
-  const int s = threadIdx.x;
-  const bool active = s < it.count;
-  const int twon = 2 * n, mask = twon - 1;
-  float ex[TAPS], ey[TAPS];
-  unsigned colD = 0, colR = 0, okD1 = 0, okR1 = 0;
-  int bx = 0, by = 0, th = 0, k = 0;
-  if (active) {
-    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
-    th = g.th;
-    k = g.k;
-    bx = g.rx - M;
-    by = g.ry - M;
-#pragma unroll
-    for (int i = 0; i < TAPS; ++i) {
-      const float dx = (float)(i - M) - g.fx, dy = (float)(i - M) - g.fy;
-      const float sx = ((bx + i) & 1) ? -1.f : 1.f, sy = ((by + i) & 1) ? -1.f : 1.f;
-      ex[i] = sx * __expf(-a * dx * dx);
-      ey[i] = sy * __expf(-a * dy * dy);
-      // direct target: l0 = bx + i in [0, n] inside the tile
-      const int l0 = bx + i;
-      if (l0 >= 0 && l0 <= n && l0 >= it.x0 && l0 < it.x0 + it.w) colD |= 1u << i;
-      const int wr = ((by + i + n) & mask) - n;
-      if (wr >= it.y0 && wr < it.y0 + it.h) okD1 |= 1u << i;
-      // folded target of k0 in {0} U [n, 2n): column -l0 (l0 <= 0) or 2n - l0 (l0 >= n)
-      if (l0 <= 0 || l0 >= n) {
-        const int c2 = (l0 <= 0) ? -l0 : twon - l0;
-        if (c2 >= it.x0 && c2 < it.x0 + it.w) colR |= 1u << i;
-      }
-      const int wr2 = ((n - (by + i)) & mask) - n;
-      if (wr2 >= it.y0 && wr2 < it.y0 + it.h) okR1 |= 1u << i;
-    }
+  const int p = threadIdx.x;
+  const int l0 = it.x0 + (p % STS), l1 = it.y0 + (p / STS);
+  for (int e = p; e < it.ne; e += SNP) {
+    const int code = entries[it.eoff + e];
+    const SampleGeom g = sample_geom(code >> 1, H, dirs);
+    const bool mir = (code & 1) != 0;
+    F[e] = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
+    TK[e] = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
   }
-  for (int idx = threadIdx.x; idx < tsz; idx += TILE_THREADS) {
-    T[2 * idx] = (Acc)0;
-    T[2 * idx + 1] = (Acc)0;
-  }
+  for (int j = 0; j < it.nl; ++j) Ls[j][p] = lists[it.loff + (long long)j * SNP + p];
   __syncthreads();
-  const bool anyD = (colD != 0) && (okD1 != 0);
-  const bool anyR = (colR != 0) && (okR1 != 0);
-  const int cD0 = bx - it.x0;
-  for (int z = z0; z < z1; ++z) {
-    if (active && (anyD || anyR)) {
-      const float2 v = yh[((long long)th * nz + z) * H + k];
-      if (anyD) {
-#pragma unroll
-        for (int i1 = 0; i1 < TAPS; ++i1) {
-          if (!((okD1 >> i1) & 1u)) continue;
-          const float wx = ey[i1] * v.x, wy = ey[i1] * v.y;
-          const int rb = ((((by + i1 + n) & mask) - n) - it.y0) * tw_ + cD0;
-#pragma unroll
-          for (int i0 = 0; i0 < TAPS; ++i0)
-            if ((colD >> i0) & 1u) lds_add<Acc>(T, rb + i0, ex[i0] * wx, ex[i0] * wy);
-        }
-      }
-      if (anyR) {
-        for (int i1 = 0; i1 < TAPS; ++i1) {
-          if (!((okR1 >> i1) & 1u)) continue;
-          const float wx = ey[i1] * v.x, wy = -ey[i1] * v.y;
-          const int rb = ((((n - (by + i1)) & mask) - n) - it.y0) * tw_ - it.x0;
-#pragma unroll
-          for (int i0 = 0; i0 < TAPS; ++i0) {
-            if (!((colR >> i0) & 1u)) continue;
-            const int l0 = bx + i0;
-            const int c2 = (l0 <= 0) ? -l0 : twon - l0;
-            lds_add<Acc>(T, rb + c2, ex[i0] * wx, ex[i0] * wy);
-          }
-        }
-      }
+  const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
+  const int mask = 2 * n - 1;
+  for (int zc = z0; zc < z1; zc += SZC) {
+    const int nzz = min(SZC, z1 - zc);
+    for (int idx = p; idx < it.ne * nzz; idx += SNP) {
+      const int zz = idx / it.ne, e = idx - zz * it.ne;
+      const int2 tk = TK[e];
+      float2 v = yh[((long long)tk.x * nz + zc + zz) * H + (tk.y & 0x3FFFFFFF)];
+      if (tk.y >> 30) v.y = -v.y;
+      V[zz][e] = v;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < tsz; idx += TILE_THREADS) {
-      const int lr = idx / tw_, lc = idx - lr * tw_;
-      const int l0 = it.x0 + lc;
-      const float2 val = make_float2((float)T[2 * idx], (float)T[2 * idx + 1]);
-      T[2 * idx] = (Acc)0;
-      T[2 * idx + 1] = (Acc)0;
-      if (l0 > n) continue;
-      if (it.slot < 0) {
-        const int k1 = (it.y0 + lr) & mask;
-        Hb[(long long)z * hzs + (long long)k1 * FS + l0] = val;
-      } else {
-        partial[((long long)it.slot * nz + z) * (FLOOR * FLOOR) + idx] = val;
+    Acc ax[SZC], ay[SZC];
+#pragma unroll
+    for (int zz = 0; zz < SZC; ++zz) {
+      ax[zz] = (Acc)0;
+      ay[zz] = (Acc)0;
+    }
+    for (int j = 0; j < it.nl; ++j) {
+      const unsigned u = Ls[j][p];
+      if (u == SL_EMPTY) break;
+      const int e = (int)(u & 0x1FFu), tap = (int)(u >> 9);
+      const int i1 = tap / TAPS, i0 = tap - TAPS * i1;
+      const float2 f = F[e];
+      const float dx = (float)(i0 - M) - f.x, dy = (float)(i1 - M) - f.y;
+      const float w = __expf(-a * (dx * dx + dy * dy));
+#pragma unroll
+      for (int zz = 0; zz < SZC; ++zz) {
+        const float2 v = V[zz][e];
+        ax[zz] += (Acc)(w * v.x);
+        ay[zz] += (Acc)(w * v.y);
+      }
+    }
+    if (l0 <= n) {
+      for (int zz = 0; zz < nzz; ++zz) {
+        const float2 val = make_float2(sgn * (float)ax[zz], sgn * (float)ay[zz]);
+        if (it.slot < 0)
+          Hb[(long long)(zc + zz) * hzs + (long long)(l1 & mask) * FS + l0] = val;
+        else
+          partial[((long long)it.slot * nz + zc + zz) * SNP + p] = val;
       }
     }
     __syncthreads();
   }
 }
 
-// Sum of the partial tiles of the split floor tiles.
-__global__ __launch_bounds__(FLOOR* FLOOR) void scatter_reduce(const float2* __restrict__ partial,
-                                                               float2* __restrict__ Hb,
-                                                               const SplitTile* __restrict__ splits, int n, int nz,
-                                                               int FS, long long hzs) {
+// Sum of the partial tiles of split tiles.
+__global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__ partial, float2* __restrict__ Hb,
+                                                      const SplitTile* __restrict__ splits, int n, int nz, int FS,
+                                                      long long hzs) {
   const SplitTile st = splits[blockIdx.x];
   const int z = blockIdx.y;
-  const int e = threadIdx.x;
-  const int lr = e / st.w, lc = e - lr * st.w;
-  const int l0 = st.x0 + lc;
-  if (lr >= st.h || l0 > n) return;
+  const int p = threadIdx.x;
+  const int l0 = st.x0 + (p % STS), l1 = st.y0 + (p / STS);
+  if (l0 > n) return;
   float2 acc = make_float2(0.f, 0.f);
   for (int sl = st.slot0; sl < st.slot0 + st.nslots; ++sl) {
-    const float2 v = partial[((long long)sl * nz + z) * (FLOOR * FLOOR) + e];
+    const float2 v = partial[((long long)sl * nz + z) * SNP + p];
     acc.x += v.x;
     acc.y += v.y;
   }
-  const int k1 = (st.y0 + lr) & (2 * n - 1);
-  Hb[(long long)z * hzs + (long long)k1 * FS + l0] = acc;
+  Hb[(long long)z * hzs + (long long)(l1 & (2 * n - 1)) * FS + l0] = acc;
 }
 
 // Row pass of RT: C2R inverse transform of the half-spectrum rows n/2 + y
@@ -597,8 +587,6 @@ struct HostGeom {
 struct Binning {
   std::vector<Item> items;
   std::vector<int> samples;
-  std::vector<SplitTile> splits;
-  int nslots = 0;
 };
 
 static bool rect_hits(const Rect& r, int x0, int y0, int w, int h) {
@@ -649,83 +637,158 @@ static void build_gather(const HostGeom& G, int root, Binning& B) {
     for (int tx = 0; tx < nrx; ++tx) gather_node(G, tx * root, ty * root - n, root, roots[(size_t)ty * nrx + tx], B);
 }
 
-// Scatter: samples binned by every output tile their taps touch.
-static void scatter_node(const HostGeom& G, int x0, int y0, int sz, std::vector<int>& ids, Binning& B,
-                         std::vector<Rect>& tmp) {
-  if (x0 > G.n) return;  // entirely right of the half plane
-  if ((int)ids.size() <= SMAX || sz <= FLOOR) {
-    if ((int)ids.size() <= SMAX) {
-      Item it{};
-      it.x0 = x0;
-      it.y0 = y0;
-      it.w = sz;
-      it.h = sz;
-      it.start = (int)B.samples.size();
-      it.count = (int)ids.size();
-      it.slot = -1;
-      B.samples.insert(B.samples.end(), ids.begin(), ids.end());
-      B.items.push_back(it);
-    } else {
-      SplitTile st{};
-      st.x0 = x0;
-      st.y0 = y0;
-      st.w = sz;
-      st.h = sz;
-      st.slot0 = B.nslots;
-      st.nslots = (int)((ids.size() + SMAX - 1) / SMAX);
-      for (int p = 0; p < st.nslots; ++p) {
-        Item it{};
-        it.x0 = x0;
-        it.y0 = y0;
-        it.w = sz;
-        it.h = sz;
-        it.start = (int)B.samples.size();
-        it.count = (int)std::min<size_t>(SMAX, ids.size() - (size_t)p * SMAX);
-        it.slot = B.nslots + p;
-        B.samples.insert(B.samples.end(), ids.begin() + (size_t)p * SMAX, ids.begin() + (size_t)p * SMAX + it.count);
-        B.items.push_back(it);
-      }
-      B.nslots += st.nslots;
-      B.splits.push_back(st);
-    }
+// ---- scatter lists (host, at plan time) -------------------------------------
+
+struct SBinning {
+  std::vector<SItem> items;
+  std::vector<int> entries;
+  std::vector<unsigned short> lists;
+  std::vector<SplitTile> splits;
+  int nslots = 0;
+};
+
+static inline int pmod(int v, int m) {
+  const int r = v % m;
+  return r < 0 ? r + m : r;
+}
+
+// Items of one 16 x 16 output tile (tx, ty) from the entry codes touching it.
+static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vector<int>& codes, SBinning& B) {
+  const int n = G.n, twon = 2 * n;
+  const int x0 = tx * STS, y0 = ty * STS - n;
+  const int ne = (int)codes.size();
+  if (ne == 0) {  // no sample touches the tile: the item writes zeros
+    SItem it{};
+    it.loff = (long long)B.lists.size();
+    it.eoff = (int)B.entries.size();
+    it.x0 = x0;
+    it.y0 = y0;
+    it.slot = -1;
+    B.items.push_back(it);
     return;
   }
-  const int h = sz / 2;
-  for (int c = 0; c < 4; ++c) {
-    const int cx = x0 + (c & 1) * h, cy = y0 + (c >> 1) * h;
-    if (cx > G.n) continue;
-    std::vector<int> sub;
-    for (int id : ids) {
-      G.rects(id, tmp);
-      for (const Rect& r : tmp)
-        if (rect_hits(r, cx, cy, h, h)) {
-          sub.push_back(id);
-          break;
-        }
+  // taps of every entry that land in this tile: (point, i0, i1)
+  std::vector<int> tap_pt;
+  std::vector<unsigned short> tap_code;
+  std::vector<int> tap_off(ne + 1, 0);
+  for (int e = 0; e < ne; ++e) {
+    int rx, ry;
+    G.centre(codes[e] >> 1, rx, ry);
+    const int cx = (codes[e] & 1) ? -rx : rx, cy = (codes[e] & 1) ? -ry : ry;
+    for (int i1 = 0; i1 < TAPS; ++i1) {
+      const int lr = wrapc(cy - M + i1, n) - y0;
+      if (lr < 0 || lr >= STS) continue;
+      for (int i0 = 0; i0 < TAPS; ++i0) {
+        const int raw = pmod(cx - M + i0, twon);
+        const int lc = raw - x0;
+        if (raw > n || lc < 0 || lc >= STS) continue;
+        tap_pt.push_back(lr * STS + lc);
+        tap_code.push_back((unsigned short)((TAPS * i1 + i0) << 9));
+      }
     }
-    scatter_node(G, cx, cy, h, sub, B, tmp);
+    tap_off[e + 1] = (int)tap_pt.size();
+  }
+  // greedy chunks of entries: <= SE_MAX entries, <= SL_MAX taps per point
+  std::vector<int> cnt(SNP, 0);
+  std::vector<std::pair<int, int>> chunks;
+  int e0 = 0;
+  for (int e = 0; e < ne; ++e) {
+    bool fits = (e - e0) < SE_MAX;
+    for (int t = tap_off[e]; fits && t < tap_off[e + 1]; ++t)
+      if (cnt[tap_pt[t]] + 1 > SL_MAX) fits = false;
+    if (!fits) {
+      chunks.push_back({e0, e});
+      std::fill(cnt.begin(), cnt.end(), 0);
+      e0 = e;
+    }
+    for (int t = tap_off[e]; t < tap_off[e + 1]; ++t) cnt[tap_pt[t]]++;
+  }
+  chunks.push_back({e0, ne});
+  const bool split = chunks.size() > 1;
+  int slot0 = -1;
+  if (split) {
+    slot0 = B.nslots;
+    B.nslots += (int)chunks.size();
+    B.splits.push_back({x0, y0, slot0, (int)chunks.size()});
+  }
+  std::vector<std::vector<unsigned short>> L(SNP);
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    const int a = chunks[c].first, b = chunks[c].second;
+    for (auto& l : L) l.clear();
+    for (int e = a; e < b; ++e)
+      for (int t = tap_off[e]; t < tap_off[e + 1]; ++t) L[tap_pt[t]].push_back((unsigned short)((e - a) | tap_code[t]));
+    size_t nl = 0;
+    for (auto& l : L) nl = std::max(nl, l.size());
+    SItem it{};
+    it.x0 = x0;
+    it.y0 = y0;
+    it.ne = b - a;
+    it.eoff = (int)B.entries.size();
+    it.nl = (int)nl;
+    it.loff = (long long)B.lists.size();
+    it.slot = split ? slot0 + (int)c : -1;
+    B.entries.insert(B.entries.end(), codes.begin() + a, codes.begin() + b);
+    for (size_t j = 0; j < nl; ++j)
+      for (int pt = 0; pt < SNP; ++pt) B.lists.push_back(j < L[pt].size() ? L[pt][j] : SL_EMPTY);
+    B.items.push_back(it);
   }
 }
 
-static void build_scatter(const HostGeom& G, int root, Binning& B) {
+static void build_scatter_lists(const HostGeom& G, SBinning& B) {
   const int n = G.n;
-  const int nrx = n / root + 1, nry = 2 * n / root;
-  std::vector<std::vector<int>> roots((size_t)nrx * nry);
-  std::vector<Rect> tmp;
-  std::vector<int> hit;
+  const int ntx = n / STS + 1, nty = 2 * n / STS;
+  std::vector<std::vector<int>> tiles((size_t)ntx * nty);
   const int total = G.ntheta * G.H;
+  std::vector<Rect> rects;
   for (int id = 0; id < total; ++id) {
-    G.rects(id, tmp);
-    hit.clear();
-    for (const Rect& r : tmp)
-      for (int ty = (r.r0 + n) / root; ty <= (r.r1 + n) / root; ++ty)
-        for (int tx = r.c0 / root; tx <= r.c1 / root; ++tx) hit.push_back(ty * nrx + tx);
-    std::sort(hit.begin(), hit.end());
-    hit.erase(std::unique(hit.begin(), hit.end()), hit.end());
-    for (int h : hit) roots[h].push_back(id);
+    int rx, ry;
+    G.centre(id, rx, ry);
+    for (int mir = 0; mir < 2; ++mir) {
+      const int cx = mir ? -rx : rx, cy = mir ? -ry : ry;
+      int c0 = n + 1, c1 = -1;
+      for (int i = 0; i < TAPS; ++i) {
+        const int raw = pmod(cx - M + i, 2 * n);
+        if (raw <= n) {
+          c0 = std::min(c0, raw);
+          c1 = std::max(c1, raw);
+        }
+      }
+      if (c1 < 0) continue;
+      rects.clear();
+      push_wrapped(rects, c0, c1, cy - M, cy + M, n);
+      for (const Rect& r : rects)
+        for (int ty = (r.r0 + n) / STS; ty <= (r.r1 + n) / STS; ++ty)
+          for (int tx = r.c0 / STS; tx <= r.c1 / STS; ++tx) tiles[(size_t)ty * ntx + tx].push_back((id << 1) | mir);
+    }
   }
-  for (int ty = 0; ty < nry; ++ty)
-    for (int tx = 0; tx < nrx; ++tx) scatter_node(G, tx * root, ty * root - n, root, roots[(size_t)ty * nrx + tx], B, tmp);
+  const size_t ntiles = tiles.size();
+  unsigned nw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<SBinning> part(nw);
+  std::vector<std::thread> pool;
+  for (unsigned w = 0; w < nw; ++w)
+    pool.emplace_back([&, w]() {
+      const size_t t0 = ntiles * w / nw, t1 = ntiles * (w + 1) / nw;
+      for (size_t t = t0; t < t1; ++t)
+        scatter_tile_items(G, (int)(t % ntx), (int)(t / ntx), tiles[t], part[w]);
+    });
+  for (auto& th : pool) th.join();
+  for (auto& P : part) {
+    const int eoff = (int)B.entries.size(), soff = B.nslots;
+    const long long loff = (long long)B.lists.size();
+    for (SItem it : P.items) {
+      it.eoff += eoff;
+      it.loff += loff;
+      if (it.slot >= 0) it.slot += soff;
+      B.items.push_back(it);
+    }
+    for (SplitTile s : P.splits) {
+      s.slot0 += soff;
+      B.splits.push_back(s);
+    }
+    B.entries.insert(B.entries.end(), P.entries.begin(), P.entries.end());
+    B.lists.insert(B.lists.end(), P.lists.begin(), P.lists.end());
+    B.nslots += P.nslots;
+  }
 }
 
 template <typename T>
@@ -764,8 +827,9 @@ struct mm_radon {
   Item* gitems = nullptr;
   int* gsamples = nullptr;
   int n_gitems = 0;
-  Item* sitems = nullptr;
-  int* ssamples = nullptr;
+  SItem* sitems = nullptr;
+  int* sentries = nullptr;
+  unsigned short* slists = nullptr;
   int n_sitems = 0;
   SplitTile* splits = nullptr;
   int n_splits = 0, n_slots = 0;
@@ -790,7 +854,7 @@ static void radon_free(mm_radon* p) {
     for (auto& x : e)
       if (x) (void)hipEventDestroy(x);
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
-                  p->gitems, p->gsamples, p->sitems, p->ssamples, p->splits};
+                  p->gitems, p->gsamples, p->sitems, p->sentries, p->slists, p->splits};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -902,17 +966,16 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   const int zblocks = (nz + zchunk - 1) / zchunk;
   const long long hzs = (long long)2 * p->n * p->FS;
   if (p->flags & MM_RADON_FP64_TILES)
-    hipLaunchKernelGGL(scatter_tiles<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(TILE_THREADS), 0,
-                       st, yh, Hb, p->partial, p->sitems, p->ssamples, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk,
-                       p->a);
+    hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
+                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
   else
-    hipLaunchKernelGGL(scatter_tiles<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st,
-                       yh, Hb, p->partial, p->sitems, p->ssamples, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
-  int rc = check_launch("scatter_tiles");
+    hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
+                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
+  int rc = check_launch("scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
-    hipLaunchKernelGGL(scatter_reduce, dim3((unsigned)p->n_splits, (unsigned)nz), dim3(FLOOR * FLOOR), 0, st,
-                       p->partial, Hb, p->splits, p->n, nz, p->FS, hzs);
+    hipLaunchKernelGGL(scatter_reduce, dim3((unsigned)p->n_splits, (unsigned)nz), dim3(SNP), 0, st, p->partial, Hb,
+                       p->splits, p->n, nz, p->FS, hzs);
     rc = check_launch("scatter_reduce");
   }
   return rc;
@@ -987,16 +1050,18 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
     dirs[t] = make_double2(G.dx[t], G.dy[t]);
     sig[t] = sg;
   }
-  Binning BG, BS;
+  Binning BG;
+  SBinning BS;
   const int root = std::min(ROOT, n);
   build_gather(G, root, BG);
-  build_scatter(G, root, BS);
+  build_scatter_lists(G, BS);
 
   size_t& bytes = p->bytes;
   if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
       (rc = upload(&p->sig, sig, bytes)) || (rc = upload(&p->gitems, BG.items, bytes)) ||
       (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->sitems, BS.items, bytes)) ||
-      (rc = upload(&p->ssamples, BS.samples, bytes)) || (rc = upload(&p->splits, BS.splits, bytes)))
+      (rc = upload(&p->sentries, BS.entries, bytes)) || (rc = upload(&p->slists, BS.lists, bytes)) ||
+      (rc = upload(&p->splits, BS.splits, bytes)))
     return bail(rc);
   p->n_gitems = (int)BG.items.size();
   p->n_sitems = (int)BS.items.size();
@@ -1006,7 +1071,7 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
       (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives")))
     return bail(rc);
   if (p->n_slots > 0 &&
-      (rc = dmalloc((void**)&p->partial, sizeof(float2) * (size_t)p->n_slots * nz_max * FLOOR * FLOOR, bytes, "partial tiles")))
+      (rc = dmalloc((void**)&p->partial, sizeof(float2) * (size_t)p->n_slots * nz_max * SNP, bytes, "partial tiles")))
     return bail(rc);
   *plan = p;
   return MM_OK;
