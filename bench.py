@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--no-model", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (PMC byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (R/RT shard over z with no "
@@ -156,6 +158,14 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     nz, n, nt = a.nz, a.n, a.ntheta
+    if a.copy_calib_mib > 0:
+        src = torch.ones(a.copy_calib_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {a.copy_calib_mib} MiB x3 done", file=sys.stderr, flush=True)
+        del src, dst
     theta = phantom.angles(nt)
     print(f"[bench] building phantom {nz}x{n}x{n}, ntheta={nt}", file=sys.stderr, flush=True)
     obj = phantom.ellipsoids(nz, n, seed=0, noise=0.01, device=dev)

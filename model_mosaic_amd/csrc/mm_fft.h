@@ -1,10 +1,9 @@
 // LDS-resident Stockham FFT for gfx950.
 //
-// A workgroup of kThreads = 1024 threads holds C = kBatchElems / L transforms of
-// length L in LDS (column-major: transform c occupies x[c*S .. c*S + L), S = L + 1
-// so that the C transforms start in different banks).  TPC = kThreads / C
-// threads work on one transform; every thread owns E = L / TPC = 16 points per
-// stage.  Stages are radix-8 (radix-4/2 for the leftover factor), each one a
+// A workgroup holds C = BATCH / L transforms of length L in LDS (transform c
+// occupies x[c*S .. c*S + L), the stride S padded so that the C transforms start
+// in different banks).  TPC = L / 16 threads work on one transform; every
+// thread owns 16 points per stage.  Stages are radix-8 (radix-4/2 for the leftover factor), each one a
 // read-all / barrier / twiddle + DFT_R in registers / write / barrier sweep
 // (Stockham autosort: natural order in and out, no bit reversal).  Twiddles come
 // from the FP64-built bank exp(-2 pi i t / L), so a length-L transform has the
@@ -15,18 +14,20 @@
 
 namespace mm {
 
-constexpr int kThreads = 1024;      // threads per FFT workgroup
-constexpr int kBatchElems = 16384;  // complex points per FFT workgroup (128 KiB of LDS)
+constexpr int kThreads = 1024;      // threads of a full-size FFT workgroup
+constexpr int kBatchElems = 16384;  // complex points per full-size FFT workgroup (128 KiB of LDS)
 constexpr int kPerThread = 16;      // points per thread per stage
 
-template <int L>
+// Shape of a workgroup holding BATCH / L transforms of length L.
+template <int L, int BATCH = kBatchElems>
 struct FftShape {
   static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
-  static constexpr int C = kBatchElems / L;  // transforms per workgroup
-  static constexpr int TPC = L / kPerThread; // threads per transform
-  static constexpr int S = L + 1;            // LDS stride between transforms
-  static constexpr int LDS = C * S;          // float2 elements of LDS
-  static_assert(C * TPC == kThreads, "shape");
+  static_assert(BATCH >= L && BATCH % L == 0, "whole transforms per workgroup");
+  static constexpr int C = BATCH / L;                    // transforms per workgroup
+  static constexpr int TPC = L / kPerThread;             // threads per transform
+  static constexpr int NT = C * TPC;                     // threads per workgroup
+  static constexpr int S = L >= 256 ? L + 16 : L + 2;    // LDS stride between transforms (>= L + 2)
+  static constexpr int LDS = C * S;                      // float2 elements of LDS
 };
 
 constexpr int pick_radix(int rem) { return (rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2); }
@@ -100,12 +101,27 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ x, int t, const f
     const int j = t + b * TPC;
     const int k = j % NS;
     if constexpr (NS > 1) {
+      // W^(k r): three table reads, the other powers by products
       constexpr int STEP = L / (NS * R);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw[k * r * STEP];
-        if (INV) w.y = -w.y;
-        v[b][r] = cmul(v[b][r], w);
+      float2 w1 = tw[k * STEP];
+      if (INV) w1.y = -w1.y;
+      if constexpr (R == 2) {
+        v[b][1] = cmul(v[b][1], w1);
+      } else {
+        float2 w2 = tw[2 * k * STEP];
+        if (INV) w2.y = -w2.y;
+        const float2 w3 = cmul(w1, w2);
+        v[b][1] = cmul(v[b][1], w1);
+        v[b][2] = cmul(v[b][2], w2);
+        v[b][3] = cmul(v[b][3], w3);
+        if constexpr (R == 8) {
+          float2 w4 = tw[4 * k * STEP];
+          if (INV) w4.y = -w4.y;
+          v[b][4] = cmul(v[b][4], w4);
+          v[b][5] = cmul(v[b][5], cmul(w1, w4));
+          v[b][6] = cmul(v[b][6], cmul(w2, w4));
+          v[b][7] = cmul(v[b][7], cmul(w3, w4));
+        }
       }
     }
     dft<R, INV>(v[b]);

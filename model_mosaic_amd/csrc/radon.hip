@@ -9,7 +9,8 @@
 //
 // Design (all FP32 arithmetic except the FP64 polar coordinates):
 //  * Real input => Hermitian spectrum.  Only the half plane l0 in [0, n] of the
-//    rfft2 is ever stored (fde: nz x 2n x FS complex, raw fftfreq rows).
+//    rfft2 is ever stored (fde: nz planes of 2n raw fftfreq rows x (n+1) columns,
+//    in blocks of 4 columns, see hs_idx).
 //  * The samples r and n - r of one angle sit at mirrored frequencies, so for
 //    real data g(n - r) = conj(g(r)).  Each angle is therefore gathered at the
 //    H = n/2 + 1 "representative" samples of its half line in the right half
@@ -66,20 +67,28 @@ struct Item {
 // Kernels
 // ---------------------------------------------------------------------------
 
+// Half-spectrum layout: column blocks of 4; inside a block the 2n rows are
+// contiguous, 4 complex values (one per column) each.  A column FFT workgroup
+// then reads one contiguous block, and the row passes read/write 32-byte
+// pieces that are contiguous over consecutive rows.
+__host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon) {
+  return ((long long)(col >> 2) * twon + row) * 4 + (col & 3);
+}
+
 // Row pass of R: two object rows per complex FFT.  Row y of slice z, times
 // phi_y[y] phi_x[x], zero-padded to 2n (object centred), is transformed and
-// the half spectrum kx in [0, n] is written to fde row n/2 + y.
+// the half spectrum kx in [0, n] is written to half-spectrum row n/2 + y.
 template <int L>
 __global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
                                                          const float* __restrict__ phx, const float* __restrict__ phy,
-                                                         int n, int nz, int FS, long long fzs,
-                                                         const float2* __restrict__ tw) {
+                                                         int n, int nz, long long fzs, const float2* __restrict__ tw) {
   using Sh = FftShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const int half = n / 2;
+  const long long npairs = (long long)nz * half;
   const long long pair = (long long)blockIdx.x * Sh::C + c;
-  const bool valid = pair < (long long)nz * half;
+  const bool valid = pair < npairs;
   int z = 0, y0 = 0;
   if (valid) {
     z = (int)(pair / half);
@@ -102,34 +111,48 @@ __global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, false>(x, t, tw);
-  if (valid) {
-    float2* oa = fde + (long long)z * fzs + (long long)(half + y0) * FS;
-    float2* ob = oa + FS;
-    for (int k = t; k <= n; k += Sh::TPC) {
-      const float2 zk = x[k], zm = x[(L - k) & (L - 1)];
-      oa[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-      ob[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-    }
+  // unpack A = (Z_k + conj Z_{L-k})/2 (row y0), B = (Z_k - conj Z_{L-k})/(2i) (row y0+1);
+  // thread order: column within block, then the 2C rows of this workgroup, then block
+  constexpr int R2 = 2 * Sh::C;
+  const int nblk = n / 4 + 1;
+  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += kThreads) {
+    const int q = idx & 3, rest = idx >> 2;
+    const int r = rest % R2, blk = rest / R2;
+    const int col = 4 * blk + q;
+    const int cc = r >> 1;
+    const long long pr = (long long)blockIdx.x * Sh::C + cc;
+    if (col > n || pr >= npairs) continue;
+    const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (r & 1);
+    const float2* xc = lds + cc * Sh::S;
+    const float2 zk = xc[col], zm = xc[(L - col) & (L - 1)];
+    const float2 v = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
+                             : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    fde[(long long)zz * fzs + hs_idx(half + yy, col, 2 * n)] = v;
   }
 }
 
-// Column transform over C adjacent columns of a (rows x FS) complex plane,
-// z-plane = blockIdx.y.  Rows [in_lo, in_hi) are read (others are zero), rows
-// [out_lo, out_hi) are written back in place, times `scale`.
+// Column transform over C adjacent columns of one z plane of the half
+// spectrum (blockIdx.y = z).  Rows [in_lo, in_hi) are read (others are zero),
+// rows [out_lo, out_hi) are written back in place, times `scale`.  Threads run
+// column-in-block fastest, then row, so each block is read and written as one
+// contiguous range.
 template <int L, bool INV>
-__global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, int ncols, int FS, long long zs,
-                                                    int in_lo, int in_hi, int out_lo, int out_hi, float scale,
+__global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, int ncols, long long zs, int in_lo,
+                                                    int in_hi, int out_lo, int out_hi, float scale,
                                                     const float2* __restrict__ tw) {
   using Sh = FftShape<L>;
+  constexpr int CQ = Sh::C < 4 ? Sh::C : 4;
   __shared__ float2 lds[Sh::LDS];
   const int col0 = blockIdx.x * Sh::C;
-  float2* b = buf + (long long)blockIdx.y * zs + col0;
+  float2* b = buf + (long long)blockIdx.y * zs;
 #pragma unroll
   for (int it = 0; it < kPerThread; ++it) {
     const int idx = threadIdx.x + it * kThreads;
-    const int c = idx % Sh::C, i = idx / Sh::C;
+    const int q = idx % CQ, rest = idx / CQ;
+    const int i = rest % L, c = (rest / L) * CQ + q;
+    const int col = col0 + c;
     float2 v = make_float2(0.f, 0.f);
-    if (i >= in_lo && i < in_hi && col0 + c < ncols) v = b[(long long)i * FS + c];
+    if (i >= in_lo && i < in_hi && col < ncols) v = b[hs_idx(i, col, L)];
     lds[c * Sh::S + i] = v;
   }
   __syncthreads();
@@ -138,8 +161,10 @@ __global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, in
 #pragma unroll
   for (int it = 0; it < kPerThread; ++it) {
     const int idx = threadIdx.x + it * kThreads;
-    const int c = idx % Sh::C, i = idx / Sh::C;
-    if (i >= out_lo && i < out_hi && col0 + c < ncols) b[(long long)i * FS + c] = cscale(lds[c * Sh::S + i], scale);
+    const int q = idx % CQ, rest = idx / CQ;
+    const int i = rest % L, c = (rest / L) * CQ + q;
+    const int col = col0 + c;
+    if (i >= out_lo && i < out_hi && col < ncols) b[hs_idx(i, col, L)] = cscale(lds[c * Sh::S + i], scale);
   }
 }
 
@@ -170,12 +195,14 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
                                                             const Item* __restrict__ items,
                                                             const int* __restrict__ samples,
                                                             const double2* __restrict__ dirs, int n, int H, int nz,
-                                                            int FS, long long fzs, int zchunk, float a, float C0) {
+                                                            long long fzs, int zchunk, float a, float C0) {
   __shared__ float2 P[PATCH * PATCH];
   const Item it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
-  const int pw = it.w + 2 * M, ph = it.h + 2 * M, np = pw * ph;
+  const int pw = it.w + 2 * M, ph = it.h + 2 * M;
   const int px0 = it.x0 - M, py0 = it.y0 - M;
+  const int bx0 = px0 & ~3;  // first 4-column block touched by the patch
+  const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
   const int s = threadIdx.x;
   const bool active = s < it.count;
   float ex[TAPS], ey[TAPS];
@@ -195,18 +222,22 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
   const int mask = 2 * n - 1;
   for (int z = z0; z < z1; ++z) {
     const float2* f = fde + (long long)z * fzs;
-    for (int idx = threadIdx.x; idx < np; idx += TILE_THREADS) {
-      const int pr = idx / pw, pc = idx - pr * pw;
-      const int k0 = (px0 + pc) & mask, k1 = (py0 + pr) & mask;
+    for (int idx = threadIdx.x; idx < nstage; idx += TILE_THREADS) {
+      const int q = idx & 3, rest = idx >> 2;
+      const int pr = rest % ph, blk = rest / ph;
+      const int l0 = bx0 + 4 * blk + q;
+      const int pc = l0 - px0;
+      if (pc < 0 || pc >= pw) continue;
+      const int k0 = l0 & mask, k1 = (py0 + pr) & mask;
       float2 v;
       if (k0 <= n) {
-        v = f[(long long)k1 * FS + k0];
+        v = f[hs_idx(k1, k0, 2 * n)];
       } else {
-        v = f[(long long)((2 * n - k1) & mask) * FS + (2 * n - k0)];
+        v = f[hs_idx((2 * n - k1) & mask, 2 * n - k0, 2 * n)];
         v.y = -v.y;
       }
       if ((k0 + k1) & 1) v = make_float2(-v.x, -v.y);
-      P[idx] = v;
+      P[pr * pw + pc] = v;
     }
     __syncthreads();
     if (active) {
@@ -380,7 +411,7 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
                                                      float2* __restrict__ partial, const SItem* __restrict__ items,
                                                      const int* __restrict__ entries,
                                                      const unsigned short* __restrict__ lists,
-                                                     const double2* __restrict__ dirs, int n, int H, int nz, int FS,
+                                                     const double2* __restrict__ dirs, int n, int H, int nz,
                                                      long long hzs, int zchunk, float a) {
   __shared__ float2 V[SZC][SE_MAX];
   __shared__ unsigned short Ls[SL_MAX][SNP];
@@ -436,7 +467,7 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
       for (int zz = 0; zz < nzz; ++zz) {
         const float2 val = make_float2(sgn * (float)ax[zz], sgn * (float)ay[zz]);
         if (it.slot < 0)
-          Hb[(long long)(zc + zz) * hzs + (long long)(l1 & mask) * FS + l0] = val;
+          Hb[(long long)(zc + zz) * hzs + hs_idx(l1 & mask, l0, 2 * n)] = val;
         else
           partial[((long long)it.slot * nz + zc + zz) * SNP + p] = val;
       }
@@ -447,7 +478,7 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
 
 // Sum of the partial tiles of split tiles.
 __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__ partial, float2* __restrict__ Hb,
-                                                      const SplitTile* __restrict__ splits, int n, int nz, int FS,
+                                                      const SplitTile* __restrict__ splits, int n, int nz,
                                                       long long hzs) {
   const SplitTile st = splits[blockIdx.x];
   const int z = blockIdx.y;
@@ -460,60 +491,82 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
     acc.x += v.x;
     acc.y += v.y;
   }
-  Hb[(long long)z * hzs + (long long)(l1 & (2 * n - 1)) * FS + l0] = acc;
+  Hb[(long long)z * hzs + hs_idx(l1 & (2 * n - 1), l0, 2 * n)] = acc;
 }
 
 // Row pass of RT: C2R inverse transform of the half-spectrum rows n/2 + y
 // (after the column IFFT), centre crop, times phi.  Real output packs two rows
 // per complex FFT; complex output packs the real/imaginary channels of one row.
+// The 2C input rows of a workgroup are first staged block-ordered (coalesced):
+// row A of transform c at x_c[0..n], row B at x_c[n+1..2n+1].
 template <int L, bool CPLX>
 __global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restrict__ Hre,
                                                          const float2* __restrict__ Him, void* __restrict__ out_v,
                                                          const float* __restrict__ phx, const float* __restrict__ phy,
-                                                         int n, int nz, int FS, long long hzs, float scale,
+                                                         int n, int nz, long long hzs, float scale,
                                                          const float2* __restrict__ tw) {
   using Sh = FftShape<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const int half = n / 2;
-  const long long unit = (long long)blockIdx.x * Sh::C + c;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
-  const bool valid = unit < nunits;
-  int z = 0, y = 0;
-  if (valid) {
+  constexpr int R2 = 2 * Sh::C;
+  const int nblk = n / 4 + 1;
+  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += kThreads) {
+    const int q = idx & 3, rest = idx >> 2;
+    const int r = rest % R2, blk = rest / R2;
+    const int col = 4 * blk + q;
+    const int cc = r >> 1, which = r & 1;
+    const long long u = (long long)blockIdx.x * Sh::C + cc;
+    if (col > n || u >= nunits) continue;
+    int z, row;
+    const float2* src;
     if (CPLX) {
-      z = (int)(unit / n);
-      y = (int)(unit - (long long)z * n);
+      z = (int)(u / n);
+      row = half + (int)(u - (long long)z * n);
+      src = which ? Him : Hre;
     } else {
-      z = (int)(unit / half);
-      y = 2 * (int)(unit - (long long)z * half);
+      z = (int)(u / half);
+      row = half + 2 * (int)(u - (long long)z * half) + which;
+      src = Hre;
     }
+    lds[cc * Sh::S + which * (n + 1) + col] = src[(long long)z * hzs + hs_idx(row, col, L)];
   }
-  const float2* A = Hre + (long long)z * hzs + (long long)(half + y) * FS;
-  const float2* B = CPLX ? (Him + (long long)z * hzs + (long long)(half + y) * FS) : (A + FS);
+  __syncthreads();
+  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   float2* x = lds + c * Sh::S;
+  float2 zv[kPerThread];
 #pragma unroll
   for (int e = 0; e < kPerThread; ++e) {
     const int kx = t + e * Sh::TPC;
-    float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
-    if (valid) {
-      if (kx <= n) {
-        a = A[kx];
-        b = B[kx];
-        if (kx == 0 || kx == n) {
-          a.y = 0.f;
-          b.y = 0.f;
-        }
-      } else {
-        a = cconj(A[L - kx]);
-        b = cconj(B[L - kx]);
+    float2 a, b;
+    if (kx <= n) {
+      a = x[kx];
+      b = x[n + 1 + kx];
+      if (kx == 0 || kx == n) {
+        a.y = 0.f;
+        b.y = 0.f;
       }
+    } else {
+      a = cconj(x[L - kx]);
+      b = cconj(x[n + 1 + L - kx]);
     }
-    x[kx] = make_float2(a.x - b.y, a.y + b.x);
+    zv[e] = make_float2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) x[t + e * Sh::TPC] = zv[e];
+  __syncthreads();
   fft_lds<L, Sh::TPC, true>(x, t, tw);
-  if (!valid) return;
+  const long long unit = (long long)blockIdx.x * Sh::C + c;
+  if (unit >= nunits) return;
+  int z, y;
+  if (CPLX) {
+    z = (int)(unit / n);
+    y = (int)(unit - (long long)z * n);
+  } else {
+    z = (int)(unit / half);
+    y = 2 * (int)(unit - (long long)z * half);
+  }
 #pragma unroll
   for (int e = 0; e < kPerThread; ++e) {
     const int xx = t + e * Sh::TPC;
@@ -811,11 +864,12 @@ using namespace mm::radon;
 
 struct mm_radon {
   int dev = 0;
-  int n = 0, nz_max = 0, ntheta = 0, H = 0, FS = 0;
+  int n = 0, nz_max = 0, ntheta = 0, H = 0, NCB = 0;
+  long long zs = 0;  // elements per z plane of the half spectrum (NCB blocks x 2n rows x 4)
   unsigned flags = 0;
   double mu = 0;
   float a = 0, C0g = 0, C0s = 0;
-  float2* fde = nullptr;   // nz_max x 2n x FS: R half spectrum / RT channel "re" half plane
+  float2* fde = nullptr;   // nz_max x zs, block layout (hs_idx): R half spectrum / RT channel "re" half plane
   float2* fde2 = nullptr;  // RT channel "im" (complex sinograms, allocated on first use)
   float2* gh = nullptr;    // ntheta x nz_max x H: representatives
   float2* gh2 = nullptr;   // RT channel "im" representatives
@@ -876,9 +930,8 @@ template <int L>
 static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
   using Sh = FftShape<L>;
   const long long units = (long long)nz * (p->n / 2);
-  const long long fzs = (long long)2 * p->n * p->FS;
   hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, obj,
-                     p->fde, p->phx, p->phy, p->n, nz, p->FS, fzs, p->tw + L);
+                     p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + L);
 }
 
 template <int L, bool INV>
@@ -886,9 +939,8 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in
                        float scale, hipStream_t st) {
   using Sh = FftShape<L>;
   const int ncols = p->n + 1;
-  const long long zs = (long long)2 * p->n * p->FS;
   hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(kThreads),
-                     0, st, buf, ncols, p->FS, zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + L);
+                     0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + L);
 }
 
 template <int L>
@@ -916,15 +968,14 @@ template <int L>
 static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
   using Sh = FftShape<L>;
   const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
-  const long long hzs = (long long)2 * p->n * p->FS;
   const dim3 grid((unsigned)((units + Sh::C - 1) / Sh::C));
   const float scale = 1.0f / (2.0f * p->n);
   if (cplx)
     hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
-                       p->n, nz, p->FS, hzs, scale, p->tw + L);
+                       p->n, nz, p->zs, scale, p->tw + L);
   else
     hipLaunchKernelGGL((rt4_rows_c2r<L, false>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
-                       p->n, nz, p->FS, hzs, scale, p->tw + L);
+                       p->n, nz, p->zs, scale, p->tw + L);
 }
 
 #define MM_DISPATCH_L(L_RUNTIME, CALL)              \
@@ -954,9 +1005,8 @@ static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
   const int zsplit = zsplit_for(p->n_gitems, nz);
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
-  const long long fzs = (long long)2 * p->n * p->FS;
   hipLaunchKernelGGL(gather_tiles, dim3((unsigned)p->n_gitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st, p->fde,
-                     p->gh, p->gitems, p->gsamples, p->dirs, p->n, p->H, nz, p->FS, fzs, zchunk, p->a, p->C0g);
+                     p->gh, p->gitems, p->gsamples, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g);
   return check_launch("gather_tiles");
 }
 
@@ -964,18 +1014,18 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   const int zsplit = zsplit_for(p->n_sitems, nz);
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
-  const long long hzs = (long long)2 * p->n * p->FS;
+  const long long hzs = p->zs;
   if (p->flags & MM_RADON_FP64_TILES)
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
-                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
+                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk, p->a);
   else
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
-                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, p->FS, hzs, zchunk, p->a);
+                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk, p->a);
   int rc = check_launch("scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
     hipLaunchKernelGGL(scatter_reduce, dim3((unsigned)p->n_splits, (unsigned)nz), dim3(SNP), 0, st, p->partial, Hb,
-                       p->splits, p->n, nz, p->FS, hzs);
+                       p->splits, p->n, nz, hzs);
     rc = check_launch("scatter_reduce");
   }
   return rc;
@@ -1017,7 +1067,8 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   p->nz_max = nz_max;
   p->ntheta = ntheta;
   p->H = n / 2 + 1;
-  p->FS = n + 2;
+  p->NCB = n / 4 + 1;
+  p->zs = (long long)p->NCB * 2 * n * 4;
   p->flags = flags;
   p->mu = mu;
   p->a = (float)(M_PI * M_PI / (mu * 4.0 * n * n));
@@ -1067,7 +1118,7 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   p->n_sitems = (int)BS.items.size();
   p->n_splits = (int)BS.splits.size();
   p->n_slots = BS.nslots;
-  if ((rc = dmalloc((void**)&p->fde, sizeof(float2) * (size_t)nz_max * 2 * n * p->FS, bytes, "fde")) ||
+  if ((rc = dmalloc((void**)&p->fde, sizeof(float2) * (size_t)nz_max * p->zs, bytes, "fde")) ||
       (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives")))
     return bail(rc);
   if (p->n_slots > 0 &&
@@ -1106,7 +1157,7 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   if (cplx) {
     int rc;
     if (!p->fde2 &&
-        (rc = dmalloc((void**)&p->fde2, sizeof(float2) * (size_t)p->nz_max * 2 * p->n * p->FS, p->bytes, "fde (im)")))
+        (rc = dmalloc((void**)&p->fde2, sizeof(float2) * (size_t)p->nz_max * p->zs, p->bytes, "fde (im)")))
       return rc;
     if (!p->gh2 &&
         (rc = dmalloc((void**)&p->gh2, sizeof(float2) * (size_t)p->ntheta * p->nz_max * p->H, p->bytes, "representatives (im)")))
