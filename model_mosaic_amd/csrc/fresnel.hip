@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ 
       sr = pad_src(i, nz, mirror);
     float2 v = make_float2(0.f, 0.f);
     if (sr >= 0 && col0 + c < n) v = src[(long long)sr * n + c];
-    lds[c * Sh::S + i] = v;
+    lds[c * Sh::S + lpad(i)] = v;
   }
   __syncthreads();
   const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ 
     const int k = t + e * Sh::TPC;
     float2 kk = K[k];
     if (ADJ) kk.y = -kk.y;
-    x[k] = cmul(x[k], kk);
+    x[lpad(k)] = cmul(x[lpad(k)], kk);
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, true>(x, t, tw);
@@ -90,10 +90,10 @@ __global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ 
     const int idx = threadIdx.x + it * kThreads;
     const int c = idx % Sh::C, r = idx / Sh::C;
     if (r >= nz || col0 + c >= n) continue;
-    float2 v = lds[c * Sh::S + r + h];
+    float2 v = lds[c * Sh::S + lpad(r + h)];
     if (ADJ && mirror) {
       const int pr = (r < h) ? (h - 1 - r) : (2 * nz + h - 1 - r);
-      v = cadd(v, lds[c * Sh::S + pr]);
+      v = cadd(v, lds[c * Sh::S + lpad(pr)]);
     }
     dst[(long long)r * n + c] = v;
   }
@@ -101,10 +101,10 @@ __global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ 
 
 // Row pass: rows of length n, C rows per block, nrows = b * nz.
 template <int L, bool ADJ>
-__global__ __launch_bounds__(kThreads) void row_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
+__global__ __launch_bounds__(RowShape<L>::NT) void row_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
                                                      long long nrows, bool mirror, const float2* __restrict__ K,
                                                      const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const long long row = (long long)blockIdx.x * Sh::C + c;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void row_pass(const float2* __restrict__ 
       sx = (i >= h && i < h + n) ? i - h : -1;
     else
       sx = pad_src(i, n, mirror);
-    x[i] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
+    x[lpad(i)] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, false>(x, t, tw);
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kThreads) void row_pass(const float2* __restrict__ 
     const int k = t + e * Sh::TPC;
     float2 kk = K[k];
     if (ADJ) kk.y = -kk.y;
-    x[k] = cmul(x[k], kk);
+    x[lpad(k)] = cmul(x[lpad(k)], kk);
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, true>(x, t, tw);
@@ -139,10 +139,10 @@ __global__ __launch_bounds__(kThreads) void row_pass(const float2* __restrict__ 
   for (int e = 0; e < kPerThread; ++e) {
     const int r = t + e * Sh::TPC;
     if (r >= n) continue;
-    float2 v = x[r + h];
+    float2 v = x[lpad(r + h)];
     if (ADJ && mirror) {
       const int pr = (r < h) ? (h - 1 - r) : (2 * n + h - 1 - r);
-      v = cadd(v, x[pr]);
+      v = cadd(v, x[lpad(pr)]);
     }
     dst[r] = v;
   }
@@ -179,9 +179,9 @@ static void launch_col_pass(const mm_fresnel* p, const float2* in, float2* out, 
 
 template <int L, bool ADJ>
 static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   const long long rows = (long long)nframes * p->nz;
-  hipLaunchKernelGGL((row_pass<L, ADJ>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, in, out,
+  hipLaunchKernelGGL((row_pass<L, ADJ>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, in, out,
                      p->n, rows, p->pad_mode == 0, p->Kx + (size_t)j * L, p->tw + L);
 }
 

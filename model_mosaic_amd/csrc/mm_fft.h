@@ -26,9 +26,20 @@ struct FftShape {
   static constexpr int C = BATCH / L;                    // transforms per workgroup
   static constexpr int TPC = L / kPerThread;             // threads per transform
   static constexpr int NT = C * TPC;                     // threads per workgroup
-  static constexpr int S = L >= 256 ? L + 16 : L + 2;    // LDS stride between transforms (>= L + 2)
+  // LDS stride between transforms: room for lpad(L + 1) plus a bank offset
+  static constexpr int S = L + L / 16 + (L >= 256 ? 16 : 2);
   static constexpr int LDS = C * S;                      // float2 elements of LDS
 };
+
+// LDS index of element i of a transform: one pad slot per 16 complex values,
+// so the stride-R writes of the first Stockham stages hit distinct banks.
+__host__ __device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 4); }
+
+// Row-pass workgroups hold half as many points (64 KiB + pad of LDS), so two of
+// them share a CU and one's global loads overlap the other's LDS stages.
+constexpr int kRowBatch = 8192;
+template <int L>
+using RowShape = FftShape<L, (L > kRowBatch ? L : kRowBatch)>;
 
 constexpr int pick_radix(int rem) { return (rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2); }
 
@@ -93,7 +104,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ x, int t, const f
   for (int b = 0; b < BPT; ++b) {
     const int j = t + b * TPC;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = x[j + r * NB];
+    for (int r = 0; r < R; ++r) v[b][r] = x[lpad(j + r * NB)];
   }
   __syncthreads();
 #pragma unroll
@@ -127,12 +138,12 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ x, int t, const f
     dft<R, INV>(v[b]);
     const int base = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[base + r * NS] = v[b][r];
+    for (int r = 0; r < R; ++r) x[lpad(base + r * NS)] = v[b][r];
   }
   __syncthreads();
 }
 
-// Full in-place transform of x[0..L) by the TPC threads with index t.
+// Full in-place transform of x[lpad(0..L)) by the TPC threads with index t.
 // Must be reached by every thread of the workgroup (it contains barriers).
 // Caller synchronises before the first stage reads x.  Unnormalised in both
 // directions (INV only conjugates the kernel).

@@ -53,7 +53,15 @@ constexpr int ROOT = 64;         // quadtree root tile
 constexpr int FLOOR = 8;         // quadtree floor tile
 constexpr int TILE_THREADS = 1024;
 constexpr int SMAX = TILE_THREADS;  // samples per tile work item
-constexpr int PATCH = ROOT + 2 * M;  // gather LDS patch edge
+constexpr int PATCH = ROOT + 2 * M;  // gather LDS patch edge (rows)
+
+// Gather patch pitch (complex values per LDS row) for tile widths 64/32/16/8:
+// chosen by simulating the ds_read_b64 bank pattern of the 81-tap loop over
+// polar sample sets (sorted by angle, then radius), for 32 and 64 banks.
+__host__ __device__ __forceinline__ int patch_pitch(int w) {
+  return w >= 64 ? 75 : (w >= 32 ? 43 : (w >= 16 ? 25 : 37));
+}
+constexpr int PATCH_ELEMS = (ROOT + 2 * M) * 75;
 
 struct Item {
   int x0, y0;        // tile origin (l0, l1), centred coordinates
@@ -79,10 +87,10 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
 // phi_y[y] phi_x[x], zero-padded to 2n (object centred), is transformed and
 // the half spectrum kx in [0, n] is written to half-spectrum row n/2 + y.
 template <int L>
-__global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
+__global__ __launch_bounds__(RowShape<L>::NT) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
                                                          const float* __restrict__ phx, const float* __restrict__ phy,
                                                          int n, int nz, long long fzs, const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const int half = n / 2;
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict
       const float p = phx[xi];
       v = make_float2(ra[xi] * pya * p, rb[xi] * pyb * p);
     }
-    x[i] = v;
+    x[lpad(i)] = v;
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, false>(x, t, tw);
@@ -115,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict
   // thread order: column within block, then the 2C rows of this workgroup, then block
   constexpr int R2 = 2 * Sh::C;
   const int nblk = n / 4 + 1;
-  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
@@ -124,7 +132,7 @@ __global__ __launch_bounds__(kThreads) void r1_rows_rfft(const float* __restrict
     if (col > n || pr >= npairs) continue;
     const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (r & 1);
     const float2* xc = lds + cc * Sh::S;
-    const float2 zk = xc[col], zm = xc[(L - col) & (L - 1)];
+    const float2 zk = xc[lpad(col)], zm = xc[lpad((L - col) & (L - 1))];
     const float2 v = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
                              : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
     fde[(long long)zz * fzs + hs_idx(half + yy, col, 2 * n)] = v;
@@ -153,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, in
     const int col = col0 + c;
     float2 v = make_float2(0.f, 0.f);
     if (i >= in_lo && i < in_hi && col < ncols) v = b[hs_idx(i, col, L)];
-    lds[c * Sh::S + i] = v;
+    lds[c * Sh::S + lpad(i)] = v;
   }
   __syncthreads();
   const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, in
     const int q = idx % CQ, rest = idx / CQ;
     const int i = rest % L, c = (rest / L) * CQ + q;
     const int col = col0 + c;
-    if (i >= out_lo && i < out_hi && col < ncols) b[hs_idx(i, col, L)] = cscale(lds[c * Sh::S + i], scale);
+    if (i >= out_lo && i < out_hi && col < ncols) b[hs_idx(i, col, L)] = cscale(lds[c * Sh::S + lpad(i)], scale);
   }
 }
 
@@ -196,10 +204,11 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
                                                             const int* __restrict__ samples,
                                                             const double2* __restrict__ dirs, int n, int H, int nz,
                                                             long long fzs, int zchunk, float a, float C0) {
-  __shared__ float2 P[PATCH * PATCH];
+  __shared__ float2 P[PATCH_ELEMS];
   const Item it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
   const int pw = it.w + 2 * M, ph = it.h + 2 * M;
+  const int pp = patch_pitch(it.w);  // LDS row pitch >= pw
   const int px0 = it.x0 - M, py0 = it.y0 - M;
   const int bx0 = px0 & ~3;  // first 4-column block touched by the patch
   const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
@@ -211,7 +220,7 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
     const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
     th = g.th;
     k = g.k;
-    base = (g.ry - it.y0) * pw + (g.rx - it.x0);
+    base = (g.ry - it.y0) * pp + (g.rx - it.x0);
 #pragma unroll
     for (int i = 0; i < TAPS; ++i) {
       const float dx = (float)(i - M) - g.fx, dy = (float)(i - M) - g.fy;
@@ -237,18 +246,18 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
         v.y = -v.y;
       }
       if ((k0 + k1) & 1) v = make_float2(-v.x, -v.y);
-      P[pr * pw + pc] = v;
+      P[pr * pp + pc] = v;
     }
     __syncthreads();
     if (active) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int i1 = 0; i1 < TAPS; ++i1) {
-        const float2* pp = P + base + i1 * pw;
+        const float2* prow = P + base + i1 * pp;
         float2 row = make_float2(0.f, 0.f);
 #pragma unroll
         for (int i0 = 0; i0 < TAPS; ++i0) {
-          const float2 v = pp[i0];
+          const float2 v = prow[i0];
           row.x = fmaf(ex[i0], v.x, row.x);
           row.y = fmaf(ex[i0], v.y, row.y);
         }
@@ -264,10 +273,10 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
 // Line pass of R: rebuild the full line g[r] from its representatives,
 // sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).
 template <int L>
-__global__ __launch_bounds__(kThreads) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
+__global__ __launch_bounds__(RowShape<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
                                                          const int* __restrict__ sig, int H, long long nrows, int nz,
                                                          float scale, const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const long long row = (long long)blockIdx.x * Sh::C + c;
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void r4_line_ifft(const float2* __restric
     float2 v = valid ? src[kk] : make_float2(0.f, 0.f);
     if (cj) v.y = -v.y;
     if (!(r & 1)) v = make_float2(-v.x, -v.y);
-    x[r] = v;
+    x[lpad(r)] = v;
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, true>(x, t, tw);
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void r4_line_ifft(const float2* __restric
 #pragma unroll
     for (int e = 0; e < kPerThread; ++e) {
       const int s = t + e * Sh::TPC;
-      const float v = x[s].x * scale;
+      const float v = x[lpad(s)].x * scale;
       dst[s] = (s & 1) ? v : -v;
     }
   }
@@ -311,11 +320,11 @@ __global__ __launch_bounds__(kThreads) void r4_line_ifft(const float2* __restric
 // with the fold weights (1/2 at k = 0 and k = n/2) times `scale`.  A complex
 // sinogram yields the two channels of its real and imaginary parts.
 template <int L, bool CPLX>
-__global__ __launch_bounds__(kThreads) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
+__global__ __launch_bounds__(RowShape<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
                                                          float2* __restrict__ yim, const int* __restrict__ sig, int H,
                                                          long long nrows, int nz, float scale,
                                                          const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
   const long long row = (long long)blockIdx.x * Sh::C + c;
@@ -333,13 +342,13 @@ __global__ __launch_bounds__(kThreads) void rt1_line_fft(const void* __restrict_
     else
       v = make_float2(valid ? ((const float*)sino_v)[rr * L + r] : 0.f, 0.f);
     if (!(r & 1)) v = make_float2(-v.x, -v.y);
-    x[r] = v;
+    x[lpad(r)] = v;
   }
   __syncthreads();
   fft_lds<L, Sh::TPC, false>(x, t, tw);
   if (!valid) return;  // no barrier below
   auto yhat = [&](int r) {
-    float2 v = x[r];
+    float2 v = x[lpad(r)];
     return (r & 1) ? v : make_float2(-v.x, -v.y);
   };
   for (int k = t; k < H; k += Sh::TPC) {
@@ -500,18 +509,18 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
 // The 2C input rows of a workgroup are first staged block-ordered (coalesced):
 // row A of transform c at x_c[0..n], row B at x_c[n+1..2n+1].
 template <int L, bool CPLX>
-__global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restrict__ Hre,
+__global__ __launch_bounds__(RowShape<L>::NT) void rt4_rows_c2r(const float2* __restrict__ Hre,
                                                          const float2* __restrict__ Him, void* __restrict__ out_v,
                                                          const float* __restrict__ phx, const float* __restrict__ phy,
                                                          int n, int nz, long long hzs, float scale,
                                                          const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int half = n / 2;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
   constexpr int R2 = 2 * Sh::C;
   const int nblk = n / 4 + 1;
-  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restric
       row = half + 2 * (int)(u - (long long)z * half) + which;
       src = Hre;
     }
-    lds[cc * Sh::S + which * (n + 1) + col] = src[(long long)z * hzs + hs_idx(row, col, L)];
+    lds[cc * Sh::S + lpad(which * (n + 1) + col)] = src[(long long)z * hzs + hs_idx(row, col, L)];
   }
   __syncthreads();
   const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
@@ -540,21 +549,21 @@ __global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restric
     const int kx = t + e * Sh::TPC;
     float2 a, b;
     if (kx <= n) {
-      a = x[kx];
-      b = x[n + 1 + kx];
+      a = x[lpad(kx)];
+      b = x[lpad(n + 1 + kx)];
       if (kx == 0 || kx == n) {
         a.y = 0.f;
         b.y = 0.f;
       }
     } else {
-      a = cconj(x[L - kx]);
-      b = cconj(x[n + 1 + L - kx]);
+      a = cconj(x[lpad(L - kx)]);
+      b = cconj(x[lpad(n + 1 + L - kx)]);
     }
     zv[e] = make_float2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) x[t + e * Sh::TPC] = zv[e];
+  for (int e = 0; e < kPerThread; ++e) x[lpad(t + e * Sh::TPC)] = zv[e];
   __syncthreads();
   fft_lds<L, Sh::TPC, true>(x, t, tw);
   const long long unit = (long long)blockIdx.x * Sh::C + c;
@@ -571,7 +580,7 @@ __global__ __launch_bounds__(kThreads) void rt4_rows_c2r(const float2* __restric
   for (int e = 0; e < kPerThread; ++e) {
     const int xx = t + e * Sh::TPC;
     if (xx >= n) continue;
-    const float2 v = cscale(x[xx + half], scale);
+    const float2 v = cscale(x[lpad(xx + half)], scale);
     const float px = phx[xx];
     if (CPLX) {
       const float p = phy[y] * px;
@@ -928,9 +937,9 @@ static int dmalloc(void** p, size_t nb, size_t& bytes, const char* what) {
 
 template <int L>
 static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   const long long units = (long long)nz * (p->n / 2);
-  hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, obj,
+  hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, obj,
                      p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + L);
 }
 
@@ -945,36 +954,36 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in
 
 template <int L>
 static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   const long long rows = (long long)p->ntheta * nz;
-  hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(kThreads), 0, st, p->gh,
+  hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
                      sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + L);
 }
 
 template <int L>
 static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   const long long rows = (long long)p->ntheta * nz;
   const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
   if (cplx)
-    hipLaunchKernelGGL((rt1_line_fft<L, true>), grid, dim3(kThreads), 0, st, sino, p->gh, p->gh2, p->sig, p->H, rows,
+    hipLaunchKernelGGL((rt1_line_fft<L, true>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H, rows,
                        nz, p->C0s, p->tw + L);
   else
-    hipLaunchKernelGGL((rt1_line_fft<L, false>), grid, dim3(kThreads), 0, st, sino, p->gh, p->gh2, p->sig, p->H,
+    hipLaunchKernelGGL((rt1_line_fft<L, false>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H,
                        rows, nz, p->C0s, p->tw + L);
 }
 
 template <int L>
 static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = RowShape<L>;
   const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
   const dim3 grid((unsigned)((units + Sh::C - 1) / Sh::C));
   const float scale = 1.0f / (2.0f * p->n);
   if (cplx)
-    hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
+    hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(Sh::NT), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
                        p->n, nz, p->zs, scale, p->tw + L);
   else
-    hipLaunchKernelGGL((rt4_rows_c2r<L, false>), grid, dim3(kThreads), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
+    hipLaunchKernelGGL((rt4_rows_c2r<L, false>), grid, dim3(Sh::NT), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
                        p->n, nz, p->zs, scale, p->tw + L);
 }
 
