@@ -156,4 +156,188 @@ __device__ __forceinline__ void fft_lds(float2* __restrict__ x, int t, const flo
   }
 }
 
+// ---- register-resident Stockham FFT -----------------------------------------
+// Thread t (0 <= t < TPC = L / 16) of a transform holds v[m] = x[t + m TPC],
+// m < 16, before and after the transform.  A radix-R pass runs the thread's
+// 16 / R butterflies j = t + b TPC on its own registers (butterfly j reads
+// x[j + s L / R] = v[b + s 16 / R]); its outputs go to LDS at their Stockham
+// positions and come back in natural order for the next pass.  The last
+// pass's Stockham positions are the natural ones, so its outputs stay in
+// registers: a length 16^a r transform (r in {1, 2, 4, 8}) costs a + [r > 1] - 1
+// LDS exchanges, against one LDS round trip per radix-8 stage plus the
+// global <-> LDS staging of fft_lds.  Global loads and stores of v are
+// therefore direct (element t + m TPC: consecutive t, consecutive addresses).
+
+// Workgroup shape: C transforms of length L, stride S between them in LDS.
+// S = 16 (mod 32) complex values, with the thread mapping of map(), keeps the
+// Stockham writes and natural reads free of bank conflicts (two transforms
+// per 32-lane half-wave at a 16-bank-pair offset).
+template <int L, int BATCH>
+struct RegShape {
+  static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
+  static constexpr int C = BATCH > L ? BATCH / L : 1;
+  static constexpr int TPC = L / kPerThread;
+  static constexpr int NT = C * TPC;
+  static constexpr int S0 = L + L / 16;                       // > lpad(L - 1)
+  static constexpr int S = TPC < 16 ? S0 : S0 + ((16 - S0 % 32) + 32) % 32;  // S = 16 (mod 32)
+  static constexpr int LDS = C * S;
+  // Thread -> (transform c, index t).  For TPC >= 16 a wave holds 16
+  // consecutive t of min(C, 4) transforms, so loads of interleaved columns
+  // (4 per 32-byte row segment) are contiguous across the wave.
+  __device__ static __forceinline__ void map(int tid, int& c, int& t) {
+    if constexpr (TPC < 16) {
+      c = tid / TPC;
+      t = tid % TPC;
+    } else {
+      constexpr int G = C < 4 ? C : 4, PER = TPC / 16;
+      const int g = tid >> 4, tg = g / G;
+      c = g % G + G * (tg / PER);
+      t = (tid & 15) + 16 * (tg % PER);
+    }
+  }
+};
+
+// W16^e times a (forward kernel exp(-2 pi i e / 16); INV conjugates).
+template <int E, bool INV>
+__host__ __device__ __forceinline__ float2 w16(float2 a) {
+  constexpr float C1 = 0.92387953251128675613f, S1 = 0.38268343236508977173f, R2 = 0.70710678118654752440f;
+  constexpr int e = E % 16;
+  float wr, wi;
+  if constexpr (e == 0) return a;
+  else if constexpr (e == 4) return mul_mi<INV>(a);
+  else {
+    if constexpr (e == 1) { wr = C1; wi = -S1; }
+    else if constexpr (e == 2) { wr = R2; wi = -R2; }
+    else if constexpr (e == 3) { wr = S1; wi = -C1; }
+    else if constexpr (e == 6) { wr = -R2; wi = -R2; }
+    else { static_assert(e == 9, "exponents of the 4 x 4 split"); wr = -C1; wi = S1; }
+    if (INV) wi = -wi;
+    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+  }
+}
+
+// 16-point DFT as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
+template <bool INV>
+__host__ __device__ __forceinline__ void dft16(float2* a) {
+  float2 b[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 t0 = a[n2], t1 = a[4 + n2], t2 = a[8 + n2], t3 = a[12 + n2];
+    dft4<INV>(t0, t1, t2, t3);
+    b[n2][0] = t0;
+    b[n2][1] = t1;
+    b[n2][2] = t2;
+    b[n2][3] = t3;
+  }
+  b[1][1] = w16<1, INV>(b[1][1]);
+  b[1][2] = w16<2, INV>(b[1][2]);
+  b[1][3] = w16<3, INV>(b[1][3]);
+  b[2][1] = w16<2, INV>(b[2][1]);
+  b[2][2] = w16<4, INV>(b[2][2]);
+  b[2][3] = w16<6, INV>(b[2][3]);
+  b[3][1] = w16<3, INV>(b[3][1]);
+  b[3][2] = w16<6, INV>(b[3][2]);
+  b[3][3] = w16<9, INV>(b[3][3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 u0 = b[0][k1], u1 = b[1][k1], u2 = b[2][k1], u3 = b[3][k1];
+    dft4<INV>(u0, u1, u2, u3);
+    a[k1] = u0;
+    a[k1 + 4] = u1;
+    a[k1 + 8] = u2;
+    a[k1 + 12] = u3;
+  }
+}
+
+// u[s] *= W_{NS R}^{s k}, s < R: table reads for the powers 1, 2, 4, 8, the
+// rest as products of at most three of them.
+template <int L, int R, int NS, bool INV>
+__device__ __forceinline__ void twiddle(float2* u, int k, const float2* __restrict__ tw) {
+  constexpr int STEP = L / (NS * R);
+  auto ld = [&](int e) {
+    float2 w = tw[e * k * STEP];
+    if (INV) w.y = -w.y;
+    return w;
+  };
+  const float2 w1 = ld(1);
+  if constexpr (R == 2) {
+    u[1] = cmul(u[1], w1);
+  } else {
+    const float2 w2 = ld(2), w3 = cmul(w1, w2);
+    u[1] = cmul(u[1], w1);
+    u[2] = cmul(u[2], w2);
+    u[3] = cmul(u[3], w3);
+    if constexpr (R >= 8) {
+      const float2 w4 = ld(4), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+      u[4] = cmul(u[4], w4);
+      u[5] = cmul(u[5], w5);
+      u[6] = cmul(u[6], w6);
+      u[7] = cmul(u[7], w7);
+      if constexpr (R == 16) {
+        const float2 w8 = ld(8);
+        u[8] = cmul(u[8], w8);
+        u[9] = cmul(u[9], cmul(w1, w8));
+        u[10] = cmul(u[10], cmul(w2, w8));
+        u[11] = cmul(u[11], cmul(w3, w8));
+        u[12] = cmul(u[12], cmul(w4, w8));
+        u[13] = cmul(u[13], cmul(w5, w8));
+        u[14] = cmul(u[14], cmul(w6, w8));
+        u[15] = cmul(u[15], cmul(w7, w8));
+      }
+    }
+  }
+}
+
+// One radix-R pass of fft_reg (NS = product of the radices already applied).
+template <int L, int R, int NS, int TPC, bool INV>
+__device__ __forceinline__ void reg_pass(float2 (&v)[kPerThread], float2* __restrict__ x, int t,
+                                         const float2* __restrict__ tw) {
+  constexpr int BPT = kPerThread / R;
+  constexpr bool LAST = NS * R == L;
+  float2 u[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = t + b * TPC;
+#pragma unroll
+    for (int s = 0; s < R; ++s) u[b][s] = v[b + s * BPT];
+    if constexpr (NS > 1) twiddle<L, R, NS, INV>(u[b], j & (NS - 1), tw);
+    if constexpr (R == 16)
+      dft16<INV>(u[b]);
+    else
+      dft<R, INV>(u[b]);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b)
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[b + q * BPT] = u[b][q];
+  } else {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = t + b * TPC;
+      const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+      for (int q = 0; q < R; ++q) x[lpad(base + q * NS)] = u[b][q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) v[m] = x[lpad(t + m * TPC)];
+    __syncthreads();
+  }
+}
+
+// Transform of the 16 points each of the TPC threads of one transform holds in
+// v (natural order in and out), x = that transform's LDS (RegShape stride).
+// Contains barriers when L > 16: every thread of the workgroup must call it.
+// Unnormalised in both directions (INV only conjugates the kernel).
+template <int L, int TPC, bool INV, int NS = 1>
+__device__ __forceinline__ void fft_reg(float2 (&v)[kPerThread], float2* __restrict__ x, int t,
+                                        const float2* __restrict__ tw) {
+  if constexpr (NS < L) {
+    constexpr int R = L / NS >= 16 ? 16 : L / NS;
+    reg_pass<L, R, NS, TPC, INV>(v, x, t, tw);
+    fft_reg<L, TPC, INV, NS * R>(v, x, t, tw);
+  }
+}
+
 }  // namespace mm

@@ -141,38 +141,36 @@ __global__ __launch_bounds__(RowShape<L>::NT) void r1_rows_rfft(const float* __r
 
 // Column transform over C adjacent columns of one z plane of the half
 // spectrum (blockIdx.y = z).  Rows [in_lo, in_hi) are read (others are zero),
-// rows [out_lo, out_hi) are written back in place, times `scale`.  Threads run
-// column-in-block fastest, then row, so each block is read and written as one
-// contiguous range.
+// rows [out_lo, out_hi) are written back in place, times `scale`.  The
+// register FFT's points t + m TPC of the 4 columns of a block are contiguous
+// (16 rows x 4 columns per wave and m), so loads and stores go straight
+// between HBM and registers.
+template <int L>
+using ColShape = RegShape<L, kBatchElems>;
+
 template <int L, bool INV>
-__global__ __launch_bounds__(kThreads) void col_fft(float2* __restrict__ buf, int ncols, long long zs, int in_lo,
-                                                    int in_hi, int out_lo, int out_hi, float scale,
-                                                    const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
-  constexpr int CQ = Sh::C < 4 ? Sh::C : 4;
+__global__ __launch_bounds__(ColShape<L>::NT) void col_fft(float2* __restrict__ buf, int ncols,
+                                                                      long long zs, int in_lo, int in_hi,
+                                                                      int out_lo, int out_hi, float scale,
+                                                                      const float2* __restrict__ tw) {
+  using Sh = ColShape<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int col0 = blockIdx.x * Sh::C;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const int col = blockIdx.x * Sh::C + c;
+  const bool live = col < ncols;
   float2* b = buf + (long long)blockIdx.y * zs;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it) {
-    const int idx = threadIdx.x + it * kThreads;
-    const int q = idx % CQ, rest = idx / CQ;
-    const int i = rest % L, c = (rest / L) * CQ + q;
-    const int col = col0 + c;
-    float2 v = make_float2(0.f, 0.f);
-    if (i >= in_lo && i < in_hi && col < ncols) v = b[hs_idx(i, col, L)];
-    lds[c * Sh::S + lpad(i)] = v;
+  for (int m = 0; m < kPerThread; ++m) {
+    const int i = t + m * Sh::TPC;
+    v[m] = (live && i >= in_lo && i < in_hi) ? b[hs_idx(i, col, L)] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
-  fft_lds<L, Sh::TPC, INV>(lds + cc * Sh::S, t, tw);
+  fft_reg<L, Sh::TPC, INV>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it) {
-    const int idx = threadIdx.x + it * kThreads;
-    const int q = idx % CQ, rest = idx / CQ;
-    const int i = rest % L, c = (rest / L) * CQ + q;
-    const int col = col0 + c;
-    if (i >= out_lo && i < out_hi && col < ncols) b[hs_idx(i, col, L)] = cscale(lds[c * Sh::S + lpad(i)], scale);
+  for (int m = 0; m < kPerThread; ++m) {
+    const int i = t + m * Sh::TPC;
+    if (live && i >= out_lo && i < out_hi) b[hs_idx(i, col, L)] = cscale(v[m], scale);
   }
 }
 
@@ -395,35 +393,50 @@ __global__ __launch_bounds__(RowShape<L>::NT) void rt1_line_fft(const void* __re
 // values are staged SZC slices at a time, and each thread accumulates its own
 // point in registers -- no atomics, fixed summation order (bitwise
 // deterministic), one write per output point.  Tiles whose lists exceed the
-// LDS budget (around the origin) are split over entry chunks into partial
-// tiles that scatter_reduce sums.
+// per-item budget (around the origin) are split over entry chunks into
+// partial tiles that scatter_reduce sums.
+//
+// A point's list is private to its thread, so it lives in registers: two
+// 16-bit slots per VGPR, in a 32 + 8 element vector indexed by the (uniform)
+// loop counter, which the compiler lowers to relative register addressing.
+// LDS only holds the shared entry data, which keeps four work items resident
+// per CU.  Each wave runs to the longest list among its own 64 points (wnp);
+// shorter lists are padded with SL_EMPTY, which decodes to entry SE_ZERO, an
+// entry whose staged values are zero, so padded slots add exact zeros.
 constexpr int STS = 16;          // output tile edge
 constexpr int SNP = STS * STS;   // points = threads per work item
-constexpr int SE_MAX = 512;      // entries per work item
+constexpr int SWAVES = SNP / 64; // waves per work item
+constexpr int SE_MAX = 512;      // entry rows in LDS per work item (the last is the zero entry)
+constexpr int SE_ZERO = SE_MAX - 1;
 constexpr int SL_MAX = 80;       // list slots per point per work item
+constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
-constexpr unsigned short SL_EMPTY = 0xFFFF;  // slot = entry | (9 * i1 + i0) << 9
+constexpr unsigned short SL_EMPTY = 0xFFFF;  // slot = entry | (9 * i1 + i0) << 9; 0xFFFF -> entry SE_ZERO
+static_assert((SL_EMPTY & 0x1FF) == SE_ZERO && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
+              "slot encoding");
 
 struct SItem {
-  long long loff;  // lists[loff + j * SNP + p], j < nl: entry | (9 i1 + i0) << 9
-  int eoff, ne;    // entries[eoff, eoff + ne): (sample id << 1) | mirror
+  long long loff;  // pairs of slots: lists[loff + (j / 2) * 2 SNP + 2 p + (j & 1)], j < nl
+  int eoff, ne;    // entries[eoff, eoff + ne): (sample id << 1) | mirror, ne < SE_MAX
   int x0, y0;      // tile origin (l0, l1), centred coordinates
-  int nl, slot;    // list slots; -1: the item owns the tile, >= 0: partial slot
+  int nl, slot;    // list slots (even); -1: the item owns the tile, >= 0: partial slot
+  unsigned wnp;     // slot pairs wave w runs (<= nl / 2): bits [8 w, 8 w + 8)
+  int pad_;
 };
 
 struct SplitTile {
   int x0, y0, slot0, nslots;
 };
 
+typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
+typedef unsigned u32x8 __attribute__((ext_vector_type(SL_PAIRS_B)));
+
 template <typename Acc>
-__global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ yh, float2* __restrict__ Hb,
-                                                     float2* __restrict__ partial, const SItem* __restrict__ items,
-                                                     const int* __restrict__ entries,
-                                                     const unsigned short* __restrict__ lists,
-                                                     const double2* __restrict__ dirs, int n, int H, int nz,
-                                                     long long hzs, int zchunk, float a) {
+__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
+    const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
+    const SItem* __restrict__ items, const int* __restrict__ entries, const unsigned short* __restrict__ lists,
+    const double2* __restrict__ dirs, int n, int H, int nz, long long hzs, int zchunk, float a) {
   __shared__ float2 V[SZC][SE_MAX];
-  __shared__ unsigned short Ls[SL_MAX][SNP];
   __shared__ float2 F[SE_MAX];
   __shared__ int2 TK[SE_MAX];
   const SItem it = items[blockIdx.x];
@@ -437,18 +450,35 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
     F[e] = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
     TK[e] = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
   }
-  for (int j = 0; j < it.nl; ++j) Ls[j][p] = lists[it.loff + (long long)j * SNP + p];
+  if (p < SZC) V[p][SE_ZERO] = make_float2(0.f, 0.f);
+  if (p == SZC) F[SE_ZERO] = make_float2(0.f, 0.f);
+  const int npairs = __builtin_amdgcn_readfirstlane((int)((it.wnp >> (8 * (p >> 6))) & 0xFFu));
+  const int na = min(npairs, SL_PAIRS_A);
+  const unsigned* lp = reinterpret_cast<const unsigned*>(lists + it.loff) + p;
+  u32x32 la;
+  u32x8 lb;
+#pragma unroll
+  for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0xFFFFFFFFu;
+#pragma unroll
+  for (int jp = 0; jp < SL_PAIRS_B; ++jp)
+    lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0xFFFFFFFFu;
   __syncthreads();
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
   const int mask = 2 * n - 1;
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
-    for (int idx = p; idx < it.ne * nzz; idx += SNP) {
-      const int zz = idx / it.ne, e = idx - zz * it.ne;
+    for (int e = p; e < it.ne; e += SNP) {
       const int2 tk = TK[e];
-      float2 v = yh[((long long)tk.x * nz + zc + zz) * H + (tk.y & 0x3FFFFFFF)];
-      if (tk.y >> 30) v.y = -v.y;
-      V[zz][e] = v;
+      const float2* src = yh + ((long long)tk.x * nz + zc) * H + (tk.y & 0x3FFFFFFF);
+      const bool cj = (tk.y >> 30) != 0;
+#pragma unroll
+      for (int zz = 0; zz < SZC; ++zz) {
+        if (zz < nzz) {
+          float2 v = src[(long long)zz * H];
+          if (cj) v.y = -v.y;
+          V[zz][e] = v;
+        }
+      }
     }
     __syncthreads();
     Acc ax[SZC], ay[SZC];
@@ -457,11 +487,9 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
       ax[zz] = (Acc)0;
       ay[zz] = (Acc)0;
     }
-    for (int j = 0; j < it.nl; ++j) {
-      const unsigned u = Ls[j][p];
-      if (u == SL_EMPTY) break;
-      const int e = (int)(u & 0x1FFu), tap = (int)(u >> 9);
-      const int i1 = tap / TAPS, i0 = tap - TAPS * i1;
+    auto tap = [&](unsigned u) {
+      const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
+      const int i1 = t / TAPS, i0 = t - TAPS * i1;
       const float2 f = F[e];
       const float dx = (float)(i0 - M) - f.x, dy = (float)(i1 - M) - f.y;
       const float w = __expf(-a * (dx * dx + dy * dy));
@@ -471,6 +499,16 @@ __global__ __launch_bounds__(SNP) void scatter_lists(const float2* __restrict__ 
         ax[zz] += (Acc)(w * v.x);
         ay[zz] += (Acc)(w * v.y);
       }
+    };
+    for (int jp = 0; jp < na; ++jp) {
+      const unsigned u = la[jp];
+      tap(u & 0xFFFFu);
+      tap(u >> 16);
+    }
+    for (int jp = SL_PAIRS_A; jp < npairs; ++jp) {
+      const unsigned u = lb[jp - SL_PAIRS_A];
+      tap(u & 0xFFFFu);
+      tap(u >> 16);
     }
     if (l0 <= n) {
       for (int zz = 0; zz < nzz; ++zz) {
@@ -750,12 +788,12 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
     }
     tap_off[e + 1] = (int)tap_pt.size();
   }
-  // greedy chunks of entries: <= SE_MAX entries, <= SL_MAX taps per point
+  // greedy chunks of entries: < SE_ZERO entries (row SE_ZERO stays zero), <= SL_MAX taps per point
   std::vector<int> cnt(SNP, 0);
   std::vector<std::pair<int, int>> chunks;
   int e0 = 0;
   for (int e = 0; e < ne; ++e) {
-    bool fits = (e - e0) < SE_MAX;
+    bool fits = (e - e0) < SE_ZERO;
     for (int t = tap_off[e]; fits && t < tap_off[e + 1]; ++t)
       if (cnt[tap_pt[t]] + 1 > SL_MAX) fits = false;
     if (!fits) {
@@ -781,7 +819,13 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
       for (int t = tap_off[e]; t < tap_off[e + 1]; ++t) L[tap_pt[t]].push_back((unsigned short)((e - a) | tap_code[t]));
     size_t nl = 0;
     for (auto& l : L) nl = std::max(nl, l.size());
+    nl = (nl + 1) & ~size_t(1);  // slots are read in pairs
     SItem it{};
+    for (int w = 0; w < SWAVES; ++w) {
+      size_t m = 0;
+      for (int pt = w * 64; pt < w * 64 + 64; ++pt) m = std::max(m, L[pt].size());
+      it.wnp |= (unsigned)((m + 1) / 2) << (8 * w);
+    }
     it.x0 = x0;
     it.y0 = y0;
     it.ne = b - a;
@@ -790,8 +834,11 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
     it.loff = (long long)B.lists.size();
     it.slot = split ? slot0 + (int)c : -1;
     B.entries.insert(B.entries.end(), codes.begin() + a, codes.begin() + b);
-    for (size_t j = 0; j < nl; ++j)
-      for (int pt = 0; pt < SNP; ++pt) B.lists.push_back(j < L[pt].size() ? L[pt][j] : SL_EMPTY);
+    for (size_t j = 0; j < nl; j += 2)
+      for (int pt = 0; pt < SNP; ++pt) {
+        B.lists.push_back(j < L[pt].size() ? L[pt][j] : SL_EMPTY);
+        B.lists.push_back(j + 1 < L[pt].size() ? L[pt][j + 1] : SL_EMPTY);
+      }
     B.items.push_back(it);
   }
 }
@@ -946,9 +993,9 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
 template <int L, bool INV>
 static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in_hi, int out_lo, int out_hi,
                        float scale, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = ColShape<L>;
   const int ncols = p->n + 1;
-  hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(kThreads),
+  hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
                      0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + L);
 }
 
