@@ -46,105 +46,108 @@ __device__ __forceinline__ int pad_src(int i, int len, bool mirror) {
   return i - h;
 }
 
+template <int L>
+using ColShape = RegShape<L, kBatchElems>;
+
 // Column pass: frames (b, nz, n); blockIdx.y = frame, C columns per block.
 // ADJ = false: out = crop IFFT K FFT pad (in);  ADJ = true: out = pad^T IFFT conj(K) FFT crop^T (in).
+// Both transforms and the chirp run on the register FFT; only the fold-add of
+// the adjoint mirror padding reads other threads' points (through LDS).
 template <int L, bool ADJ>
-__global__ __launch_bounds__(kThreads) void col_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
-                                                     int nz, bool mirror, const float2* __restrict__ K,
-                                                     const float2* __restrict__ tw) {
-  using Sh = FftShape<L>;
+__global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __restrict__ in, float2* __restrict__ out,
+                                                            int n, int nz, bool mirror, const float2* __restrict__ K,
+                                                            const float2* __restrict__ tw) {
+  using Sh = ColShape<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int col0 = blockIdx.x * Sh::C;
-  const long long fo = (long long)blockIdx.y * nz * n + col0;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const int col = blockIdx.x * Sh::C + c;
+  const bool live = col < n;
+  const long long fo = (long long)blockIdx.y * nz * n + col;
   const float2* src = in + fo;
   float2* dst = out + fo;
+  float2* x = lds + c * Sh::S;
   const int h = nz / 2;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it) {
-    const int idx = threadIdx.x + it * kThreads;
-    const int c = idx % Sh::C, i = idx / Sh::C;
-    int sr;
-    if (ADJ)
-      sr = (i >= h && i < h + nz) ? i - h : -1;
-    else
-      sr = pad_src(i, nz, mirror);
-    float2 v = make_float2(0.f, 0.f);
-    if (sr >= 0 && col0 + c < n) v = src[(long long)sr * n + c];
-    lds[c * Sh::S + lpad(i)] = v;
+  for (int m = 0; m < kPerThread; ++m) {
+    const int i = t + m * Sh::TPC;
+    const int sr = ADJ ? ((i >= h && i < h + nz) ? i - h : -1) : pad_src(i, nz, mirror);
+    v[m] = (live && sr >= 0) ? src[(long long)sr * n] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  const int cc = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
-  float2* x = lds + cc * Sh::S;
-  fft_lds<L, Sh::TPC, false>(x, t, tw);
+  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int k = t + e * Sh::TPC;
-    float2 kk = K[k];
+  for (int m = 0; m < kPerThread; ++m) {
+    float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
-    x[lpad(k)] = cmul(x[lpad(k)], kk);
+    v[m] = cmul(v[m], kk);
   }
-  __syncthreads();
-  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  fft_reg<L, Sh::TPC, true>(v, x, t, tw);
+  if (ADJ && mirror) {
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it) {
-    const int idx = threadIdx.x + it * kThreads;
-    const int c = idx % Sh::C, r = idx / Sh::C;
-    if (r >= nz || col0 + c >= n) continue;
-    float2 v = lds[c * Sh::S + lpad(r + h)];
-    if (ADJ && mirror) {
+    for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+    __syncthreads();
+    if (!live) return;  // no barrier below
+    for (int r = t; r < nz; r += Sh::TPC) {
       const int pr = (r < h) ? (h - 1 - r) : (2 * nz + h - 1 - r);
-      v = cadd(v, lds[c * Sh::S + lpad(pr)]);
+      dst[(long long)r * n] = cadd(x[lpad(r + h)], x[lpad(pr)]);
     }
-    dst[(long long)r * n + c] = v;
+  } else {
+    if (!live) return;
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+      const int r = t + m * Sh::TPC - h;
+      if (r >= 0 && r < nz) dst[(long long)r * n] = v[m];
+    }
   }
 }
 
 // Row pass: rows of length n, C rows per block, nrows = b * nz.
 template <int L, bool ADJ>
-__global__ __launch_bounds__(RowShape<L>::NT) void row_pass(const float2* __restrict__ in, float2* __restrict__ out, int n,
-                                                     long long nrows, bool mirror, const float2* __restrict__ K,
-                                                     const float2* __restrict__ tw) {
-  using Sh = RowShape<L>;
+__global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restrict__ in, float2* __restrict__ out,
+                                                          int n, long long nrows, bool mirror,
+                                                          const float2* __restrict__ K, const float2* __restrict__ tw) {
+  using Sh = RowReg<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
   const long long row = (long long)blockIdx.x * Sh::C + c;
   const bool valid = row < nrows;
   const float2* src = in + (valid ? row : 0) * n;
   float2* x = lds + c * Sh::S;
   const int h = n / 2;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int i = t + e * Sh::TPC;
-    int sx;
-    if (ADJ)
-      sx = (i >= h && i < h + n) ? i - h : -1;
-    else
-      sx = pad_src(i, n, mirror);
-    x[lpad(i)] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
+  for (int m = 0; m < kPerThread; ++m) {
+    const int i = t + m * Sh::TPC;
+    const int sx = ADJ ? ((i >= h && i < h + n) ? i - h : -1) : pad_src(i, n, mirror);
+    v[m] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  fft_lds<L, Sh::TPC, false>(x, t, tw);
+  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int k = t + e * Sh::TPC;
-    float2 kk = K[k];
+  for (int m = 0; m < kPerThread; ++m) {
+    float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
-    x[lpad(k)] = cmul(x[lpad(k)], kk);
+    v[m] = cmul(v[m], kk);
   }
-  __syncthreads();
-  fft_lds<L, Sh::TPC, true>(x, t, tw);
-  if (!valid) return;
-  float2* dst = out + row * n;
+  fft_reg<L, Sh::TPC, true>(v, x, t, tw);
+  float2* dst = out + (valid ? row : 0) * n;
+  if (ADJ && mirror) {
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int r = t + e * Sh::TPC;
-    if (r >= n) continue;
-    float2 v = x[lpad(r + h)];
-    if (ADJ && mirror) {
+    for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+    __syncthreads();
+    if (!valid) return;  // no barrier below
+    for (int r = t; r < n; r += Sh::TPC) {
       const int pr = (r < h) ? (h - 1 - r) : (2 * n + h - 1 - r);
-      v = cadd(v, x[lpad(pr)]);
+      dst[r] = cadd(x[lpad(r + h)], x[lpad(pr)]);
     }
-    dst[r] = v;
+  } else {
+    if (!valid) return;
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+      const int r = t + m * Sh::TPC - h;
+      if (r >= 0 && r < n) dst[r] = v[m];
+    }
   }
 }
 
@@ -172,14 +175,14 @@ static void fresnel_free(mm_fresnel* p) {
 
 template <int L, bool ADJ>
 static void launch_col_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
-  using Sh = FftShape<L>;
+  using Sh = ColShape<L>;
   hipLaunchKernelGGL((col_pass<L, ADJ>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
-                     dim3(kThreads), 0, st, in, out, p->n, p->nz, p->pad_mode == 0, p->Ky + (size_t)j * L, p->tw + L);
+                     dim3(Sh::NT), 0, st, in, out, p->n, p->nz, p->pad_mode == 0, p->Ky + (size_t)j * L, p->tw + L);
 }
 
 template <int L, bool ADJ>
 static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
-  using Sh = RowShape<L>;
+  using Sh = RowReg<L>;
   const long long rows = (long long)nframes * p->nz;
   hipLaunchKernelGGL((row_pass<L, ADJ>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, in, out,
                      p->n, rows, p->pad_mode == 0, p->Kx + (size_t)j * L, p->tw + L);

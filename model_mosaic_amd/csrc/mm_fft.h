@@ -168,24 +168,27 @@ __device__ __forceinline__ void fft_lds(float2* __restrict__ x, int t, const flo
 // global <-> LDS staging of fft_lds.  Global loads and stores of v are
 // therefore direct (element t + m TPC: consecutive t, consecutive addresses).
 
-// Workgroup shape: C transforms of length L, stride S between them in LDS.
-// S = 16 (mod 32) complex values, with the thread mapping of map(), keeps the
-// Stockham writes and natural reads free of bank conflicts (two transforms
-// per 32-lane half-wave at a 16-bank-pair offset).
-template <int L, int BATCH>
+// Workgroup shape: C transforms of length L, stride S between them in LDS
+// (room for XTRA more values per transform, for kernels that stage more than L
+// inputs there).  S = 16 (mod 32) complex values, with the interleaved thread
+// mapping of map(), keeps the Stockham writes and natural reads free of bank
+// conflicts (two transforms per 32-lane half-wave at a 16-bank-pair offset).
+// The plain mapping (64 consecutive t of one transform per wave) reads rows
+// in longer contiguous runs; it leaves some 2-way conflicts on 64-bank LDS.
+template <int L, int BATCH, int XTRA = 0, bool INTERLEAVE = true>
 struct RegShape {
   static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
   static constexpr int C = BATCH > L ? BATCH / L : 1;
   static constexpr int TPC = L / kPerThread;
   static constexpr int NT = C * TPC;
-  static constexpr int S0 = L + L / 16;                       // > lpad(L - 1)
+  static constexpr int S0 = L + L / 16 + XTRA;  // > lpad(L - 1) + XTRA
   static constexpr int S = TPC < 16 ? S0 : S0 + ((16 - S0 % 32) + 32) % 32;  // S = 16 (mod 32)
   static constexpr int LDS = C * S;
-  // Thread -> (transform c, index t).  For TPC >= 16 a wave holds 16
-  // consecutive t of min(C, 4) transforms, so loads of interleaved columns
+  // Thread -> (transform c, index t).  Interleaved (TPC >= 16): a wave holds
+  // 16 consecutive t of min(C, 4) transforms, so loads of interleaved columns
   // (4 per 32-byte row segment) are contiguous across the wave.
   __device__ static __forceinline__ void map(int tid, int& c, int& t) {
-    if constexpr (TPC < 16) {
+    if constexpr (TPC < 16 || !INTERLEAVE) {
       c = tid / TPC;
       t = tid % TPC;
     } else {
@@ -196,6 +199,10 @@ struct RegShape {
     }
   }
 };
+
+// Row-pass shape of the register FFT: 8192 points per workgroup, plain mapping.
+template <int L, int XTRA = 0>
+using RowReg = RegShape<L, kRowBatch, XTRA, false>;
 
 // W16^e times a (forward kernel exp(-2 pi i e / 16); INV conjugates).
 template <int E, bool INV>

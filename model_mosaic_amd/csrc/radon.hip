@@ -85,14 +85,17 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
 
 // Row pass of R: two object rows per complex FFT.  Row y of slice z, times
 // phi_y[y] phi_x[x], zero-padded to 2n (object centred), is transformed and
-// the half spectrum kx in [0, n] is written to half-spectrum row n/2 + y.
+// the half spectrum kx in [0, n] is written to half-spectrum row n/2 + y.  The
+// inputs go straight into the register FFT; the spectrum passes once through
+// LDS to pair Z_k with Z_{L-k} and to write the half-spectrum blocks in order.
 template <int L>
-__global__ __launch_bounds__(RowShape<L>::NT) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
-                                                         const float* __restrict__ phx, const float* __restrict__ phy,
-                                                         int n, int nz, long long fzs, const float2* __restrict__ tw) {
-  using Sh = RowShape<L>;
+__global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
+                                                           const float* __restrict__ phx, const float* __restrict__ phy,
+                                                           int n, int nz, long long fzs, const float2* __restrict__ tw) {
+  using Sh = RowReg<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
   const int half = n / 2;
   const long long npairs = (long long)nz * half;
   const long long pair = (long long)blockIdx.x * Sh::C + c;
@@ -106,19 +109,20 @@ __global__ __launch_bounds__(RowShape<L>::NT) void r1_rows_rfft(const float* __r
   const float* ra = obj + ((long long)z * n + y0) * n;
   const float* rb = ra + n;
   const float pya = valid ? phy[y0] : 0.f, pyb = valid ? phy[y0 + 1] : 0.f;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int i = t + e * Sh::TPC;
-    const int xi = i - half;
-    float2 v = make_float2(0.f, 0.f);
+  for (int m = 0; m < kPerThread; ++m) {
+    const int xi = t + m * Sh::TPC - half;
+    v[m] = make_float2(0.f, 0.f);
     if (valid && xi >= 0 && xi < n) {
       const float p = phx[xi];
-      v = make_float2(ra[xi] * pya * p, rb[xi] * pyb * p);
+      v[m] = make_float2(ra[xi] * pya * p, rb[xi] * pyb * p);
     }
-    x[lpad(i)] = v;
   }
+  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
+#pragma unroll
+  for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
-  fft_lds<L, Sh::TPC, false>(x, t, tw);
   // unpack A = (Z_k + conj Z_{L-k})/2 (row y0), B = (Z_k - conj Z_{L-k})/(2i) (row y0+1);
   // thread order: column within block, then the 2C rows of this workgroup, then block
   constexpr int R2 = 2 * Sh::C;
@@ -133,9 +137,9 @@ __global__ __launch_bounds__(RowShape<L>::NT) void r1_rows_rfft(const float* __r
     const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (r & 1);
     const float2* xc = lds + cc * Sh::S;
     const float2 zk = xc[lpad(col)], zm = xc[lpad((L - col) & (L - 1))];
-    const float2 v = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
+    const float2 w = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
                              : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-    fde[(long long)zz * fzs + hs_idx(half + yy, col, 2 * n)] = v;
+    fde[(long long)zz * fzs + hs_idx(half + yy, col, 2 * n)] = w;
   }
 }
 
@@ -269,23 +273,25 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
 }
 
 // Line pass of R: rebuild the full line g[r] from its representatives,
-// sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).
+// sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).  Inputs and
+// outputs go straight between HBM and the register FFT.
 template <int L>
-__global__ __launch_bounds__(RowShape<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
-                                                         const int* __restrict__ sig, int H, long long nrows, int nz,
-                                                         float scale, const float2* __restrict__ tw) {
-  using Sh = RowShape<L>;
+__global__ __launch_bounds__(RowReg<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
+                                                           const int* __restrict__ sig, int H, long long nrows, int nz,
+                                                           float scale, const float2* __restrict__ tw) {
+  using Sh = RowReg<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
   const long long row = (long long)blockIdx.x * Sh::C + c;
   const bool valid = row < nrows;
   const int sg = valid ? sig[row / nz] : 1;
   const float2* src = gh + (valid ? row : 0) * H;
-  float2* x = lds + c * Sh::S;
   constexpr int half = L / 2;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int r = t + e * Sh::TPC;
+  for (int m = 0; m < kPerThread; ++m) {
+    const int r = t + m * Sh::TPC;
     const int d = r - half;
     int kk;
     bool cj;
@@ -296,88 +302,105 @@ __global__ __launch_bounds__(RowShape<L>::NT) void r4_line_ifft(const float2* __
       kk = d <= 0 ? -d : d;
       cj = d > 0;
     }
-    float2 v = valid ? src[kk] : make_float2(0.f, 0.f);
-    if (cj) v.y = -v.y;
-    if (!(r & 1)) v = make_float2(-v.x, -v.y);
-    x[lpad(r)] = v;
+    float2 w = valid ? src[kk] : make_float2(0.f, 0.f);
+    if (cj) w.y = -w.y;
+    if (!(r & 1)) w = make_float2(-w.x, -w.y);
+    v[m] = w;
   }
-  __syncthreads();
-  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  fft_reg<L, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
   if (valid) {
     float* dst = sino + row * L;
 #pragma unroll
-    for (int e = 0; e < kPerThread; ++e) {
-      const int s = t + e * Sh::TPC;
-      const float v = x[lpad(s)].x * scale;
-      dst[s] = (s & 1) ? v : -v;
+    for (int m = 0; m < kPerThread; ++m) {
+      const int s = t + m * Sh::TPC;
+      const float w = v[m].x * scale;
+      dst[s] = (s & 1) ? w : -w;
     }
   }
 }
 
 // Line pass of RT: yhat = c1d . FFT(c1d . y), written at the representatives
 // with the fold weights (1/2 at k = 0 and k = n/2) times `scale`.  A complex
-// sinogram yields the two channels of its real and imaginary parts.
+// sinogram yields the two channels of its real and imaginary parts (which pair
+// yhat(r) with yhat(L - r), so that output passes through LDS; the real one is
+// written straight from registers).
 template <int L, bool CPLX>
-__global__ __launch_bounds__(RowShape<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
-                                                         float2* __restrict__ yim, const int* __restrict__ sig, int H,
-                                                         long long nrows, int nz, float scale,
-                                                         const float2* __restrict__ tw) {
-  using Sh = RowShape<L>;
+__global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
+                                                           float2* __restrict__ yim, const int* __restrict__ sig, int H,
+                                                           long long nrows, int nz, float scale,
+                                                           const float2* __restrict__ tw) {
+  using Sh = RowReg<L>;
   __shared__ float2 lds[Sh::LDS];
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
   const long long row = (long long)blockIdx.x * Sh::C + c;
   const bool valid = row < nrows;
   const long long rr = valid ? row : 0;
   const int sg = valid ? sig[rr / nz] : 1;
   float2* x = lds + c * Sh::S;
   constexpr int half = L / 2;
+  float2 v[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int r = t + e * Sh::TPC;
-    float2 v;
+  for (int m = 0; m < kPerThread; ++m) {
+    const int r = t + m * Sh::TPC;
+    float2 w;
     if (CPLX)
-      v = valid ? ((const float2*)sino_v)[rr * L + r] : make_float2(0.f, 0.f);
+      w = valid ? ((const float2*)sino_v)[rr * L + r] : make_float2(0.f, 0.f);
     else
-      v = make_float2(valid ? ((const float*)sino_v)[rr * L + r] : 0.f, 0.f);
-    if (!(r & 1)) v = make_float2(-v.x, -v.y);
-    x[lpad(r)] = v;
+      w = make_float2(valid ? ((const float*)sino_v)[rr * L + r] : 0.f, 0.f);
+    if (!(r & 1)) w = make_float2(-w.x, -w.y);
+    v[m] = w;
   }
+  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
+  if (!CPLX) {
+    if (!valid) return;
+    float2* dst = yre + rr * H;
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+      const int r = t + m * Sh::TPC;
+      const float2 y = (r & 1) ? v[m] : make_float2(-v[m].x, -v[m].y);  // yhat(r)
+      if (sg > 0) {
+        if (r >= half) {
+          const int k = r - half;
+          dst[k] = cscale(y, (k == 0 ? 0.5f : 1.0f) * scale);
+        } else if (r == 0) {
+          dst[half] = cscale(cconj(y), 0.5f * scale);
+        }
+      } else if (r <= half) {
+        const int k = half - r;
+        dst[k] = cscale(y, ((k == 0 || k == half) ? 0.5f : 1.0f) * scale);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
-  fft_lds<L, Sh::TPC, false>(x, t, tw);
   if (!valid) return;  // no barrier below
   auto yhat = [&](int r) {
-    float2 v = x[lpad(r)];
-    return (r & 1) ? v : make_float2(-v.x, -v.y);
+    float2 w = x[lpad(r)];
+    return (r & 1) ? w : make_float2(-w.x, -w.y);
   };
   for (int k = t; k < H; k += Sh::TPC) {
     const float f = ((k == 0 || k == half) ? 0.5f : 1.0f) * scale;
-    if (!CPLX) {
-      float2 v;
-      if (sg > 0)
-        v = (k < half) ? yhat(half + k) : cconj(yhat(0));
-      else
-        v = yhat(half - k);
-      yre[rr * H + k] = cscale(v, f);
-    } else {
-      float2 vr, vi;
-      if (k == half) {
-        const float2 v0 = yhat(0);
-        if (sg > 0) {
-          vr = cconj(v0);
-          vi = make_float2(v0.y, v0.x);  // conj(-i v0)
-        } else {
-          vr = v0;
-          vi = make_float2(v0.y, -v0.x);  // -i v0
-        }
+    float2 vr, vi;
+    if (k == half) {
+      const float2 v0 = yhat(0);
+      if (sg > 0) {
+        vr = cconj(v0);
+        vi = make_float2(v0.y, v0.x);  // conj(-i v0)
       } else {
-        const int r = sg > 0 ? half + k : half - k;
-        const float2 p = yhat(r), q = yhat((L - r) & (L - 1));
-        vr = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
-        vi = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+        vr = v0;
+        vi = make_float2(v0.y, -v0.x);  // -i v0
       }
-      yre[rr * H + k] = cscale(vr, f);
-      yim[rr * H + k] = cscale(vi, f);
+    } else {
+      const int r = sg > 0 ? half + k : half - k;
+      const float2 p = yhat(r), q = yhat((L - r) & (L - 1));
+      vr = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+      vi = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
     }
+    yre[rr * H + k] = cscale(vr, f);
+    yim[rr * H + k] = cscale(vi, f);
   }
 }
 
@@ -546,13 +569,16 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
 // per complex FFT; complex output packs the real/imaginary channels of one row.
 // The 2C input rows of a workgroup are first staged block-ordered (coalesced):
 // row A of transform c at x_c[0..n], row B at x_c[n+1..2n+1].
+template <int L>
+using C2rShape = RowReg<L, 2>;  // staging holds 2 (n + 1) values per transform
+
 template <int L, bool CPLX>
-__global__ __launch_bounds__(RowShape<L>::NT) void rt4_rows_c2r(const float2* __restrict__ Hre,
-                                                         const float2* __restrict__ Him, void* __restrict__ out_v,
-                                                         const float* __restrict__ phx, const float* __restrict__ phy,
-                                                         int n, int nz, long long hzs, float scale,
-                                                         const float2* __restrict__ tw) {
-  using Sh = RowShape<L>;
+__global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __restrict__ Hre,
+                                                              const float2* __restrict__ Him, void* __restrict__ out_v,
+                                                              const float* __restrict__ phx, const float* __restrict__ phy,
+                                                              int n, int nz, long long hzs, float scale,
+                                                              const float2* __restrict__ tw) {
+  using Sh = C2rShape<L>;
   __shared__ float2 lds[Sh::LDS];
   const int half = n / 2;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
@@ -579,12 +605,13 @@ __global__ __launch_bounds__(RowShape<L>::NT) void rt4_rows_c2r(const float2* __
     lds[cc * Sh::S + lpad(which * (n + 1) + col)] = src[(long long)z * hzs + hs_idx(row, col, L)];
   }
   __syncthreads();
-  const int c = threadIdx.x / Sh::TPC, t = threadIdx.x % Sh::TPC;
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
   float2* x = lds + c * Sh::S;
-  float2 zv[kPerThread];
+  float2 v[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int kx = t + e * Sh::TPC;
+  for (int m = 0; m < kPerThread; ++m) {
+    const int kx = t + m * Sh::TPC;
     float2 a, b;
     if (kx <= n) {
       a = x[lpad(kx)];
@@ -597,13 +624,10 @@ __global__ __launch_bounds__(RowShape<L>::NT) void rt4_rows_c2r(const float2* __
       a = cconj(x[lpad(L - kx)]);
       b = cconj(x[lpad(n + 1 + L - kx)]);
     }
-    zv[e] = make_float2(a.x - b.y, a.y + b.x);
+    v[m] = make_float2(a.x - b.y, a.y + b.x);
   }
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < kPerThread; ++e) x[lpad(t + e * Sh::TPC)] = zv[e];
-  __syncthreads();
-  fft_lds<L, Sh::TPC, true>(x, t, tw);
+  __syncthreads();  // the transform's exchanges reuse the staging area
+  fft_reg<L, Sh::TPC, true>(v, x, t, tw);
   const long long unit = (long long)blockIdx.x * Sh::C + c;
   if (unit >= nunits) return;
   int z, y;
@@ -615,18 +639,18 @@ __global__ __launch_bounds__(RowShape<L>::NT) void rt4_rows_c2r(const float2* __
     y = 2 * (int)(unit - (long long)z * half);
   }
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
-    const int xx = t + e * Sh::TPC;
-    if (xx >= n) continue;
-    const float2 v = cscale(x[lpad(xx + half)], scale);
+  for (int m = 0; m < kPerThread; ++m) {
+    const int xx = t + m * Sh::TPC - half;
+    if (xx < 0 || xx >= n) continue;
+    const float2 w = cscale(v[m], scale);
     const float px = phx[xx];
     if (CPLX) {
       const float p = phy[y] * px;
-      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(v.x * p, v.y * p);
+      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(w.x * p, w.y * p);
     } else {
       float* o = (float*)out_v + ((long long)z * n + y) * n;
-      o[xx] = v.x * phy[y] * px;
-      o[n + xx] = v.y * phy[y + 1] * px;
+      o[xx] = w.x * phy[y] * px;
+      o[n + xx] = w.y * phy[y + 1] * px;
     }
   }
 }
@@ -984,7 +1008,7 @@ static int dmalloc(void** p, size_t nb, size_t& bytes, const char* what) {
 
 template <int L>
 static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
-  using Sh = RowShape<L>;
+  using Sh = RowReg<L>;
   const long long units = (long long)nz * (p->n / 2);
   hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, obj,
                      p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + L);
@@ -1001,7 +1025,7 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in
 
 template <int L>
 static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
-  using Sh = RowShape<L>;
+  using Sh = RowReg<L>;
   const long long rows = (long long)p->ntheta * nz;
   hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
                      sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + L);
@@ -1009,7 +1033,7 @@ static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
 
 template <int L>
 static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
-  using Sh = RowShape<L>;
+  using Sh = RowReg<L>;
   const long long rows = (long long)p->ntheta * nz;
   const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
   if (cplx)
@@ -1022,7 +1046,7 @@ static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, h
 
 template <int L>
 static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
-  using Sh = RowShape<L>;
+  using Sh = C2rShape<L>;
   const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
   const dim3 grid((unsigned)((units + Sh::C - 1) / Sh::C));
   const float scale = 1.0f / (2.0f * p->n);
