@@ -61,9 +61,12 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
-  const int col = blockIdx.x * Sh::C + c;
+  // four column groups share each 128-byte line of a row: keep them on one XCD
+  const unsigned lw = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int bx = (int)(lw % gridDim.x), frame = (int)(lw / gridDim.x);
+  const int col = bx * Sh::C + c;
   const bool live = col < n;
-  const long long fo = (long long)blockIdx.y * nz * n + col;
+  const long long fo = (long long)frame * nz * n + col;
   const float2* src = in + fo;
   float2* dst = out + fo;
   float2* x = lds + c * Sh::S;

@@ -57,6 +57,18 @@ void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max);
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// gfx950 dispatches workgroup w (linear id, x fastest) to XCD w mod 8, and each
+// XCD has its own L2.  xcd_remap turns w into a logical id such that every XCD
+// runs one contiguous range of logical ids: workgroups whose data share cache
+// lines (neighbouring columns, tiles with overlapping halos) then meet in one
+// L2 instead of each XCD fetching the lines again.  A bijection on [0, total).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned w, unsigned total) {
+  const unsigned per = total / kXcds, rem = total % kXcds;
+  const unsigned k = w % kXcds, pos = w / kXcds;
+  return k * per + (k < rem ? k : rem) + pos;
+}
 inline int ilog2(long v) { int r = 0; while ((1L << r) < v) ++r; return r; }
 
 }  // namespace mm
