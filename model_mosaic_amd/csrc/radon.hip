@@ -56,12 +56,18 @@ constexpr int SMAX = TILE_THREADS;  // samples per tile work item
 constexpr int PATCH = ROOT + 2 * M;  // gather LDS patch edge (rows)
 
 // Gather patch pitch (complex values per LDS row) for tile widths 64/32/16/8:
-// chosen by simulating the ds_read_b64 bank pattern of the 81-tap loop over
-// polar sample sets (sorted by angle, then radius), for 32 and 64 banks.
+// the smallest pitch >= w + 2M that is 4 (mod 8) with pitch / 4 odd, so the
+// staging writes (4 columns x 8 rows per half-wave) hit 32 distinct bank
+// pairs.  Taps are read as 16-byte pieces (a 9-tap row = five ds_read_b128
+// from the window base rounded down to even; ds_read2_b64 moves half the bytes
+// per cycle), which the LDS serves 16 lanes per cycle: the plan gives the 16
+// samples of each quarter-wave distinct (base / 2) mod 16, so the pieces of
+// one read never share a bank (64 banks = 16 groups of 16 bytes).
 __host__ __device__ __forceinline__ int patch_pitch(int w) {
-  return w >= 64 ? 75 : (w >= 32 ? 43 : (w >= 16 ? 25 : 37));
+  return w >= 64 ? 76 : (w >= 32 ? 44 : (w >= 16 ? 28 : 20));
 }
-constexpr int PATCH_ELEMS = (ROOT + 2 * M) * 75;
+constexpr int PATCH_ELEMS = (ROOT + 2 * M) * 76;  // even: the 16-byte reads stay inside
+constexpr int STG_MAX = (((ROOT + 2 * M + 3) / 4 + 1) * 4 * (ROOT + 2 * M) + TILE_THREADS - 1) / TILE_THREADS;
 
 struct Item {
   int x0, y0;        // tile origin (l0, l1), centred coordinates
@@ -201,12 +207,19 @@ __device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* 
 }
 
 // Tiled polar gather (forward R): gh[theta, z, k] = C0 sum ey sum ex F_c.
-__global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh,
+// Thread s computes sample samples[start + s] (lane order: each half-wave's
+// window bases are distinct mod 32, see patch_pitch); its results go through
+// LDS so that thread t writes sorted sample t (ascending theta, k: coalesced).
+// Each thread's patch staging addresses, reflections and signs are worked out
+// once per item; the z loop only moves data.
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh,
                                                             const Item* __restrict__ items,
                                                             const int* __restrict__ samples,
+                                                            const unsigned short* __restrict__ order,
                                                             const double2* __restrict__ dirs, int n, int H, int nz,
                                                             long long fzs, int zchunk, float a, float C0) {
-  __shared__ float2 P[PATCH_ELEMS];
+  __shared__ __align__(16) float2 P[PATCH_ELEMS + 2];  // + a sink for idle staging slots
+  __shared__ float2 O[TILE_THREADS];
   const Item it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
   const int pw = it.w + 2 * M, ph = it.h + 2 * M;
@@ -216,59 +229,98 @@ __global__ __launch_bounds__(TILE_THREADS) void gather_tiles(const float2* __res
   const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
   const int s = threadIdx.x;
   const bool active = s < it.count;
-  float ex[TAPS], ey[TAPS];
-  int base = 0, th = 0, k = 0;
-  if (active) {
-    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
-    th = g.th;
-    k = g.k;
-    base = (g.ry - it.y0) * pp + (g.rx - it.x0);
-#pragma unroll
-    for (int i = 0; i < TAPS; ++i) {
-      const float dx = (float)(i - M) - g.fx, dy = (float)(i - M) - g.fy;
-      ex[i] = __expf(-a * dx * dx);
-      ey[i] = C0 * __expf(-a * dy * dy);
-    }
-  }
   const int mask = 2 * n - 1;
-  for (int z = z0; z < z1; ++z) {
-    const float2* f = fde + (long long)z * fzs;
-    for (int idx = threadIdx.x; idx < nstage; idx += TILE_THREADS) {
+  // staging descriptors: source offset in the z plane; LDS slot << 2 | negate << 1 | conj.
+  // Idle slots read offset 0 and write the sink, so the loads issue together.
+  int soff[STG_MAX], sdst[STG_MAX];
+#pragma unroll
+  for (int e = 0; e < STG_MAX; ++e) {
+    const int idx = threadIdx.x + e * TILE_THREADS;
+    sdst[e] = PATCH_ELEMS << 2;
+    soff[e] = 0;
+    if (idx < nstage) {
       const int q = idx & 3, rest = idx >> 2;
       const int pr = rest % ph, blk = rest / ph;
       const int l0 = bx0 + 4 * blk + q;
       const int pc = l0 - px0;
-      if (pc < 0 || pc >= pw) continue;
-      const int k0 = l0 & mask, k1 = (py0 + pr) & mask;
-      float2 v;
-      if (k0 <= n) {
-        v = f[hs_idx(k1, k0, 2 * n)];
-      } else {
-        v = f[hs_idx((2 * n - k1) & mask, 2 * n - k0, 2 * n)];
-        v.y = -v.y;
+      if (pc >= 0 && pc < pw) {
+        const int k0 = l0 & mask, k1 = (py0 + pr) & mask;
+        int fl = 0;
+        if (k0 <= n) {
+          soff[e] = (int)hs_idx(k1, k0, 2 * n);
+        } else {
+          soff[e] = (int)hs_idx((2 * n - k1) & mask, 2 * n - k0, 2 * n);
+          fl = 1;
+        }
+        if ((k0 + k1) & 1) fl |= 2;
+        sdst[e] = ((pr * pp + pc) << 2) | fl;
       }
-      if ((k0 + k1) & 1) v = make_float2(-v.x, -v.y);
-      P[pr * pp + pc] = v;
+    }
+  }
+  // ex10[j] weighs patch column (base & ~1) + j: the 9 taps shifted by the parity of base
+  float ex10[TAPS + 1], ey[TAPS];
+  int base = 0;
+  if (active) {
+    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
+    const int b = (g.ry - it.y0) * pp + (g.rx - it.x0);
+    const int par = b & 1;
+    base = b - par;
+#pragma unroll
+    for (int j = 0; j <= TAPS; ++j) {
+      const int i = j - par;
+      const float dx = (float)(i - M) - g.fx;
+      ex10[j] = (i >= 0 && i < TAPS) ? __expf(-a * dx * dx) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < TAPS; ++i) {
+      const float dy = (float)(i - M) - g.fy;
+      ey[i] = C0 * __expf(-a * dy * dy);
+    }
+  }
+  // output of sorted sample t = threadIdx.x
+  const bool writer = threadIdx.x < it.count;
+  int src_lane = 0;
+  long long out0 = 0;
+  if (writer) {
+    src_lane = order[it.start + threadIdx.x];
+    const int id = samples[it.start + src_lane];
+    const int th = id / H;
+    out0 = (long long)th * nz * H + (id - th * H);
+  }
+  for (int z = z0; z < z1; ++z) {
+    const float2* f = fde + (long long)z * fzs;
+    float2 sv[STG_MAX];
+#pragma unroll
+    for (int e = 0; e < STG_MAX; ++e) sv[e] = f[soff[e]];
+#pragma unroll
+    for (int e = 0; e < STG_MAX; ++e) {
+      float2 v = sv[e];
+      if (sdst[e] & 1) v.y = -v.y;
+      if (sdst[e] & 2) v = make_float2(-v.x, -v.y);
+      P[sdst[e] >> 2] = v;
     }
     __syncthreads();
     if (active) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int i1 = 0; i1 < TAPS; ++i1) {
-        const float2* prow = P + base + i1 * pp;
+        const float4* prow = reinterpret_cast<const float4*>(P + base + i1 * pp);  // base, pp even
         float2 row = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int i0 = 0; i0 < TAPS; ++i0) {
-          const float2 v = prow[i0];
-          row.x = fmaf(ex[i0], v.x, row.x);
-          row.y = fmaf(ex[i0], v.y, row.y);
+        for (int c = 0; c < (TAPS + 1) / 2; ++c) {
+          const float4 v = prow[c];
+          row.x = fmaf(ex10[2 * c], v.x, row.x);
+          row.y = fmaf(ex10[2 * c], v.y, row.y);
+          row.x = fmaf(ex10[2 * c + 1], v.z, row.x);
+          row.y = fmaf(ex10[2 * c + 1], v.w, row.y);
         }
         acc.x = fmaf(ey[i1], row.x, acc.x);
         acc.y = fmaf(ey[i1], row.y, acc.y);
       }
-      gh[((long long)th * nz + z) * H + k] = acc;
+      O[s] = acc;
     }
     __syncthreads();
+    if (writer) gh[out0 + (long long)z * H] = O[src_lane];
   }
 }
 
@@ -710,8 +762,52 @@ struct HostGeom {
 
 struct Binning {
   std::vector<Item> items;
-  std::vector<int> samples;
+  std::vector<int> samples;            // per item, in lane order
+  std::vector<unsigned short> order;   // per item: lane of the t-th sample in ascending id order
 };
+
+// Lane order of one gather item (ids ascending): every quarter-wave of 16
+// lanes takes samples from 16 different colours ((window base / 2) mod 16 =
+// the LDS bank group of each 16-byte tap read), drawing from the fullest
+// colours first, so the tap reads are conflict-free wherever the colours allow.
+static void gather_lanes(const HostGeom& G, int x0, int y0, int w, const int* ids, int count, Binning& B) {
+  const int pp = patch_pitch(w);
+  std::vector<int> bucket[16];
+  for (int t = 0; t < count; ++t) {
+    int rx, ry;
+    G.centre(ids[t], rx, ry);
+    bucket[(((ry - y0) * pp + (rx - x0)) >> 1) & 15].push_back(t);
+  }
+  size_t head[16] = {};
+  std::vector<int> lane_of(count);
+  const size_t start = B.samples.size();
+  B.samples.resize(start + count);
+  int lane = 0;
+  while (lane < count) {
+    const int size = std::min(16, count - lane);
+    int col[16];
+    for (int c = 0; c < 16; ++c) col[c] = c;
+    std::sort(col, col + 16, [&](int u, int v) { return bucket[u].size() - head[u] > bucket[v].size() - head[v]; });
+    int taken = 0;
+    for (int i = 0; i < 16 && taken < size; ++i) {
+      const int c = col[i];
+      if (head[c] == bucket[c].size()) break;
+      const int t = bucket[c][head[c]++];
+      lane_of[t] = lane + taken++;
+    }
+    for (int i = 0; taken < size; i = (i + 1) % 16) {  // fewer colours than lanes left
+      const int c = col[i];
+      if (head[c] == bucket[c].size()) continue;
+      const int t = bucket[c][head[c]++];
+      lane_of[t] = lane + taken++;
+    }
+    lane += size;
+  }
+  for (int t = 0; t < count; ++t) {
+    B.samples[start + lane_of[t]] = ids[t];
+    B.order.push_back((unsigned short)lane_of[t]);
+  }
+}
 
 static bool rect_hits(const Rect& r, int x0, int y0, int w, int h) {
   return r.c0 <= x0 + w - 1 && r.c1 >= x0 && r.r0 <= y0 + h - 1 && r.r1 >= y0;
@@ -730,7 +826,7 @@ static void gather_node(const HostGeom& G, int x0, int y0, int sz, std::vector<i
       it.start = (int)B.samples.size();
       it.count = (int)std::min<size_t>(SMAX, ids.size() - s);
       it.slot = -1;
-      B.samples.insert(B.samples.end(), ids.begin() + s, ids.begin() + s + it.count);
+      gather_lanes(G, x0, y0, sz, ids.data() + s, it.count, B);
       B.items.push_back(it);
     }
     return;
@@ -960,6 +1056,7 @@ struct mm_radon {
   int* sig = nullptr;
   Item* gitems = nullptr;
   int* gsamples = nullptr;
+  unsigned short* gorder = nullptr;
   int n_gitems = 0;
   SItem* sitems = nullptr;
   int* sentries = nullptr;
@@ -988,7 +1085,7 @@ static void radon_free(mm_radon* p) {
     for (auto& x : e)
       if (x) (void)hipEventDestroy(x);
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
-                  p->gitems, p->gsamples, p->sitems, p->sentries, p->slists, p->splits};
+                  p->gitems, p->gsamples, p->gorder, p->sitems, p->sentries, p->slists, p->splits};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1086,7 +1183,8 @@ static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
   hipLaunchKernelGGL(gather_tiles, dim3((unsigned)p->n_gitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st, p->fde,
-                     p->gh, p->gitems, p->gsamples, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g);
+                     p->gh, p->gitems, p->gsamples, p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a,
+                     p->C0g);
   return check_launch("gather_tiles");
 }
 
@@ -1190,7 +1288,8 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   size_t& bytes = p->bytes;
   if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
       (rc = upload(&p->sig, sig, bytes)) || (rc = upload(&p->gitems, BG.items, bytes)) ||
-      (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->sitems, BS.items, bytes)) ||
+      (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->gorder, BG.order, bytes)) ||
+      (rc = upload(&p->sitems, BS.items, bytes)) ||
       (rc = upload(&p->sentries, BS.entries, bytes)) || (rc = upload(&p->slists, BS.lists, bytes)) ||
       (rc = upload(&p->splits, BS.splits, bytes)))
     return bail(rc);
