@@ -474,28 +474,41 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
 // A point's list is private to its thread, so it lives in registers: two
 // 16-bit slots per VGPR, in a 32 + 8 element vector indexed by the (uniform)
 // loop counter, which the compiler lowers to relative register addressing.
-// LDS only holds the shared entry data, which keeps four work items resident
-// per CU.  Each wave runs to the longest list among its own 64 points (wnp);
-// shorter lists are padded with SL_EMPTY, which decodes to entry SE_ZERO, an
-// entry whose staged values are zero, so padded slots add exact zeros.
+// Each entry has one 80-byte LDS row: its SZC staged values, its offset F and
+// its (theta, k | mirror) key.  A tap reads the row as five 16-byte pieces
+// (ds_read_b128; ds_read2_b64 moves half the bytes per cycle), which the LDS
+// serves 16 lanes at a time; the plan orders every 16-lane group's slots so
+// that the rows read in one step differ mod 16 or coincide (schedule_group),
+// keeping the reads free of bank conflicts.  Each wave runs to the longest
+// schedule among its four groups (wnp); idle slots read a row another lane
+// reads with tap code SL_IDLE_TAP, whose weight is zero.  Four work items
+// share a CU (4 x 39.7 KB of LDS).
 constexpr int STS = 16;          // output tile edge
 constexpr int SNP = STS * STS;   // points = threads per work item
 constexpr int SWAVES = SNP / 64; // waves per work item
-constexpr int SE_MAX = 512;      // entry rows in LDS per work item (the last is the zero entry)
-constexpr int SE_ZERO = SE_MAX - 1;
+constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
+constexpr int SE_MAX = 496;      // entry rows in LDS per work item
 constexpr int SL_MAX = 80;       // list slots per point per work item
+constexpr int SL_CHUNK = SL_MAX - 8;  // taps per point per work item (room for idle steps)
 constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
-constexpr unsigned short SL_EMPTY = 0xFFFF;  // slot = entry | (9 * i1 + i0) << 9; 0xFFFF -> entry SE_ZERO
-static_assert((SL_EMPTY & 0x1FF) == SE_ZERO && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
+constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
+static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
               "slot encoding");
+
+struct __align__(16) SRow {
+  float2 v[SZC];  // staged values, z = zc + 0 .. SZC - 1
+  float2 f;       // fractional offset (mirror: negated)
+  int2 tk;        // theta, k | mirror << 30
+};
+static_assert(sizeof(SRow) == 80, "five 16-byte pieces per row");
 
 struct SItem {
   long long loff;  // pairs of slots: lists[loff + (j / 2) * 2 SNP + 2 p + (j & 1)], j < nl
-  int eoff, ne;    // entries[eoff, eoff + ne): (sample id << 1) | mirror, ne < SE_MAX
+  int eoff, ne;    // entries[eoff, eoff + ne): (sample id << 1) | mirror, ne <= SE_MAX
   int x0, y0;      // tile origin (l0, l1), centred coordinates
   int nl, slot;    // list slots (even); -1: the item owns the tile, >= 0: partial slot
-  unsigned wnp;     // slot pairs wave w runs (<= nl / 2): bits [8 w, 8 w + 8)
+  unsigned wnp;    // slot pairs wave w runs (<= nl / 2): bits [8 w, 8 w + 8)
   int pad_;
 };
 
@@ -511,9 +524,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
     const SItem* __restrict__ items, const int* __restrict__ entries, const unsigned short* __restrict__ lists,
     const double2* __restrict__ dirs, int n, int H, int nz, long long hzs, int zchunk, float a) {
-  __shared__ float2 V[SZC][SE_MAX];
-  __shared__ float2 F[SE_MAX];
-  __shared__ int2 TK[SE_MAX];
+  __shared__ SRow R[SE_MAX];
   const SItem it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
   const int p = threadIdx.x;
@@ -522,38 +533,36 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int code = entries[it.eoff + e];
     const SampleGeom g = sample_geom(code >> 1, H, dirs);
     const bool mir = (code & 1) != 0;
-    F[e] = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
-    TK[e] = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
+    R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
+    R[e].tk = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
   }
-  if (p < SZC) V[p][SE_ZERO] = make_float2(0.f, 0.f);
-  if (p == SZC) F[SE_ZERO] = make_float2(0.f, 0.f);
   const int npairs = __builtin_amdgcn_readfirstlane((int)((it.wnp >> (8 * (p >> 6))) & 0xFFu));
   const int na = min(npairs, SL_PAIRS_A);
   const unsigned* lp = reinterpret_cast<const unsigned*>(lists + it.loff) + p;
   u32x32 la;
   u32x8 lb;
 #pragma unroll
-  for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0xFFFFFFFFu;
+  for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0u;
 #pragma unroll
-  for (int jp = 0; jp < SL_PAIRS_B; ++jp)
-    lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0xFFFFFFFFu;
+  for (int jp = 0; jp < SL_PAIRS_B; ++jp) lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0u;
   __syncthreads();
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
   const int mask = 2 * n - 1;
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
     for (int e = p; e < it.ne; e += SNP) {
-      const int2 tk = TK[e];
+      const int2 tk = R[e].tk;
       const float2* src = yh + ((long long)tk.x * nz + zc) * H + (tk.y & 0x3FFFFFFF);
       const bool cj = (tk.y >> 30) != 0;
+      float2 v[SZC];
 #pragma unroll
       for (int zz = 0; zz < SZC; ++zz) {
-        if (zz < nzz) {
-          float2 v = src[(long long)zz * H];
-          if (cj) v.y = -v.y;
-          V[zz][e] = v;
-        }
+        v[zz] = zz < nzz ? src[(long long)zz * H] : make_float2(0.f, 0.f);
+        if (cj) v[zz].y = -v[zz].y;
       }
+      float4* dst = reinterpret_cast<float4*>(R[e].v);
+#pragma unroll
+      for (int q = 0; q < SZC / 2; ++q) dst[q] = make_float4(v[2 * q].x, v[2 * q].y, v[2 * q + 1].x, v[2 * q + 1].y);
     }
     __syncthreads();
     Acc ax[SZC], ay[SZC];
@@ -565,14 +574,17 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     auto tap = [&](unsigned u) {
       const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
       const int i1 = t / TAPS, i0 = t - TAPS * i1;
-      const float2 f = F[e];
-      const float dx = (float)(i0 - M) - f.x, dy = (float)(i1 - M) - f.y;
-      const float w = __expf(-a * (dx * dx + dy * dy));
+      const float4* row = reinterpret_cast<const float4*>(&R[e]);
+      const float4 ft = row[SZC / 2];  // f, tk
+      const float dx = (float)(i0 - M) - ft.x, dy = (float)(i1 - M) - ft.y;
+      const float w = t == (int)SL_IDLE_TAP ? 0.f : __expf(-a * (dx * dx + dy * dy));
 #pragma unroll
-      for (int zz = 0; zz < SZC; ++zz) {
-        const float2 v = V[zz][e];
-        ax[zz] += (Acc)(w * v.x);
-        ay[zz] += (Acc)(w * v.y);
+      for (int q = 0; q < SZC / 2; ++q) {
+        const float4 v = row[q];
+        ax[2 * q] += (Acc)(w * v.x);
+        ay[2 * q] += (Acc)(w * v.y);
+        ax[2 * q + 1] += (Acc)(w * v.z);
+        ay[2 * q + 1] += (Acc)(w * v.w);
       }
     };
     for (int jp = 0; jp < na; ++jp) {
@@ -872,6 +884,89 @@ static inline int pmod(int v, int m) {
   return r < 0 ? r + m : r;
 }
 
+// Step order of the taps (row | tap << 9) of one 16-lane group (one tile row
+// of points).  The kernel reads each tap's 80-byte entry row as 16-byte
+// pieces, which the LDS serves 16 lanes per cycle; two rows collide iff they
+// are equal mod 16 (their colour) and differ.  Greedy per step: lanes with the
+// most taps left choose first, a tap whose row another lane already reads
+// (broadcast) before a tap of a free colour, free colours by descending
+// demand.  Lanes without a usable tap idle on a row read in the same step,
+// with tap code SL_IDLE_TAP (weight 0).  If the steps would exceed SL_MAX the
+// lists are used in their own order (bank conflicts, never more steps than
+// the longest list, which chunking keeps <= SL_CHUNK).  Returns the steps.
+static int schedule_group(const std::vector<unsigned short>* L, std::vector<unsigned short>* S) {
+  constexpr int G = SGROUP;
+  unsigned short tap[G][SL_MAX];
+  int beg[G][G], cur[G][G], rem[G] = {}, demand[G] = {};
+  int longest = 0, total = 0;
+  for (int l = 0; l < G; ++l) {
+    const int nt = (int)L[l].size();
+    longest = std::max(longest, nt);
+    total += nt;
+    rem[l] = nt;
+    int k = 0;
+    for (int c = 0; c < G; ++c) {  // counting sort by colour
+      beg[l][c] = cur[l][c] = k;
+      for (unsigned short u : L[l])
+        if ((u & 15) == c) tap[l][k++] = u;
+      demand[c] += k - beg[l][c];
+    }
+  }
+  auto end_of = [&](int l, int c) { return c + 1 < G ? beg[l][c + 1] : (int)L[l].size(); };
+  for (int l = 0; l < G; ++l) S[l].clear();
+  int steps = 0;
+  bool ok = true;
+  while (total > 0) {
+    if (steps == SL_MAX) {
+      ok = false;
+      break;
+    }
+    int row_of[G];
+    for (int c = 0; c < G; ++c) row_of[c] = -1;
+    int order[G];
+    for (int l = 0; l < G; ++l) order[l] = l;
+    std::sort(order, order + G, [&](int u, int v) { return rem[u] > rem[v]; });
+    int pick[G];
+    int any_row = -1;
+    for (int oi = 0; oi < G; ++oi) {
+      const int l = order[oi];
+      pick[l] = -1;
+      if (rem[l] == 0) continue;
+      int best = -1, best_c = -1;
+      for (int c = 0; c < G && best < 0; ++c) {  // broadcast: a row already read this step
+        if (row_of[c] < 0) continue;
+        for (int k = cur[l][c]; k < end_of(l, c); ++k)
+          if ((tap[l][k] & 0x1FF) == row_of[c]) {
+            best = k;
+            best_c = c;
+            break;
+          }
+      }
+      if (best < 0) {  // the free colour in most demand
+        for (int c = 0; c < G; ++c)
+          if (row_of[c] < 0 && cur[l][c] < end_of(l, c) && (best_c < 0 || demand[c] > demand[best_c])) best_c = c;
+        if (best_c >= 0) best = cur[l][best_c];
+      }
+      if (best < 0) continue;
+      const unsigned short u = tap[l][best];
+      std::swap(tap[l][best], tap[l][cur[l][best_c]]);  // consume from the colour range
+      ++cur[l][best_c];
+      --rem[l];
+      --total;
+      --demand[best_c];
+      row_of[best_c] = u & 0x1FF;
+      any_row = u & 0x1FF;
+      pick[l] = u;
+    }
+    for (int l = 0; l < G; ++l)
+      S[l].push_back(pick[l] >= 0 ? (unsigned short)pick[l] : (unsigned short)(any_row | (SL_IDLE_TAP << 9)));
+    ++steps;
+  }
+  if (ok) return steps;
+  for (int l = 0; l < G; ++l) S[l] = L[l];  // conflicting but no longer than the longest list
+  return longest;
+}
+
 // Items of one 16 x 16 output tile (tx, ty) from the entry codes touching it.
 static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vector<int>& codes, SBinning& B) {
   const int n = G.n, twon = 2 * n;
@@ -908,14 +1003,14 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
     }
     tap_off[e + 1] = (int)tap_pt.size();
   }
-  // greedy chunks of entries: < SE_ZERO entries (row SE_ZERO stays zero), <= SL_MAX taps per point
+  // greedy chunks of entries: <= SE_MAX entries, <= SL_CHUNK taps per point
   std::vector<int> cnt(SNP, 0);
   std::vector<std::pair<int, int>> chunks;
   int e0 = 0;
   for (int e = 0; e < ne; ++e) {
-    bool fits = (e - e0) < SE_ZERO;
+    bool fits = (e - e0) < SE_MAX;
     for (int t = tap_off[e]; fits && t < tap_off[e + 1]; ++t)
-      if (cnt[tap_pt[t]] + 1 > SL_MAX) fits = false;
+      if (cnt[tap_pt[t]] + 1 > SL_CHUNK) fits = false;
     if (!fits) {
       chunks.push_back({e0, e});
       std::fill(cnt.begin(), cnt.end(), 0);
@@ -931,33 +1026,37 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
     B.nslots += (int)chunks.size();
     B.splits.push_back({x0, y0, slot0, (int)chunks.size()});
   }
-  std::vector<std::vector<unsigned short>> L(SNP);
+  std::vector<std::vector<unsigned short>> L(SNP), S(SNP);
   for (size_t c = 0; c < chunks.size(); ++c) {
     const int a = chunks[c].first, b = chunks[c].second;
     for (auto& l : L) l.clear();
     for (int e = a; e < b; ++e)
       for (int t = tap_off[e]; t < tap_off[e + 1]; ++t) L[tap_pt[t]].push_back((unsigned short)((e - a) | tap_code[t]));
-    size_t nl = 0;
-    for (auto& l : L) nl = std::max(nl, l.size());
-    nl = (nl + 1) & ~size_t(1);  // slots are read in pairs
+    // step order per 16-lane group, then every wave runs to its longest group
+    int glen[SNP / SGROUP];
+    for (int g = 0; g < SNP / SGROUP; ++g) glen[g] = schedule_group(&L[g * SGROUP], &S[g * SGROUP]);
     SItem it{};
+    int nl = 0;
     for (int w = 0; w < SWAVES; ++w) {
-      size_t m = 0;
-      for (int pt = w * 64; pt < w * 64 + 64; ++pt) m = std::max(m, L[pt].size());
-      it.wnp |= (unsigned)((m + 1) / 2) << (8 * w);
+      int m = 0;
+      for (int g = w * 64 / SGROUP; g < (w + 1) * 64 / SGROUP; ++g) m = std::max(m, glen[g]);
+      m = (m + 1) & ~1;  // slots are read in pairs
+      nl = std::max(nl, m);
+      it.wnp |= (unsigned)(m / 2) << (8 * w);
     }
     it.x0 = x0;
     it.y0 = y0;
     it.ne = b - a;
     it.eoff = (int)B.entries.size();
-    it.nl = (int)nl;
+    it.nl = nl;
     it.loff = (long long)B.lists.size();
     it.slot = split ? slot0 + (int)c : -1;
     B.entries.insert(B.entries.end(), codes.begin() + a, codes.begin() + b);
-    for (size_t j = 0; j < nl; j += 2)
+    const unsigned short idle = (unsigned short)(SL_IDLE_TAP << 9);  // row 0, weight 0: a broadcast read
+    for (int j = 0; j < nl; j += 2)
       for (int pt = 0; pt < SNP; ++pt) {
-        B.lists.push_back(j < L[pt].size() ? L[pt][j] : SL_EMPTY);
-        B.lists.push_back(j + 1 < L[pt].size() ? L[pt][j + 1] : SL_EMPTY);
+        B.lists.push_back(j < (int)S[pt].size() ? S[pt][j] : idle);
+        B.lists.push_back(j + 1 < (int)S[pt].size() ? S[pt][j + 1] : idle);
       }
     B.items.push_back(it);
   }
