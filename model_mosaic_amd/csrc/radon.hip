@@ -99,11 +99,22 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
                                                            const float* __restrict__ phx, const float* __restrict__ phy,
                                                            int n, int nz, long long fzs, const float2* __restrict__ tw) {
   using Sh = RowReg<L>;
+  constexpr int R2 = 2 * Sh::C;
   __shared__ float2 lds[Sh::LDS];
+  __shared__ float2* rowp[R2];  // destination row r of this workgroup in the half spectrum (null: none)
   int c, t;
   Sh::map(threadIdx.x, c, t);
   const int half = n / 2;
   const long long npairs = (long long)nz * half;
+  if (threadIdx.x < R2) {
+    const long long pr = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
+    float2* rp = nullptr;
+    if (pr < npairs) {
+      const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (threadIdx.x & 1);
+      rp = fde + (long long)zz * fzs + 4LL * (half + yy);  // hs_idx(half + yy, col) = this + col part
+    }
+    rowp[threadIdx.x] = rp;
+  }
   const long long pair = (long long)blockIdx.x * Sh::C + c;
   const bool valid = pair < npairs;
   int z = 0, y0 = 0;
@@ -131,21 +142,18 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   __syncthreads();
   // unpack A = (Z_k + conj Z_{L-k})/2 (row y0), B = (Z_k - conj Z_{L-k})/(2i) (row y0+1);
   // thread order: column within block, then the 2C rows of this workgroup, then block
-  constexpr int R2 = 2 * Sh::C;
   const int nblk = n / 4 + 1;
   for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
-    const int cc = r >> 1;
-    const long long pr = (long long)blockIdx.x * Sh::C + cc;
-    if (col > n || pr >= npairs) continue;
-    const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (r & 1);
-    const float2* xc = lds + cc * Sh::S;
+    float2* rp = rowp[r];
+    if (col > n || rp == nullptr) continue;
+    const float2* xc = lds + (r >> 1) * Sh::S;
     const float2 zk = xc[lpad(col)], zm = xc[lpad((L - col) & (L - 1))];
     const float2 w = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
                              : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-    fde[(long long)zz * fzs + hs_idx(half + yy, col, 2 * n)] = w;
+    rp[(long long)blk * 4 * L + q] = w;  // hs_idx(row, col, L) = 4 row + 4 L (col >> 2) + (col & 3)
   }
 }
 
@@ -632,9 +640,18 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
 // (after the column IFFT), centre crop, times phi.  Real output packs two rows
 // per complex FFT; complex output packs the real/imaginary channels of one row.
 // The 2C input rows of a workgroup are first staged block-ordered (coalesced):
-// row A of transform c at x_c[0..n], row B at x_c[n+1..2n+1].
+// row A of transform c at x_c[lpad(col)], row B at x_c[OFF + lpad(col)], with
+// OFF = 8 (mod 32) bank pairs so that the staging writes of a half-wave (8
+// columns x rows A, B of 2 transforms at a 16-pair stride) hit 32 distinct
+// bank pairs.  Row addresses are worked out once per workgroup.
 template <int L>
-using C2rShape = RowReg<L, 2>;  // staging holds 2 (n + 1) values per transform
+struct C2rOff {
+  static constexpr int A = lpad(L / 2) + 1;  // > every lpad(col), col <= n
+  // = 8 (mod 32); short transforms (many per workgroup) skip the alignment to fit LDS
+  static constexpr int OFF = L >= 512 ? A + ((8 - A % 32) + 32) % 32 : A;
+};
+template <int L>
+using C2rShape = RowReg<L, C2rOff<L>::OFF + C2rOff<L>::A - (L + L / 16)>;
 
 template <int L, bool CPLX>
 __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __restrict__ Hre,
@@ -643,30 +660,41 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
                                                               int n, int nz, long long hzs, float scale,
                                                               const float2* __restrict__ tw) {
   using Sh = C2rShape<L>;
+  constexpr int OFF = C2rOff<L>::OFF;
+  constexpr int R2 = 2 * Sh::C;
+  static_assert(OFF + C2rOff<L>::A <= Sh::S, "staging fits the transform's LDS");
   __shared__ float2 lds[Sh::LDS];
+  __shared__ const float2* rowp[R2];  // source row r (transform r / 2, A/B = r & 1); null: none
   const int half = n / 2;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
-  constexpr int R2 = 2 * Sh::C;
+  if (threadIdx.x < R2) {
+    const long long u = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
+    const int which = threadIdx.x & 1;
+    const float2* rp = nullptr;
+    if (u < nunits) {
+      int z, row;
+      if (CPLX) {
+        z = (int)(u / n);
+        row = half + (int)(u - (long long)z * n);
+        rp = which ? Him : Hre;
+      } else {
+        z = (int)(u / half);
+        row = half + 2 * (int)(u - (long long)z * half) + which;
+        rp = Hre;
+      }
+      rp += (long long)z * hzs + 4LL * row;  // hs_idx(row, col) = this + col part
+    }
+    rowp[threadIdx.x] = rp;
+  }
+  __syncthreads();
   const int nblk = n / 4 + 1;
   for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
-    const int cc = r >> 1, which = r & 1;
-    const long long u = (long long)blockIdx.x * Sh::C + cc;
-    if (col > n || u >= nunits) continue;
-    int z, row;
-    const float2* src;
-    if (CPLX) {
-      z = (int)(u / n);
-      row = half + (int)(u - (long long)z * n);
-      src = which ? Him : Hre;
-    } else {
-      z = (int)(u / half);
-      row = half + 2 * (int)(u - (long long)z * half) + which;
-      src = Hre;
-    }
-    lds[cc * Sh::S + lpad(which * (n + 1) + col)] = src[(long long)z * hzs + hs_idx(row, col, L)];
+    const float2* rp = rowp[r];
+    if (col > n || rp == nullptr) continue;
+    lds[(r >> 1) * Sh::S + (r & 1) * OFF + lpad(col)] = rp[(long long)blk * 4 * L + q];
   }
   __syncthreads();
   int c, t;
@@ -679,14 +707,14 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
     float2 a, b;
     if (kx <= n) {
       a = x[lpad(kx)];
-      b = x[lpad(n + 1 + kx)];
+      b = x[OFF + lpad(kx)];
       if (kx == 0 || kx == n) {
         a.y = 0.f;
         b.y = 0.f;
       }
     } else {
       a = cconj(x[lpad(L - kx)]);
-      b = cconj(x[lpad(n + 1 + L - kx)]);
+      b = cconj(x[OFF + lpad(L - kx)]);
     }
     v[m] = make_float2(a.x - b.y, a.y + b.x);
   }
