@@ -15,6 +15,9 @@ Also reported in the same JSON line:
     batched/plain ratio;
   * the forward+backward model iteration (nz=128 x 1024^2, ntheta=1024, 30 keV,
     z = 0.2 m, pixel 1.38 um, Poisson data at 1e4 photons/pixel) in s/iter;
+  * Fresnel D and DT of 128 complex64 frames 2048 x 2048 (psi = exp(i p - 1e-3 p)
+    of a smooth random field p with a 2 rad peak, 30 keV, pixel 0.69 um,
+    z in {0.05, 0.2, 0.8} m) in frames/s, against B_D = 4 nz n 8 bytes per frame;
   * cpu_baseline: the NumPy FP32 restatement (oracle) of R+RT on the host,
     one n=2048 slice at 256 and 512 angles, extrapolated linearly in ntheta
     to the 2048-angle slice, reported in slices/s.
@@ -82,6 +85,40 @@ def cpu_baseline():
     }
 
 
+def run_fresnel(torch, reps=3):
+    """D and DT of 128 frames of 2048 x 2048 at the three distances; seconds per call, frames/s."""
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.processing.propagation import Propagation
+    from model_mosaic_amd.step_kernels import psi_from_proj
+
+    b, nz, n = 128, 2048, 2048
+    wl, vox, dist = 1.24e-9 / 30.0, 0.69e-6, [0.05, 0.2, 0.8]
+    dev = torch.device("cuda", 0)
+    p = phantom.smooth_field(b, nz, n, peak=2.0, seed=3, device=dev)
+    psi = psi_from_proj(p, 1.0, 1e-3)  # exp(i p - 1e-3 p)
+    del p
+    out = torch.empty_like(psi)
+    prop = Propagation(n, nz, b, len(dist), wl, vox, dist)
+    frame_bytes = 4 * nz * n * 8
+    res = {"config": f"{b} frames {nz}x{n} c64, 30keV, pixel 0.69um, z={dist} m, symmetric pad",
+           "algorithmic_bytes_per_frame": frame_bytes}
+    for name, f in (("D", prop.D), ("DT", prop.DT)):
+        for j in range(len(dist)):
+            f(psi, j, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for j in range(len(dist)):
+                f(psi, j, out=out)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / (reps * len(dist))
+        res[name] = {"ms_per_call": round(dt * 1e3, 4), "frames_per_s": round(b / dt, 1),
+                     "achieved_GBs": round(b * frame_bytes / dt / 1e9, 1),
+                     "hbm_frac": round(b * frame_bytes / dt / 1e9 / PEAK_HBM_GBS, 4)}
+    prop.free()
+    return res
+
+
 def run_model(torch, steps=3, warmup=1):
     import numpy as np
 
@@ -140,6 +177,7 @@ def main():
     ap.add_argument("--no-model", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fresnel", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -239,6 +277,14 @@ def main():
         model = run_model(torch)
         print(f"[bench] model {model['s_per_iter'] * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
 
+    fres = None
+    if not a.no_fresnel:
+        print("[bench] Fresnel D/DT, 128 frames 2048^2", file=sys.stderr, flush=True)
+        fres = run_fresnel(torch)
+        print(f"[bench] D {fres['D']['ms_per_call']:.2f} ms, DT {fres['DT']['ms_per_call']:.2f} ms per 128 frames",
+              file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not a.no_cpu:
         print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
@@ -268,6 +314,7 @@ def main():
         "plain_one_slice_slices_per_s": None if plain is None else round(plain, 3),
         "batched_over_plain": None if plain is None else round(value / plain, 3),
         "model_fwd_bwd": model,
+        "fresnel_D_DT": fres,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
