@@ -530,13 +530,15 @@ typedef unsigned u32x8 __attribute__((ext_vector_type(SL_PAIRS_B)));
 template <typename Acc>
 __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
-    const SItem* __restrict__ items, const int* __restrict__ entries, const unsigned short* __restrict__ lists,
-    const double2* __restrict__ dirs, int n, int H, int nz, long long hzs, int zchunk, float a) {
+    const SItem* __restrict__ items, const unsigned char* __restrict__ perm, const int* __restrict__ entries,
+    const unsigned short* __restrict__ lists, const double2* __restrict__ dirs, int n, int H, int nz, long long hzs,
+    int zchunk, float a) {
   __shared__ SRow R[SE_MAX];
   const SItem it = items[blockIdx.x];
   const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
-  const int p = threadIdx.x;
-  const int l0 = it.x0 + (p % STS), l1 = it.y0 + (p / STS);
+  const int p = threadIdx.x;                                 // lane: list lp[.. p ..]
+  const int pt = perm[(long long)blockIdx.x * SNP + p];     // the tile point it owns
+  const int l0 = it.x0 + (pt % STS), l1 = it.y0 + (pt / STS);
   for (int e = p; e < it.ne; e += SNP) {
     const int code = entries[it.eoff + e];
     const SampleGeom g = sample_geom(code >> 1, H, dirs);
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         if (it.slot < 0)
           Hb[(long long)(zc + zz) * hzs + hs_idx(l1 & mask, l0, 2 * n)] = val;
         else
-          partial[((long long)it.slot * nz + zc + zz) * SNP + p] = val;
+          partial[((long long)it.slot * nz + zc + zz) * SNP + pt] = val;
       }
     }
     __syncthreads();
@@ -901,6 +903,7 @@ static void build_gather(const HostGeom& G, int root, Binning& B) {
 
 struct SBinning {
   std::vector<SItem> items;
+  std::vector<unsigned char> perm;  // per item, SNP bytes: the tile point lane p owns
   std::vector<int> entries;
   std::vector<unsigned short> lists;
   std::vector<SplitTile> splits;
@@ -1001,6 +1004,7 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
   const int x0 = tx * STS, y0 = ty * STS - n;
   const int ne = (int)codes.size();
   if (ne == 0) {  // no sample touches the tile: the item writes zeros
+    for (int pt = 0; pt < SNP; ++pt) B.perm.push_back((unsigned char)pt);
     SItem it{};
     it.loff = (long long)B.lists.size();
     it.eoff = (int)B.entries.size();
@@ -1054,15 +1058,24 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
     B.nslots += (int)chunks.size();
     B.splits.push_back({x0, y0, slot0, (int)chunks.size()});
   }
-  std::vector<std::vector<unsigned short>> L(SNP), S(SNP);
+  std::vector<std::vector<unsigned short>> L(SNP), LP(SNP), S(SNP);
   for (size_t c = 0; c < chunks.size(); ++c) {
     const int a = chunks[c].first, b = chunks[c].second;
     for (auto& l : L) l.clear();
+    for (auto& l : LP) l.clear();
     for (int e = a; e < b; ++e)
       for (int t = tap_off[e]; t < tap_off[e + 1]; ++t) L[tap_pt[t]].push_back((unsigned short)((e - a) | tap_code[t]));
-    // step order per 16-lane group, then every wave runs to its longest group
+    // lanes take the points in order of decreasing list length, so each wave (and each 16-lane
+    // group) holds lists of similar length; then the step order of each group
+    int perm[SNP];
+    for (int pt = 0; pt < SNP; ++pt) perm[pt] = pt;
+    std::stable_sort(perm, perm + SNP, [&](int u, int v) { return L[u].size() > L[v].size(); });
+    for (int l = 0; l < SNP; ++l) {
+      LP[l].swap(L[perm[l]]);
+      B.perm.push_back((unsigned char)perm[l]);
+    }
     int glen[SNP / SGROUP];
-    for (int g = 0; g < SNP / SGROUP; ++g) glen[g] = schedule_group(&L[g * SGROUP], &S[g * SGROUP]);
+    for (int g = 0; g < SNP / SGROUP; ++g) glen[g] = schedule_group(&LP[g * SGROUP], &S[g * SGROUP]);
     SItem it{};
     int nl = 0;
     for (int w = 0; w < SWAVES; ++w) {
@@ -1142,6 +1155,7 @@ static void build_scatter_lists(const HostGeom& G, SBinning& B) {
       B.splits.push_back(s);
     }
     B.entries.insert(B.entries.end(), P.entries.begin(), P.entries.end());
+    B.perm.insert(B.perm.end(), P.perm.begin(), P.perm.end());
     B.lists.insert(B.lists.end(), P.lists.begin(), P.lists.end());
     B.nslots += P.nslots;
   }
@@ -1186,6 +1200,7 @@ struct mm_radon {
   unsigned short* gorder = nullptr;
   int n_gitems = 0;
   SItem* sitems = nullptr;
+  unsigned char* sperm = nullptr;
   int* sentries = nullptr;
   unsigned short* slists = nullptr;
   int n_sitems = 0;
@@ -1212,7 +1227,8 @@ static void radon_free(mm_radon* p) {
     for (auto& x : e)
       if (x) (void)hipEventDestroy(x);
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
-                  p->gitems, p->gsamples, p->gorder, p->sitems, p->sentries, p->slists, p->splits};
+                  p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
+                  p->splits};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1322,10 +1338,12 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   const long long hzs = p->zs;
   if (p->flags & MM_RADON_FP64_TILES)
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
-                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk, p->a);
+                       p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
+                       p->a);
   else
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
-                       p->partial, p->sitems, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk, p->a);
+                       p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
+                       p->a);
   int rc = check_launch("scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
@@ -1416,7 +1434,7 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
       (rc = upload(&p->sig, sig, bytes)) || (rc = upload(&p->gitems, BG.items, bytes)) ||
       (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->gorder, BG.order, bytes)) ||
-      (rc = upload(&p->sitems, BS.items, bytes)) ||
+      (rc = upload(&p->sitems, BS.items, bytes)) || (rc = upload(&p->sperm, BS.perm, bytes)) ||
       (rc = upload(&p->sentries, BS.entries, bytes)) || (rc = upload(&p->slists, BS.lists, bytes)) ||
       (rc = upload(&p->splits, BS.splits, bytes)))
     return bail(rc);
