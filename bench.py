@@ -178,6 +178,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fresnel", action="store_true")
+    ap.add_argument("--gather-zp", type=int, default=2, choices=(1, 2),
+                    help="z slices per gather patch element (TomoReal gather_zp)")
+    ap.add_argument("--gather-xcd-group", type=int, default=0, help="override the gather XCD group knob")
+    ap.add_argument("--scatter-xcd-group", type=int, default=0, help="override the scatter XCD group knob")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -208,8 +212,12 @@ def main():
     print(f"[bench] building phantom {nz}x{n}x{n}, ntheta={nt}", file=sys.stderr, flush=True)
     obj = phantom.ellipsoids(nz, n, seed=0, noise=0.01, device=dev)
     t0 = time.perf_counter()
-    tomo = TomoReal(n, nz, theta)
+    tomo = TomoReal(n, nz, theta, gather_zp=a.gather_zp)
     plan_s = time.perf_counter() - t0
+    if a.gather_xcd_group:
+        _lib.check(lib.mm_radon_tune(tomo._plan, _lib.MM_RADON_TUNE_GATHER_XCD_GROUP, a.gather_xcd_group), "tune")
+    if a.scatter_xcd_group:
+        _lib.check(lib.mm_radon_tune(tomo._plan, _lib.MM_RADON_TUNE_SCATTER_XCD_GROUP, a.scatter_xcd_group), "tune")
     sino = torch.empty((nt, nz, n), dtype=torch.float32, device=dev)
     back = torch.empty((nz, n, n), dtype=torch.float32, device=dev)
     print(f"[bench] plan {plan_s:.2f} s, workspace {tomo.workspace_bytes() / 2**30:.2f} GiB; warmup",

@@ -49,9 +49,13 @@ int mm_version(void);
  * mm_radon_create: n a power of two in [16, 8192], nz_max >= 1,
  *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
  *   the reference stores theta as float32 -- pass those values).
- *   flags: 0, or MM_RADON_FP64_TILES (scatter tiles accumulate in FP64).
+ *   flags: 0 or an OR of
+ *     MM_RADON_FP64_TILES  scatter tiles accumulate in FP64;
+ *     MM_RADON_GATHER_ZP1  gather stages one z slice per round (default: two
+ *                          slices interleaved per patch element).
  */
 #define MM_RADON_FP64_TILES 1u
+#define MM_RADON_GATHER_ZP1 2u
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags);
 
 /* R: obj (nz, n, n) float32 -> sino (ntheta, nz, n) float32, 1 <= nz <= nz_max.
@@ -74,6 +78,16 @@ int mm_radon_adj(mm_radon* plan, const void* sino_dev, int sino_is_c64, void* ob
 #define MM_RADON_NSTAGES 8
 int mm_radon_profile(mm_radon* plan, int enable);
 int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
+
+/* Launch knobs (no counterpart in the reference; used by the tile autotuner).
+ * MM_RADON_TUNE_GATHER_XCD_GROUP / MM_RADON_TUNE_SCATTER_XCD_GROUP: number G
+ * of neighbouring gather / scatter work items that run on one XCD (one L2)
+ * before the next XCD takes the next G; 1 = plain round-robin order.
+ * Results do not depend on the knobs, only the speed does. */
+#define MM_RADON_TUNE_GATHER_XCD_GROUP 1
+#define MM_RADON_TUNE_SCATTER_XCD_GROUP 2
+#define MM_RADON_TUNE_NKNOBS 3
+int mm_radon_tune(mm_radon* plan, int knob, int value);
 
 /* Device bytes the plan holds (workspace + tables). */
 size_t mm_radon_workspace_bytes(const mm_radon* plan);

@@ -17,6 +17,9 @@ MM_ERR_OOM = -2
 MM_ERR_HIP = -3
 MM_ERR_UNSUPPORTED = -4
 MM_RADON_FP64_TILES = 1
+MM_RADON_GATHER_ZP1 = 2
+MM_RADON_TUNE_GATHER_XCD_GROUP = 1
+MM_RADON_TUNE_SCATTER_XCD_GROUP = 2
 MM_RADON_NSTAGES = 8
 RADON_STAGES = ("r_rows_rfft", "r_col_fft", "r_gather", "r_line_ifft",
                 "rt_line_fft", "rt_scatter", "rt_col_ifft", "rt_rows_c2r")
@@ -29,6 +32,7 @@ _SIGS = {
     "mm_radon_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint]),
     "mm_radon_fwd": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "mm_radon_adj": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
+    "mm_radon_tune": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mm_radon_profile": (C.c_int, [_vp, C.c_int]),
     "mm_radon_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int]),
     "mm_radon_workspace_bytes": (C.c_size_t, [_vp]),

@@ -69,6 +69,18 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned w, unsigned total) {
   const unsigned k = w % kXcds, pos = w / kXcds;
   return k * per + (k < rem ? k : rem) + pos;
 }
+// Grouped XCD order: for every k, logical ids [8 G k + G x, 8 G k + G x + G)
+// run on XCD x, so G neighbouring work items (tiles that share halo and
+// input lines) meet in one L2 while dense regions still spread over all
+// XCDs.  Identity for G <= 1 and past the last whole span of 8 G.  A
+// bijection on [0, total).
+__device__ __forceinline__ unsigned xcd_group(unsigned w, unsigned total, unsigned G) {
+  if (G <= 1) return w;
+  const unsigned span = kXcds * G, full = total - total % span;
+  if (w >= full) return w;
+  const unsigned x = w % kXcds, pos = w / kXcds;
+  return (pos / G) * span + x * G + pos % G;
+}
 inline int ilog2(long v) { int r = 0; while ((1L << r) < v) ++r; return r; }
 
 }  // namespace mm

@@ -40,6 +40,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "mm_fft.h"
@@ -215,21 +216,33 @@ __device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* 
 }
 
 // Tiled polar gather (forward R): gh[theta, z, k] = C0 sum ey sum ex F_c.
-// Thread s computes sample samples[start + s] (lane order: each half-wave's
-// window bases are distinct mod 32, see patch_pitch); its results go through
-// LDS so that thread t writes sorted sample t (ascending theta, k: coalesced).
-// Each thread's patch staging addresses, reflections and signs are worked out
-// once per item; the z loop only moves data.
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh,
-                                                            const Item* __restrict__ items,
-                                                            const int* __restrict__ samples,
-                                                            const unsigned short* __restrict__ order,
-                                                            const double2* __restrict__ dirs, int n, int H, int nz,
-                                                            long long fzs, int zchunk, float a, float C0) {
-  __shared__ __align__(16) float2 P[PATCH_ELEMS + 2];  // + a sink for idle staging slots
-  __shared__ float2 O[TILE_THREADS];
-  const Item it = items[blockIdx.x];
-  const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
+// Thread s computes sample samples[start + s]; its results go through LDS so
+// that thread t writes sorted sample t (ascending theta, k: coalesced).
+// Each thread's patch staging addresses and reflections are worked out once
+// per item; the centring sign (-1)^(k0 + k1) is separable and rides in the
+// weights (reflection keeps the parity of k0 and k1), so the z loop only moves
+// data.
+//
+// ZP = z slices per patch element:
+//  * ZP = 1: float2 patch; a 9-tap row is read as five 16-byte pieces from the
+//    window base rounded down to even (a tenth weight of zero absorbs the
+//    parity); lanes of a quarter-wave have distinct (base / 2) mod 16.
+//  * ZP = 2: float4 patch holding slices z and z + 1 of each grid point; a tap
+//    is one 16-byte read for both slices (9 per row, half the barriers per
+//    slice); lanes of a quarter-wave have distinct base mod 16.
+// Either way the 16 pieces of one read hit 16 distinct bank groups.
+template <int ZP>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(ZP == 1 ? 8 : 4, ZP == 1 ? 8 : 4)))
+void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const Item* __restrict__ items,
+                  const int* __restrict__ samples, const unsigned short* __restrict__ order,
+                  const double2* __restrict__ dirs, int n, int H, int nz, long long fzs, int zchunk, float a,
+                  float C0, int xg) {
+  using PT = typename std::conditional<ZP == 1, float2, float4>::type;
+  __shared__ __align__(16) PT P[PATCH_ELEMS + 2];  // + a sink for idle staging slots
+  __shared__ PT O[TILE_THREADS];
+  const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
+  const Item it = items[lw % gridDim.x];
+  const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
   const int pw = it.w + 2 * M, ph = it.h + 2 * M;
   const int pp = patch_pitch(it.w);  // LDS row pitch >= pw
   const int px0 = it.x0 - M, py0 = it.y0 - M;
@@ -238,13 +251,13 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(8,
   const int s = threadIdx.x;
   const bool active = s < it.count;
   const int mask = 2 * n - 1;
-  // staging descriptors: source offset in the z plane; LDS slot << 2 | negate << 1 | conj.
+  // staging descriptors: source offset in the z plane; LDS slot << 1 | conj.
   // Idle slots read offset 0 and write the sink, so the loads issue together.
   int soff[STG_MAX], sdst[STG_MAX];
 #pragma unroll
   for (int e = 0; e < STG_MAX; ++e) {
     const int idx = threadIdx.x + e * TILE_THREADS;
-    sdst[e] = PATCH_ELEMS << 2;
+    sdst[e] = PATCH_ELEMS << 1;
     soff[e] = 0;
     if (idx < nstage) {
       const int q = idx & 3, rest = idx >> 2;
@@ -260,29 +273,33 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(8,
           soff[e] = (int)hs_idx((2 * n - k1) & mask, 2 * n - k0, 2 * n);
           fl = 1;
         }
-        if ((k0 + k1) & 1) fl |= 2;
-        sdst[e] = ((pr * pp + pc) << 2) | fl;
+        sdst[e] = ((pr * pp + pc) << 1) | fl;
       }
     }
   }
-  // ex10[j] weighs patch column (base & ~1) + j: the 9 taps shifted by the parity of base
-  float ex10[TAPS + 1], ey[TAPS];
+  // ZP = 1: ex[j] weighs patch column (base & ~1) + j, the 9 taps shifted by
+  // the parity of base (ex[9] = 0 or ex[0] = 0).  ZP = 2: ex[j] weighs base + j.
+  constexpr int NX = ZP == 1 ? TAPS + 1 : TAPS;
+  float ex[NX], ey[TAPS];
   int base = 0;
   if (active) {
     const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
     const int b = (g.ry - it.y0) * pp + (g.rx - it.x0);
-    const int par = b & 1;
+    const int par = ZP == 1 ? (b & 1) : 0;
     base = b - par;
+    // centring sign of tap (i0, i1): (-1)^(rx - M + i0) (-1)^(ry - M + i1)
+    const float sx = ((g.rx - M) & 1) ? -1.f : 1.f, sy = ((g.ry - M) & 1) ? -C0 : C0;
 #pragma unroll
-    for (int j = 0; j <= TAPS; ++j) {
+    for (int j = 0; j < NX; ++j) {
       const int i = j - par;
       const float dx = (float)(i - M) - g.fx;
-      ex10[j] = (i >= 0 && i < TAPS) ? __expf(-a * dx * dx) : 0.f;
+      const float sg = (i & 1) ? -sx : sx;
+      ex[j] = (i >= 0 && i < TAPS) ? sg * __expf(-a * dx * dx) : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < TAPS; ++i) {
       const float dy = (float)(i - M) - g.fy;
-      ey[i] = C0 * __expf(-a * dy * dy);
+      ey[i] = ((i & 1) ? -sy : sy) * __expf(-a * dy * dy);
     }
   }
   // output of sorted sample t = threadIdx.x
@@ -295,40 +312,91 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(8,
     const int th = id / H;
     out0 = (long long)th * nz * H + (id - th * H);
   }
-  for (int z = z0; z < z1; ++z) {
-    const float2* f = fde + (long long)z * fzs;
-    float2 sv[STG_MAX];
+  float2 sa[STG_MAX], sb[STG_MAX];
+  auto fetch = [&](int z) {
+    const float2* f0 = fde + (long long)z * fzs;
 #pragma unroll
-    for (int e = 0; e < STG_MAX; ++e) sv[e] = f[soff[e]];
+    for (int e = 0; e < STG_MAX; ++e) sa[e] = f0[soff[e]];
+    if constexpr (ZP == 2) {
+      const bool two = z + 1 < z1;
+      const float2* f1 = two ? f0 + fzs : f0;
+#pragma unroll
+      for (int e = 0; e < STG_MAX; ++e) sb[e] = f1[soff[e]];
+      if (!two)
+#pragma unroll
+        for (int e = 0; e < STG_MAX; ++e) sb[e] = make_float2(0.f, 0.f);
+    }
+  };
+  // ZP = 2 prefetches the next round during the taps; ZP = 1 (64 VGPRs at two
+  // workgroups per CU) has no registers to spare and fetches at the top.
+  if (ZP == 2 && z0 < z1) fetch(z0);
+  for (int z = z0; z < z1; z += ZP) {
+    if constexpr (ZP == 1) fetch(z);
 #pragma unroll
     for (int e = 0; e < STG_MAX; ++e) {
-      float2 v = sv[e];
-      if (sdst[e] & 1) v.y = -v.y;
-      if (sdst[e] & 2) v = make_float2(-v.x, -v.y);
-      P[sdst[e] >> 2] = v;
-    }
-    __syncthreads();
-    if (active) {
-      float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int i1 = 0; i1 < TAPS; ++i1) {
-        const float4* prow = reinterpret_cast<const float4*>(P + base + i1 * pp);  // base, pp even
-        float2 row = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int c = 0; c < (TAPS + 1) / 2; ++c) {
-          const float4 v = prow[c];
-          row.x = fmaf(ex10[2 * c], v.x, row.x);
-          row.y = fmaf(ex10[2 * c], v.y, row.y);
-          row.x = fmaf(ex10[2 * c + 1], v.z, row.x);
-          row.y = fmaf(ex10[2 * c + 1], v.w, row.y);
-        }
-        acc.x = fmaf(ey[i1], row.x, acc.x);
-        acc.y = fmaf(ey[i1], row.y, acc.y);
+      float2 u = sa[e];
+      if (sdst[e] & 1) u.y = -u.y;
+      if constexpr (ZP == 1) {
+        P[sdst[e] >> 1] = u;
+      } else {
+        float2 v = sb[e];
+        if (sdst[e] & 1) v.y = -v.y;
+        P[sdst[e] >> 1] = make_float4(u.x, u.y, v.x, v.y);
       }
-      O[s] = acc;
     }
     __syncthreads();
-    if (writer) gh[out0 + (long long)z * H] = O[src_lane];
+    if (ZP == 2 && z + ZP < z1) fetch(z + ZP);
+    if (active) {
+      if constexpr (ZP == 1) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          const float4* prow = reinterpret_cast<const float4*>(P + base + i1 * pp);  // base, pp even
+          float2 row = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int c = 0; c < (TAPS + 1) / 2; ++c) {
+            const float4 v = prow[c];
+            row.x = fmaf(ex[2 * c], v.x, row.x);
+            row.y = fmaf(ex[2 * c], v.y, row.y);
+            row.x = fmaf(ex[2 * c + 1], v.z, row.x);
+            row.y = fmaf(ex[2 * c + 1], v.w, row.y);
+          }
+          acc.x = fmaf(ey[i1], row.x, acc.x);
+          acc.y = fmaf(ey[i1], row.y, acc.y);
+        }
+        O[s] = acc;
+      } else {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          const float4* prow = P + base + i1 * pp;
+          float4 row = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int i0 = 0; i0 < TAPS; ++i0) {
+            const float4 v = prow[i0];
+            row.x = fmaf(ex[i0], v.x, row.x);
+            row.y = fmaf(ex[i0], v.y, row.y);
+            row.z = fmaf(ex[i0], v.z, row.z);
+            row.w = fmaf(ex[i0], v.w, row.w);
+          }
+          acc.x = fmaf(ey[i1], row.x, acc.x);
+          acc.y = fmaf(ey[i1], row.y, acc.y);
+          acc.z = fmaf(ey[i1], row.z, acc.z);
+          acc.w = fmaf(ey[i1], row.w, acc.w);
+        }
+        O[s] = acc;
+      }
+    }
+    __syncthreads();
+    if (writer) {
+      const PT o = O[src_lane];
+      if constexpr (ZP == 1) {
+        gh[out0 + (long long)z * H] = o;
+      } else {
+        gh[out0 + (long long)z * H] = make_float2(o.x, o.y);
+        if (z + 1 < z1) gh[out0 + (long long)(z + 1) * H] = make_float2(o.z, o.w);
+      }
+    }
   }
 }
 
@@ -472,7 +540,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
 // folded taps of k0 in {0} U [n, 2n).  At plan time every 16 x 16 output tile
 // gets the list of entries touching it and, for each of its points, the
 // (entry, i0, i1) taps landing there.  A workgroup owns one tile: thread p
-// keeps the list of point p in LDS for the whole z loop, the tile's entry
+// keeps the list of point p for the whole z loop, the tile's entry
 // values are staged SZC slices at a time, and each thread accumulates its own
 // point in registers -- no atomics, fixed summation order (bitwise
 // deterministic), one write per output point.  Tiles whose lists exceed the
@@ -480,34 +548,37 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
 // partial tiles that scatter_reduce sums.
 //
 // A point's list is private to its thread, so it lives in registers: two
-// 16-bit slots per VGPR, in a 32 + 8 element vector indexed by the (uniform)
-// loop counter, which the compiler lowers to relative register addressing.
-// Each entry has one 80-byte LDS row: its SZC staged values, its offset F and
-// its (theta, k | mirror) key.  A tap reads the row as five 16-byte pieces
+// 16-bit slots per VGPR.  The Gaussian weight of every slot depends only on
+// the entry's offset and the tap, not on z, so each thread computes its
+// slots' weights once per work item, before the z loop, and keeps them in
+// registers next to the slots' LDS row addresses, in register vectors that
+// the slot loop indexes with its (wave-uniform) counter.
+// Inside the z loop a tap is then four 16-byte LDS reads and 16 FMAs.
+// Each entry has one 80-byte LDS row: its SZC staged values and its offset F.
+// A tap reads the values as four 16-byte pieces
 // (ds_read_b128; ds_read2_b64 moves half the bytes per cycle), which the LDS
-// serves 16 lanes at a time; the plan orders every 16-lane group's slots so
-// that the rows read in one step differ mod 16 or coincide (schedule_group),
-// keeping the reads free of bank conflicts.  Each wave runs to the longest
-// schedule among its four groups (wnp); idle slots read a row another lane
-// reads with tap code SL_IDLE_TAP, whose weight is zero.  Four work items
-// share a CU (4 x 39.7 KB of LDS).
+// serves 16 lanes at a time; with the odd pitch of five pieces per row, rows
+// that differ mod 16 use distinct bank groups, and the plan orders every
+// 16-lane group's slots so that the rows read in one step differ mod 16 or
+// coincide (schedule_group), keeping the reads free of bank conflicts.  Each
+// wave runs to the longest schedule among its four groups (wnp); idle slots
+// read a row another lane reads with tap code SL_IDLE_TAP, whose weight is
+// zero.  Three work items share a CU.
 constexpr int STS = 16;          // output tile edge
 constexpr int SNP = STS * STS;   // points = threads per work item
 constexpr int SWAVES = SNP / 64; // waves per work item
 constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
 constexpr int SE_MAX = 496;      // entry rows in LDS per work item
-constexpr int SL_MAX = 80;       // list slots per point per work item
+constexpr int SL_MAX = 64;       // list slots per point per work item
 constexpr int SL_CHUNK = SL_MAX - 8;  // taps per point per work item (room for idle steps)
-constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
 constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
-static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
-              "slot encoding");
+static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SWAVES <= 4 && SL_MAX / 2 < 256, "slot encoding");
 
 struct __align__(16) SRow {
   float2 v[SZC];  // staged values, z = zc + 0 .. SZC - 1
   float2 f;       // fractional offset (mirror: negated)
-  int2 tk;        // theta, k | mirror << 30
+  int2 pad;       // keeps the pitch at five 16-byte pieces (odd: rows that differ mod 16 use distinct banks)
 };
 static_assert(sizeof(SRow) == 80, "five 16-byte pieces per row");
 
@@ -524,55 +595,96 @@ struct SplitTile {
   int x0, y0, slot0, nslots;
 };
 
-typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
-typedef unsigned u32x8 __attribute__((ext_vector_type(SL_PAIRS_B)));
+typedef unsigned u32x32 __attribute__((ext_vector_type(32)));
+typedef float f32x32 __attribute__((ext_vector_type(32)));
+static_assert(SL_MAX == 64, "slot registers: 32 pairs of offsets, 2 x 32 weights");
 
 template <typename Acc>
-__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
+__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) void scatter_lists(
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
     const SItem* __restrict__ items, const unsigned char* __restrict__ perm, const int* __restrict__ entries,
     const unsigned short* __restrict__ lists, const double2* __restrict__ dirs, int n, int H, int nz, long long hzs,
-    int zchunk, float a) {
+    int zchunk, float a, int xg) {
   __shared__ SRow R[SE_MAX];
-  const SItem it = items[blockIdx.x];
-  const int z0 = blockIdx.y * zchunk, z1 = min(nz, z0 + zchunk);
+  static_assert(SE_MAX <= 2 * SNP, "at most two entries staged per thread");
+  const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
+  const unsigned item = lw % gridDim.x;
+  const SItem it = items[item];
+  const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
   const int p = threadIdx.x;                                 // lane: list lp[.. p ..]
-  const int pt = perm[(long long)blockIdx.x * SNP + p];     // the tile point it owns
+  const int pt = perm[(long long)item * SNP + p];           // the tile point it owns
   const int l0 = it.x0 + (pt % STS), l1 = it.y0 + (pt / STS);
-  for (int e = p; e < it.ne; e += SNP) {
-    const int code = entries[it.eoff + e];
-    const SampleGeom g = sample_geom(code >> 1, H, dirs);
-    const bool mir = (code & 1) != 0;
-    R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
-    R[e].tk = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
+  // entries p and p + SNP: offset into LDS, staging source (z0 plane) and conjugation
+  const float2* src[2] = {yh, yh};
+  bool cj[2] = {false, false};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = p + h * SNP;
+    if (e < it.ne) {
+      const int code = entries[it.eoff + e];
+      const SampleGeom g = sample_geom(code >> 1, H, dirs);
+      const bool mir = (code & 1) != 0;
+      R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
+      src[h] = yh + ((long long)g.th * nz + z0) * H + g.k;
+      cj[h] = mir;
+    }
   }
   const int npairs = __builtin_amdgcn_readfirstlane((int)((it.wnp >> (8 * (p >> 6))) & 0xFFu));
-  const int na = min(npairs, SL_PAIRS_A);
   const unsigned* lp = reinterpret_cast<const unsigned*>(lists + it.loff) + p;
-  u32x32 la;
-  u32x8 lb;
+  u32x32 ro;  // LDS byte offsets of the rows of slots 2 jp (low half) and 2 jp + 1 (high half)
 #pragma unroll
-  for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0u;
-#pragma unroll
-  for (int jp = 0; jp < SL_PAIRS_B; ++jp) lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0u;
+  for (int jp = 0; jp < SL_MAX / 2; ++jp) ro[jp] = jp < npairs ? lp[jp * SNP] : 0u;
   __syncthreads();
+  // weights of the slots, once per work item: slots [0, 32) in wa, [32, 64) in wb
+  f32x32 wa, wb;
+  auto weight = [&](unsigned u) {
+    const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
+    const int i1 = t / TAPS, i0 = t - TAPS * i1;
+    const float2 f = R[e].f;
+    const float dx = (float)(i0 - M) - f.x, dy = (float)(i1 - M) - f.y;
+    const float w = __expf(-a * (dx * dx + dy * dy));
+    return t == (int)SL_IDLE_TAP ? 0.f : w;
+  };
+#pragma unroll
+  for (int jp = 0; jp < SL_MAX / 2; ++jp) {
+    const unsigned u = ro[jp];
+    const float w0 = jp < npairs ? weight(u & 0xFFFFu) : 0.f, w1 = jp < npairs ? weight(u >> 16) : 0.f;
+    if (jp < SL_MAX / 4) {
+      wa[2 * jp] = w0;
+      wa[2 * jp + 1] = w1;
+    } else {
+      wb[2 * (jp - SL_MAX / 4)] = w0;
+      wb[2 * (jp - SL_MAX / 4) + 1] = w1;
+    }
+    ro[jp] = (u & 0x1FFu) * (unsigned)sizeof(SRow) + ((((u >> 16) & 0x1FFu) * (unsigned)sizeof(SRow)) << 16);
+  }
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
   const int mask = 2 * n - 1;
+  const char* rbase = reinterpret_cast<const char*>(R);
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
-    for (int e = p; e < it.ne; e += SNP) {
-      const int2 tk = R[e].tk;
-      const float2* src = yh + ((long long)tk.x * nz + zc) * H + (tk.y & 0x3FFFFFFF);
-      const bool cj = (tk.y >> 30) != 0;
-      float2 v[SZC];
+    __syncthreads();  // the previous round's taps are done with R[].v
 #pragma unroll
-      for (int zz = 0; zz < SZC; ++zz) {
-        v[zz] = zz < nzz ? src[(long long)zz * H] : make_float2(0.f, 0.f);
-        if (cj) v[zz].y = -v[zz].y;
+    for (int h = 0; h < 2; ++h) {
+      const int e = p + h * SNP;
+      if (e < it.ne) {
+        const float2* s = src[h] + (long long)(zc - z0) * H;
+        float2 v[SZC];
+        if (nzz == SZC) {
+#pragma unroll
+          for (int zz = 0; zz < SZC; ++zz) v[zz] = s[(long long)zz * H];
+        } else {
+#pragma unroll
+          for (int zz = 0; zz < SZC; ++zz) v[zz] = zz < nzz ? s[(long long)zz * H] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int zz = 0; zz < SZC; ++zz)
+          if (cj[h]) v[zz].y = -v[zz].y;
+        float4* dst = reinterpret_cast<float4*>(R[e].v);
+#pragma unroll
+        for (int q = 0; q < SZC / 2; ++q)
+          dst[q] = make_float4(v[2 * q].x, v[2 * q].y, v[2 * q + 1].x, v[2 * q + 1].y);
       }
-      float4* dst = reinterpret_cast<float4*>(R[e].v);
-#pragma unroll
-      for (int q = 0; q < SZC / 2; ++q) dst[q] = make_float4(v[2 * q].x, v[2 * q].y, v[2 * q + 1].x, v[2 * q + 1].y);
     }
     __syncthreads();
     Acc ax[SZC], ay[SZC];
@@ -581,13 +693,8 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       ax[zz] = (Acc)0;
       ay[zz] = (Acc)0;
     }
-    auto tap = [&](unsigned u) {
-      const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
-      const int i1 = t / TAPS, i0 = t - TAPS * i1;
-      const float4* row = reinterpret_cast<const float4*>(&R[e]);
-      const float4 ft = row[SZC / 2];  // f, tk
-      const float dx = (float)(i0 - M) - ft.x, dy = (float)(i1 - M) - ft.y;
-      const float w = t == (int)SL_IDLE_TAP ? 0.f : __expf(-a * (dx * dx + dy * dy));
+    auto tap = [&](unsigned off, float w) {
+      const float4* row = reinterpret_cast<const float4*>(rbase + off);
 #pragma unroll
       for (int q = 0; q < SZC / 2; ++q) {
         const float4 v = row[q];
@@ -597,15 +704,18 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         ay[2 * q + 1] += (Acc)(w * v.w);
       }
     };
+    // Runtime loop over the slot pairs; the (wave-uniform) loop counter
+    // indexes the register vectors (relative register addressing).
+    const int na = min(npairs, SL_MAX / 4);
     for (int jp = 0; jp < na; ++jp) {
-      const unsigned u = la[jp];
-      tap(u & 0xFFFFu);
-      tap(u >> 16);
+      const unsigned o = ro[jp];
+      tap(o & 0xFFFFu, wa[2 * jp]);
+      tap(o >> 16, wa[2 * jp + 1]);
     }
-    for (int jp = SL_PAIRS_A; jp < npairs; ++jp) {
-      const unsigned u = lb[jp - SL_PAIRS_A];
-      tap(u & 0xFFFFu);
-      tap(u >> 16);
+    for (int jp = SL_MAX / 4; jp < npairs; ++jp) {
+      const unsigned o = ro[jp];
+      tap(o & 0xFFFFu, wb[2 * (jp - SL_MAX / 4)]);
+      tap(o >> 16, wb[2 * (jp - SL_MAX / 4) + 1]);
     }
     if (l0 <= n) {
       for (int zz = 0; zz < nzz; ++zz) {
@@ -616,7 +726,6 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
           partial[((long long)it.slot * nz + zc + zz) * SNP + pt] = val;
       }
     }
-    __syncthreads();
   }
 }
 
@@ -803,22 +912,25 @@ struct HostGeom {
 };
 
 struct Binning {
+  int zp = 2;                          // z slices per patch element of the gather kernel
   std::vector<Item> items;
   std::vector<int> samples;            // per item, in lane order
   std::vector<unsigned short> order;   // per item: lane of the t-th sample in ascending id order
 };
 
 // Lane order of one gather item (ids ascending): every quarter-wave of 16
-// lanes takes samples from 16 different colours ((window base / 2) mod 16 =
-// the LDS bank group of each 16-byte tap read), drawing from the fullest
+// lanes takes samples from 16 different colours (the LDS bank group of each
+// 16-byte tap read: (window base / 2) mod 16 for float2 patch elements, base
+// mod 16 for float4 ones), drawing from the fullest
 // colours first, so the tap reads are conflict-free wherever the colours allow.
 static void gather_lanes(const HostGeom& G, int x0, int y0, int w, const int* ids, int count, Binning& B) {
   const int pp = patch_pitch(w);
+  const int shift = B.zp == 1 ? 1 : 0;  // 16-byte pieces: two float2 (zp = 1) or one float4 (zp = 2) patch elements
   std::vector<int> bucket[16];
   for (int t = 0; t < count; ++t) {
     int rx, ry;
     G.centre(ids[t], rx, ry);
-    bucket[(((ry - y0) * pp + (rx - x0)) >> 1) & 15].push_back(t);
+    bucket[(((ry - y0) * pp + (rx - x0)) >> shift) & 15].push_back(t);
   }
   size_t head[16] = {};
   std::vector<int> lane_of(count);
@@ -1199,6 +1311,8 @@ struct mm_radon {
   int* gsamples = nullptr;
   unsigned short* gorder = nullptr;
   int n_gitems = 0;
+  int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
+  int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   SItem* sitems = nullptr;
   unsigned char* sperm = nullptr;
   int* sentries = nullptr;
@@ -1325,9 +1439,13 @@ static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
   const int zsplit = zsplit_for(p->n_gitems, nz);
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
-  hipLaunchKernelGGL(gather_tiles, dim3((unsigned)p->n_gitems, (unsigned)zblocks), dim3(TILE_THREADS), 0, st, p->fde,
-                     p->gh, p->gitems, p->gsamples, p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a,
-                     p->C0g);
+  const dim3 grid((unsigned)p->n_gitems, (unsigned)zblocks);
+  if (p->gzp == 1)
+    hipLaunchKernelGGL(gather_tiles<1>, grid, dim3(TILE_THREADS), 0, st, p->fde, p->gh, p->gitems, p->gsamples,
+                       p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP]);
+  else
+    hipLaunchKernelGGL(gather_tiles<2>, grid, dim3(TILE_THREADS), 0, st, p->fde, p->gh, p->gitems, p->gsamples,
+                       p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP]);
   return check_launch("gather_tiles");
 }
 
@@ -1336,14 +1454,15 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
   const long long hzs = p->zs;
+  const int xg = p->tune[MM_RADON_TUNE_SCATTER_XCD_GROUP];
   if (p->flags & MM_RADON_FP64_TILES)
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
-                       p->a);
+                       p->a, xg);
   else
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
-                       p->a);
+                       p->a, xg);
   int rc = check_launch("scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
@@ -1372,7 +1491,7 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
-  MM_ARG_CHECK((flags & ~MM_RADON_FP64_TILES) == 0, "mm_radon_create: unknown flags");
+  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1)) == 0, "mm_radon_create: unknown flags");
   const double eps = 1e-3;
   const double mu = -std::log(eps) / (2.0 * n * n);
   const int m = (int)std::ceil(2.0 * n / M_PI * std::sqrt(-mu * std::log(eps) + (mu * n) * (mu * n) / 4.0));
@@ -1425,6 +1544,8 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
     sig[t] = sg;
   }
   Binning BG;
+  BG.zp = (flags & MM_RADON_GATHER_ZP1) ? 1 : 2;
+  p->gzp = BG.zp;
   SBinning BS;
   const int root = std::min(ROOT, n);
   build_gather(G, root, BG);
@@ -1507,6 +1628,15 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   MM_DISPATCH_L(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
   if ((rc = check_launch("rt4_rows_c2r"))) return rc;
   return mark(p, 7, 1, st);
+}
+
+int mm_radon_tune(mm_radon* p, int knob, int value) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_tune: plan is NULL");
+  MM_ARG_CHECK(knob == MM_RADON_TUNE_GATHER_XCD_GROUP || knob == MM_RADON_TUNE_SCATTER_XCD_GROUP,
+               "mm_radon_tune: unknown knob " + std::to_string(knob));
+  MM_ARG_CHECK(value >= 1 && value <= 64, "mm_radon_tune: XCD group size must be in [1, 64]");
+  p->tune[knob] = value;
+  return MM_OK;
 }
 
 int mm_radon_profile(mm_radon* p, int enable) {

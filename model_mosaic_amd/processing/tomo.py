@@ -24,9 +24,11 @@ class TomoReal:
     width), ``nz`` the maximum number of slices per call, ``theta`` the
     projection angles in radians (stored as float32, like the reference).
     ``fp64_tiles`` accumulates the adjoint scatter tiles in FP64.
+    ``gather_zp`` (1 or 2) is the number of z slices the forward gather stages
+    per patch element (2, the default, is the faster variant on gfx950).
     """
 
-    def __init__(self, n, nz, theta, fp64_tiles: bool = False):
+    def __init__(self, n, nz, theta, fp64_tiles: bool = False, gather_zp: int = 2):
         require_gpu()
         self._lib = _lib.load()
         self.n = int(n)
@@ -36,7 +38,9 @@ class TomoReal:
         self.device = torch.device("cuda", torch.cuda.current_device())
         th = np.ascontiguousarray(self.theta.astype(np.float64))
         plan = C.c_void_p()
-        flags = _lib.MM_RADON_FP64_TILES if fp64_tiles else 0
+        if gather_zp not in (1, 2):
+            raise ValueError(f"TomoReal: gather_zp must be 1 or 2, got {gather_zp}")
+        flags = (_lib.MM_RADON_FP64_TILES if fp64_tiles else 0) | (_lib.MM_RADON_GATHER_ZP1 if gather_zp == 1 else 0)
         _lib.check(self._lib.mm_radon_create(C.byref(plan), self.n, self._nz, self.ntheta,
                                              th.ctypes.data_as(C.POINTER(C.c_double)), flags),
                    "TomoReal")

@@ -62,14 +62,16 @@ def test_R_matches_fp32_reference_restatement(tomo_cls):
     assert rel(r, usfft.tomo_R(x, th, np.float64)) < 1e-5
 
 
-@pytest.mark.parametrize("n,nt,span", [(16, 7, np.pi), (32, 33, np.pi), (128, 64, 2 * np.pi), (512, 96, np.pi)])
-def test_sizes_vs_fp64(tomo_cls, n, nt, span):
-    nz = 2
+@pytest.mark.parametrize("n,nt,span,nz,zp", [(16, 7, np.pi, 2, 2), (32, 33, np.pi, 3, 2), (128, 64, 2 * np.pi, 2, 2),
+                                             (512, 96, np.pi, 2, 2), (128, 64, 2 * np.pi, 3, 1), (512, 96, np.pi, 5, 1),
+                                             (256, 80, np.pi, 11, 2)])
+def test_sizes_vs_fp64(tomo_cls, n, nt, span, nz, zp):
+    """Both gather variants (one or two z slices per patch element), odd and even nz."""
     th = np.linspace(0, span, nt, endpoint=False).astype(np.float32)
     rng = np.random.default_rng(n + nt)
     x = rng.standard_normal((nz, n, n)).astype(np.float32)
     y = rng.standard_normal((nt, nz, n)).astype(np.float32)
-    t = tomo_cls(n, nz, th)
+    t = tomo_cls(n, nz, th, gather_zp=zp)
     assert rel(t.R(torch.from_numpy(x).cuda()).cpu().numpy(), usfft.tomo_R(x, th, np.float64)) < 1e-5
     assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
 
