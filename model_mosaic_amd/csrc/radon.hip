@@ -54,7 +54,6 @@ constexpr int ROOT = 64;         // quadtree root tile
 constexpr int FLOOR = 8;         // quadtree floor tile
 constexpr int TILE_THREADS = 1024;
 constexpr int SMAX = TILE_THREADS;  // samples per tile work item
-constexpr int PATCH = ROOT + 2 * M;  // gather LDS patch edge (rows)
 
 // Gather patch pitch (complex values per LDS row) for tile widths 64/32/16/8:
 // the smallest pitch >= w + 2M that is 4 (mod 8) with pitch / 4 odd, so the
@@ -548,37 +547,34 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
 // partial tiles that scatter_reduce sums.
 //
 // A point's list is private to its thread, so it lives in registers: two
-// 16-bit slots per VGPR.  The Gaussian weight of every slot depends only on
-// the entry's offset and the tap, not on z, so each thread computes its
-// slots' weights once per work item, before the z loop, and keeps them in
-// registers next to the slots' LDS row addresses, in register vectors that
-// the slot loop indexes with its (wave-uniform) counter.
-// Inside the z loop a tap is then four 16-byte LDS reads and 16 FMAs.
-// Each entry has one 80-byte LDS row: its SZC staged values and its offset F.
-// A tap reads the values as four 16-byte pieces
+// 16-bit slots per VGPR, in a 32 + 8 element vector indexed by the (uniform)
+// loop counter, which the compiler lowers to relative register addressing.
+// Each entry has one 80-byte LDS row: its SZC staged values, its offset F and
+// its (theta, k | mirror) key.  A tap reads the row as five 16-byte pieces
 // (ds_read_b128; ds_read2_b64 moves half the bytes per cycle), which the LDS
-// serves 16 lanes at a time; with the odd pitch of five pieces per row, rows
-// that differ mod 16 use distinct bank groups, and the plan orders every
-// 16-lane group's slots so that the rows read in one step differ mod 16 or
-// coincide (schedule_group), keeping the reads free of bank conflicts.  Each
-// wave runs to the longest schedule among its four groups (wnp); idle slots
-// read a row another lane reads with tap code SL_IDLE_TAP, whose weight is
-// zero.  Three work items share a CU.
+// serves 16 lanes at a time; the plan orders every 16-lane group's slots so
+// that the rows read in one step differ mod 16 or coincide (schedule_group),
+// keeping the reads free of bank conflicts.  Each wave runs to the longest
+// schedule among its four groups (wnp); idle slots read a row another lane
+// reads with tap code SL_IDLE_TAP, whose weight is zero.  Four work items
+// share a CU (4 x 39.7 KB of LDS).
 constexpr int STS = 16;          // output tile edge
 constexpr int SNP = STS * STS;   // points = threads per work item
 constexpr int SWAVES = SNP / 64; // waves per work item
 constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
 constexpr int SE_MAX = 496;      // entry rows in LDS per work item
-constexpr int SL_MAX = 64;       // list slots per point per work item
+constexpr int SL_MAX = 80;       // list slots per point per work item
 constexpr int SL_CHUNK = SL_MAX - 8;  // taps per point per work item (room for idle steps)
+constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
 constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
-static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SWAVES <= 4 && SL_MAX / 2 < 256, "slot encoding");
+static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
+              "slot encoding");
 
 struct __align__(16) SRow {
   float2 v[SZC];  // staged values, z = zc + 0 .. SZC - 1
   float2 f;       // fractional offset (mirror: negated)
-  int2 pad;       // keeps the pitch at five 16-byte pieces (odd: rows that differ mod 16 use distinct banks)
+  int2 tk;        // theta, k | mirror << 30
 };
 static_assert(sizeof(SRow) == 80, "five 16-byte pieces per row");
 
@@ -595,18 +591,16 @@ struct SplitTile {
   int x0, y0, slot0, nslots;
 };
 
-typedef unsigned u32x32 __attribute__((ext_vector_type(32)));
-typedef float f32x32 __attribute__((ext_vector_type(32)));
-static_assert(SL_MAX == 64, "slot registers: 32 pairs of offsets, 2 x 32 weights");
+typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
+typedef unsigned u32x8 __attribute__((ext_vector_type(SL_PAIRS_B)));
 
 template <typename Acc>
-__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) void scatter_lists(
+__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
     const SItem* __restrict__ items, const unsigned char* __restrict__ perm, const int* __restrict__ entries,
     const unsigned short* __restrict__ lists, const double2* __restrict__ dirs, int n, int H, int nz, long long hzs,
     int zchunk, float a, int xg) {
   __shared__ SRow R[SE_MAX];
-  static_assert(SE_MAX <= 2 * SNP, "at most two entries staged per thread");
   const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
   const unsigned item = lw % gridDim.x;
   const SItem it = items[item];
@@ -614,77 +608,40 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const int p = threadIdx.x;                                 // lane: list lp[.. p ..]
   const int pt = perm[(long long)item * SNP + p];           // the tile point it owns
   const int l0 = it.x0 + (pt % STS), l1 = it.y0 + (pt / STS);
-  // entries p and p + SNP: offset into LDS, staging source (z0 plane) and conjugation
-  const float2* src[2] = {yh, yh};
-  bool cj[2] = {false, false};
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int e = p + h * SNP;
-    if (e < it.ne) {
-      const int code = entries[it.eoff + e];
-      const SampleGeom g = sample_geom(code >> 1, H, dirs);
-      const bool mir = (code & 1) != 0;
-      R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
-      src[h] = yh + ((long long)g.th * nz + z0) * H + g.k;
-      cj[h] = mir;
-    }
+  for (int e = p; e < it.ne; e += SNP) {
+    const int code = entries[it.eoff + e];
+    const SampleGeom g = sample_geom(code >> 1, H, dirs);
+    const bool mir = (code & 1) != 0;
+    R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
+    R[e].tk = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
   }
   const int npairs = __builtin_amdgcn_readfirstlane((int)((it.wnp >> (8 * (p >> 6))) & 0xFFu));
+  const int na = min(npairs, SL_PAIRS_A);
   const unsigned* lp = reinterpret_cast<const unsigned*>(lists + it.loff) + p;
-  u32x32 ro;  // LDS byte offsets of the rows of slots 2 jp (low half) and 2 jp + 1 (high half)
+  u32x32 la;
+  u32x8 lb;
 #pragma unroll
-  for (int jp = 0; jp < SL_MAX / 2; ++jp) ro[jp] = jp < npairs ? lp[jp * SNP] : 0u;
+  for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0u;
+#pragma unroll
+  for (int jp = 0; jp < SL_PAIRS_B; ++jp) lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0u;
   __syncthreads();
-  // weights of the slots, once per work item: slots [0, 32) in wa, [32, 64) in wb
-  f32x32 wa, wb;
-  auto weight = [&](unsigned u) {
-    const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
-    const int i1 = t / TAPS, i0 = t - TAPS * i1;
-    const float2 f = R[e].f;
-    const float dx = (float)(i0 - M) - f.x, dy = (float)(i1 - M) - f.y;
-    const float w = __expf(-a * (dx * dx + dy * dy));
-    return t == (int)SL_IDLE_TAP ? 0.f : w;
-  };
-#pragma unroll
-  for (int jp = 0; jp < SL_MAX / 2; ++jp) {
-    const unsigned u = ro[jp];
-    const float w0 = jp < npairs ? weight(u & 0xFFFFu) : 0.f, w1 = jp < npairs ? weight(u >> 16) : 0.f;
-    if (jp < SL_MAX / 4) {
-      wa[2 * jp] = w0;
-      wa[2 * jp + 1] = w1;
-    } else {
-      wb[2 * (jp - SL_MAX / 4)] = w0;
-      wb[2 * (jp - SL_MAX / 4) + 1] = w1;
-    }
-    ro[jp] = (u & 0x1FFu) * (unsigned)sizeof(SRow) + ((((u >> 16) & 0x1FFu) * (unsigned)sizeof(SRow)) << 16);
-  }
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
   const int mask = 2 * n - 1;
-  const char* rbase = reinterpret_cast<const char*>(R);
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
-    __syncthreads();  // the previous round's taps are done with R[].v
+    for (int e = p; e < it.ne; e += SNP) {
+      const int2 tk = R[e].tk;
+      const float2* src = yh + ((long long)tk.x * nz + zc) * H + (tk.y & 0x3FFFFFFF);
+      const bool cj = (tk.y >> 30) != 0;
+      float2 v[SZC];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = p + h * SNP;
-      if (e < it.ne) {
-        const float2* s = src[h] + (long long)(zc - z0) * H;
-        float2 v[SZC];
-        if (nzz == SZC) {
-#pragma unroll
-          for (int zz = 0; zz < SZC; ++zz) v[zz] = s[(long long)zz * H];
-        } else {
-#pragma unroll
-          for (int zz = 0; zz < SZC; ++zz) v[zz] = zz < nzz ? s[(long long)zz * H] : make_float2(0.f, 0.f);
-        }
-#pragma unroll
-        for (int zz = 0; zz < SZC; ++zz)
-          if (cj[h]) v[zz].y = -v[zz].y;
-        float4* dst = reinterpret_cast<float4*>(R[e].v);
-#pragma unroll
-        for (int q = 0; q < SZC / 2; ++q)
-          dst[q] = make_float4(v[2 * q].x, v[2 * q].y, v[2 * q + 1].x, v[2 * q + 1].y);
+      for (int zz = 0; zz < SZC; ++zz) {
+        v[zz] = zz < nzz ? src[(long long)zz * H] : make_float2(0.f, 0.f);
+        if (cj) v[zz].y = -v[zz].y;
       }
+      float4* dst = reinterpret_cast<float4*>(R[e].v);
+#pragma unroll
+      for (int q = 0; q < SZC / 2; ++q) dst[q] = make_float4(v[2 * q].x, v[2 * q].y, v[2 * q + 1].x, v[2 * q + 1].y);
     }
     __syncthreads();
     Acc ax[SZC], ay[SZC];
@@ -693,8 +650,17 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       ax[zz] = (Acc)0;
       ay[zz] = (Acc)0;
     }
-    auto tap = [&](unsigned off, float w) {
-      const float4* row = reinterpret_cast<const float4*>(rbase + off);
+    auto tap = [&](unsigned u) {
+      const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
+      const int i1 = t / TAPS, i0 = t - TAPS * i1;
+      const float4* row = reinterpret_cast<const float4*>(&R[e]);
+      // f read as a full 16-byte piece (one LDS cycle per 16 lanes, issued with the
+      // value pieces); the weight is computed for every slot and zeroed for idle ones
+      // (a select, not a branch around the read)
+      const float4 ft = row[SZC / 2];  // f, tk
+      const float dx = (float)(i0 - M) - ft.x, dy = (float)(i1 - M) - ft.y;
+      const float we = __expf(-a * (dx * dx + dy * dy));
+      const float w = t == (int)SL_IDLE_TAP ? 0.f : we;
 #pragma unroll
       for (int q = 0; q < SZC / 2; ++q) {
         const float4 v = row[q];
@@ -704,18 +670,15 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         ay[2 * q + 1] += (Acc)(w * v.w);
       }
     };
-    // Runtime loop over the slot pairs; the (wave-uniform) loop counter
-    // indexes the register vectors (relative register addressing).
-    const int na = min(npairs, SL_MAX / 4);
     for (int jp = 0; jp < na; ++jp) {
-      const unsigned o = ro[jp];
-      tap(o & 0xFFFFu, wa[2 * jp]);
-      tap(o >> 16, wa[2 * jp + 1]);
+      const unsigned u = la[jp];
+      tap(u & 0xFFFFu);
+      tap(u >> 16);
     }
-    for (int jp = SL_MAX / 4; jp < npairs; ++jp) {
-      const unsigned o = ro[jp];
-      tap(o & 0xFFFFu, wb[2 * (jp - SL_MAX / 4)]);
-      tap(o >> 16, wb[2 * (jp - SL_MAX / 4) + 1]);
+    for (int jp = SL_PAIRS_A; jp < npairs; ++jp) {
+      const unsigned u = lb[jp - SL_PAIRS_A];
+      tap(u & 0xFFFFu);
+      tap(u >> 16);
     }
     if (l0 <= n) {
       for (int zz = 0; zz < nzz; ++zz) {
@@ -726,6 +689,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
           partial[((long long)it.slot * nz + zc + zz) * SNP + pt] = val;
       }
     }
+    __syncthreads();
   }
 }
 
