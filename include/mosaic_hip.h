@@ -89,6 +89,12 @@ int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
 #define MM_RADON_TUNE_NKNOBS 3
 int mm_radon_tune(mm_radon* plan, int knob, int value);
 
+/* Host-only self-check (no counterpart in the reference; for tests on machines
+ * without a GPU): builds the gather / scatter tables of an (n, theta) plan and
+ * checks them against the kernels' index arithmetic.  MM_OK or MM_ERR_ARG with
+ * the first violation in mm_last_error(). */
+int mm_radon_plan_check(int n, int ntheta, const double* theta_host);
+
 /* Device bytes the plan holds (workspace + tables). */
 size_t mm_radon_workspace_bytes(const mm_radon* plan);
 void mm_radon_destroy(mm_radon* plan);

@@ -33,6 +33,7 @@ _SIGS = {
     "mm_radon_fwd": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "mm_radon_adj": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "mm_radon_tune": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mm_radon_plan_check": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "mm_radon_profile": (C.c_int, [_vp, C.c_int]),
     "mm_radon_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int]),
     "mm_radon_workspace_bytes": (C.c_size_t, [_vp]),

@@ -32,12 +32,15 @@ int twiddle_bank(const float2** out) {
   std::lock_guard<std::mutex> lock(g_bank_mu);
   if ((int)g_bank.size() <= dev) g_bank.resize(dev + 1, nullptr);
   if (!g_bank[dev]) {
-    std::vector<float2> host(2 * kMaxFFT, make_float2(0.f, 0.f));
-    for (int L = 2; L <= kMaxFFT; L *= 2)
+    std::vector<float2> host(kTwiddleBank, make_float2(0.f, 0.f));
+    auto fill = [&](int L) {
       for (int t = 0; t < L; ++t) {
         double a = -2.0 * M_PI * (double)t / (double)L;
-        host[L + t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        host[tw_offset(L) + t] = make_float2((float)std::cos(a), (float)std::sin(a));
       }
+    };
+    for (int L = 2; L <= kMaxFFT; L *= 2) fill(L);
+    for (int L = 12; L <= kMaxFFT; L *= 2) fill(L);
     float2* d = nullptr;
     hipError_t e = hipMalloc(&d, host.size() * sizeof(float2));
     if (e != hipSuccess) return fail(MM_ERR_OOM, std::string("twiddle bank: ") + hipGetErrorString(e));

@@ -71,16 +71,16 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   float2* dst = out + fo;
   float2* x = lds + c * Sh::S;
   const int h = nz / 2;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     const int sr = ADJ ? ((i >= h && i < h + nz) ? i - h : -1) : pad_src(i, nz, mirror);
     v[m] = (live && sr >= 0) ? src[(long long)sr * n] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
     v[m] = cmul(v[m], kk);
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   fft_reg<L, Sh::TPC, true>(v, x, t, tw);
   if (ADJ && mirror) {
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+    for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
     __syncthreads();
     if (!live) return;  // no barrier below
     for (int r = t; r < nz; r += Sh::TPC) {
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   } else {
     if (!live) return;
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) {
+    for (int m = 0; m < Sh::PT; ++m) {
       const int r = t + m * Sh::TPC - h;
       if (r >= 0 && r < nz) dst[(long long)r * n] = v[m];
     }
@@ -119,16 +119,16 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
   const float2* src = in + (valid ? row : 0) * n;
   float2* x = lds + c * Sh::S;
   const int h = n / 2;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     const int sx = ADJ ? ((i >= h && i < h + n) ? i - h : -1) : pad_src(i, n, mirror);
     v[m] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
     v[m] = cmul(v[m], kk);
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
   float2* dst = out + (valid ? row : 0) * n;
   if (ADJ && mirror) {
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+    for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
     __syncthreads();
     if (!valid) return;  // no barrier below
     for (int r = t; r < n; r += Sh::TPC) {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
   } else {
     if (!valid) return;
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) {
+    for (int m = 0; m < Sh::PT; ++m) {
       const int r = t + m * Sh::TPC - h;
       if (r >= 0 && r < n) dst[r] = v[m];
     }
@@ -180,7 +180,7 @@ template <int L, bool ADJ>
 static void launch_col_pass(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
   using Sh = ColShape<L>;
   hipLaunchKernelGGL((col_pass<L, ADJ>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
-                     dim3(Sh::NT), 0, st, in, out, p->n, p->nz, p->pad_mode == 0, p->Ky + (size_t)j * L, p->tw + L);
+                     dim3(Sh::NT), 0, st, in, out, p->n, p->nz, p->pad_mode == 0, p->Ky + (size_t)j * L, p->tw + tw_offset(L));
 }
 
 template <int L, bool ADJ>
@@ -188,24 +188,9 @@ static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, 
   using Sh = RowReg<L>;
   const long long rows = (long long)nframes * p->nz;
   hipLaunchKernelGGL((row_pass<L, ADJ>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, in, out,
-                     p->n, rows, p->pad_mode == 0, p->Kx + (size_t)j * L, p->tw + L);
+                     p->n, rows, p->pad_mode == 0, p->Kx + (size_t)j * L, p->tw + tw_offset(L));
 }
 
-#define MM_DISPATCH_F(L_RUNTIME, CALL)                     \
-  switch (L_RUNTIME) {                                     \
-    case 16: { constexpr int LL = 16; CALL; } break;       \
-    case 32: { constexpr int LL = 32; CALL; } break;       \
-    case 64: { constexpr int LL = 64; CALL; } break;       \
-    case 128: { constexpr int LL = 128; CALL; } break;     \
-    case 256: { constexpr int LL = 256; CALL; } break;     \
-    case 512: { constexpr int LL = 512; CALL; } break;     \
-    case 1024: { constexpr int LL = 1024; CALL; } break;   \
-    case 2048: { constexpr int LL = 2048; CALL; } break;   \
-    case 4096: { constexpr int LL = 4096; CALL; } break;   \
-    case 8192: { constexpr int LL = 8192; CALL; } break;   \
-    case 16384: { constexpr int LL = 16384; CALL; } break; \
-    default: return fail(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(L_RUNTIME) + " not built"); \
-  }
 
 static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, int j, void* stream, bool adj) {
   MM_ARG_CHECK(p != nullptr, "mm_fresnel: plan is NULL");
@@ -219,16 +204,16 @@ static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, 
   float2* dst = (float2*)out;
   const int Ly = 2 * p->nz, Lx = 2 * p->n;
   if (adj) {
-    MM_DISPATCH_F(Ly, (launch_col_pass<LL, true>(p, src, dst, nframes, j, st)));
+    MM_DISPATCH_FFT(Ly, (launch_col_pass<LL, true>(p, src, dst, nframes, j, st)));
   } else {
-    MM_DISPATCH_F(Ly, (launch_col_pass<LL, false>(p, src, dst, nframes, j, st)));
+    MM_DISPATCH_FFT(Ly, (launch_col_pass<LL, false>(p, src, dst, nframes, j, st)));
   }
   int rc = check_launch("fresnel col_pass");
   if (rc) return rc;
   if (adj) {
-    MM_DISPATCH_F(Lx, (launch_row_pass<LL, true>(p, dst, dst, nframes, j, st)));
+    MM_DISPATCH_FFT(Lx, (launch_row_pass<LL, true>(p, dst, dst, nframes, j, st)));
   } else {
-    MM_DISPATCH_F(Lx, (launch_row_pass<LL, false>(p, dst, dst, nframes, j, st)));
+    MM_DISPATCH_FFT(Lx, (launch_row_pass<LL, false>(p, dst, dst, nframes, j, st)));
   }
   return check_launch("fresnel row_pass");
 }
@@ -247,10 +232,11 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
                       const double* dist_host, int pad_mode) {
   MM_ARG_CHECK(plan != nullptr, "mm_fresnel_create: plan is NULL");
   *plan = nullptr;
-  MM_ARG_CHECK(is_pow2(n) && n >= 8 && 2 * n <= kMaxFFT,
-               "mm_fresnel_create: n must be a power of two in [8, 8192], got " + std::to_string(n));
-  MM_ARG_CHECK(is_pow2(nz) && nz >= 8 && 2 * nz <= kMaxFFT,
-               "mm_fresnel_create: nz must be a power of two in [8, 8192], got " + std::to_string(nz));
+  auto size_ok = [](int v) { return ((is_pow2(v) && v >= 8) || (is_fft_len(v) && v >= 24)) && 2 * v <= kMaxFFT; };
+  MM_ARG_CHECK(size_ok(n), "mm_fresnel_create: n must be 2^k in [8, 8192] or 3 * 2^k in [24, 6144], got " +
+                               std::to_string(n));
+  MM_ARG_CHECK(size_ok(nz), "mm_fresnel_create: nz must be 2^k in [8, 8192] or 3 * 2^k in [24, 6144], got " +
+                                std::to_string(nz));
   MM_ARG_CHECK(ndist >= 1 && dist_host != nullptr, "mm_fresnel_create: need ndist >= 1 distances");
   MM_ARG_CHECK(pad_mode == 0 || pad_mode == 1, "mm_fresnel_create: pad_mode must be 0 (symmetric) or 1 (zero)");
   MM_ARG_CHECK(voxelsize > 0 && wavelength >= 0, "mm_fresnel_create: bad wavelength / voxelsize");

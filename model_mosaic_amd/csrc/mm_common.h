@@ -46,9 +46,14 @@ __host__ __device__ __forceinline__ float2 mul_mi(float2 a) {
 }
 
 // ---- twiddle bank ----------------------------------------------------------
-// For every power of two L in [2, kMaxFFT] the bank holds exp(-2 pi i t / L),
-// t in [0, L), at element offset L (computed in FP64 on the host).
+// For every FFT length L = 2^k in [2, kMaxFFT] and L = 3 * 2^k in [12, kMaxFFT]
+// the bank holds exp(-2 pi i t / L), t in [0, L), at element offset
+// tw_offset(L) (computed in FP64 on the host).
 constexpr int kMaxFFT = 16384;
+constexpr long kTwiddleBank = 2L * kMaxFFT + 2L * (3 * kMaxFFT / 4);  // elements
+__host__ __device__ inline constexpr long tw_offset(int L) {
+  return (L & (L - 1)) == 0 ? (long)L : 2L * kMaxFFT + L;  // 3 * 2^k: [2 kMax + L, 2 kMax + 2 L)
+}
 // Returns the device pointer of the bank on the current device (built on first use).
 int twiddle_bank(const float2** out);
 
@@ -57,6 +62,8 @@ void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max);
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+// FFT lengths the kernels are built for: 2^k and 3 * 2^k.
+inline bool is_fft_len(long v) { return is_pow2(v) || (v % 3 == 0 && is_pow2(v / 3)); }
 
 // gfx950 dispatches workgroup w (linear id, x fastest) to XCD w mod 8, and each
 // XCD has its own L2.  xcd_remap turns w into a logical id such that every XCD

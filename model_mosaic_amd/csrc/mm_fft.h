@@ -1,47 +1,23 @@
-// LDS-resident Stockham FFT for gfx950.
+// Register-resident Stockham FFT for gfx950 (lengths 2^k and 3 * 2^k).
 //
-// A workgroup holds C = BATCH / L transforms of length L in LDS (transform c
-// occupies x[c*S .. c*S + L), the stride S padded so that the C transforms start
-// in different banks).  TPC = L / 16 threads work on one transform; every
-// thread owns 16 points per stage.  Stages are radix-8 (radix-4/2 for the leftover factor), each one a
-// read-all / barrier / twiddle + DFT_R in registers / write / barrier sweep
-// (Stockham autosort: natural order in and out, no bit reversal).  Twiddles come
-// from the FP64-built bank exp(-2 pi i t / L), so a length-L transform has the
-// accuracy of its radix arithmetic only.
+// Twiddles come from the FP64-built bank exp(-2 pi i t / L) (twiddle_bank,
+// offset tw_offset(L)), so a length-L transform has the accuracy of its radix
+// arithmetic only.
 #pragma once
 
 #include "mm_common.h"
 
 namespace mm {
 
-constexpr int kThreads = 1024;      // threads of a full-size FFT workgroup
 constexpr int kBatchElems = 16384;  // complex points per full-size FFT workgroup (128 KiB of LDS)
-constexpr int kPerThread = 16;      // points per thread per stage
-
-// Shape of a workgroup holding BATCH / L transforms of length L.
-template <int L, int BATCH = kBatchElems>
-struct FftShape {
-  static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
-  static_assert(BATCH >= L && BATCH % L == 0, "whole transforms per workgroup");
-  static constexpr int C = BATCH / L;                    // transforms per workgroup
-  static constexpr int TPC = L / kPerThread;             // threads per transform
-  static constexpr int NT = C * TPC;                     // threads per workgroup
-  // LDS stride between transforms: room for lpad(L + 1) plus a bank offset
-  static constexpr int S = L + L / 16 + (L >= 256 ? 16 : 2);
-  static constexpr int LDS = C * S;                      // float2 elements of LDS
-};
 
 // LDS index of element i of a transform: one pad slot per 16 complex values,
-// so the stride-R writes of the first Stockham stages hit distinct banks.
+// so strided Stockham writes hit distinct banks.
 __host__ __device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 4); }
 
 // Row-pass workgroups hold half as many points (64 KiB + pad of LDS), so two of
 // them share a CU and one's global loads overlap the other's LDS stages.
 constexpr int kRowBatch = 8192;
-template <int L>
-using RowShape = FftShape<L, (L > kRowBatch ? L : kRowBatch)>;
-
-constexpr int pick_radix(int rem) { return (rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2); }
 
 template <bool INV>
 __host__ __device__ __forceinline__ void dft2(float2& a, float2& b) {
@@ -80,6 +56,66 @@ __host__ __device__ __forceinline__ void dft8(float2* a) {
   a[7] = csub(e3, w3);
 }
 
+// 3-point DFT: y1,2 = a - (b + c)/2 -/+ i (sqrt 3 / 2)(b - c) (forward; INV conjugates).
+template <bool INV>
+__host__ __device__ __forceinline__ void dft3(float2& a, float2& b, float2& c) {
+  constexpr float H3 = 0.86602540378443864676f;
+  const float2 s = cadd(b, c), d = csub(b, c);
+  const float2 m = make_float2(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+  const float2 r = cscale(mul_mi<INV>(d), H3);
+  a = cadd(a, s);
+  b = cadd(m, r);
+  c = csub(m, r);
+}
+
+// W12^e times a (forward kernel exp(-2 pi i e / 12); INV conjugates).
+template <int E, bool INV>
+__host__ __device__ __forceinline__ float2 w12(float2 a) {
+  constexpr float H3 = 0.86602540378443864676f;
+  constexpr int e = E % 12;
+  float wr, wi;
+  if constexpr (e == 0) return a;
+  else if constexpr (e == 3) return mul_mi<INV>(a);
+  else if constexpr (e == 6) return make_float2(-a.x, -a.y);
+  else {
+    if constexpr (e == 1) { wr = H3; wi = -0.5f; }
+    else if constexpr (e == 2) { wr = 0.5f; wi = -H3; }
+    else { static_assert(e == 4, "exponents of the 3 x 4 split"); wr = -0.5f; wi = -H3; }
+    if (INV) wi = -wi;
+    return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+  }
+}
+
+// 12-point DFT as 3 x 4: n = n2 + 4 n1, k = k1 + 3 k2;
+// X[k1 + 3 k2] = sum_n2 W4^(n2 k2) W12^(n2 k1) sum_n1 W3^(n1 k1) x[n2 + 4 n1].
+template <bool INV>
+__host__ __device__ __forceinline__ void dft12(float2* a) {
+  float2 b[4][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 t0 = a[n2], t1 = a[n2 + 4], t2 = a[n2 + 8];
+    dft3<INV>(t0, t1, t2);
+    b[n2][0] = t0;
+    b[n2][1] = t1;
+    b[n2][2] = t2;
+  }
+  b[1][1] = w12<1, INV>(b[1][1]);
+  b[1][2] = w12<2, INV>(b[1][2]);
+  b[2][1] = w12<2, INV>(b[2][1]);
+  b[2][2] = w12<4, INV>(b[2][2]);
+  b[3][1] = w12<3, INV>(b[3][1]);
+  b[3][2] = w12<6, INV>(b[3][2]);
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    float2 u0 = b[0][k1], u1 = b[1][k1], u2 = b[2][k1], u3 = b[3][k1];
+    dft4<INV>(u0, u1, u2, u3);
+    a[k1] = u0;
+    a[k1 + 3] = u1;
+    a[k1 + 6] = u2;
+    a[k1 + 9] = u3;
+  }
+}
+
 template <int R, bool INV>
 __host__ __device__ __forceinline__ void dft(float2* v) {
   if constexpr (R == 2) {
@@ -87,99 +123,48 @@ __host__ __device__ __forceinline__ void dft(float2* v) {
   } else if constexpr (R == 4) {
     dft4<INV>(v[0], v[1], v[2], v[3]);
   } else {
+    static_assert(R == 8, "radix 2, 4 or 8");
     dft8<INV>(v);
   }
 }
 
-// One Stockham radix-R stage over a length-L transform at x (LDS), NS = product
-// of the radices already applied.  Thread t in [0, TPC) handles butterflies
-// j = t + b*TPC.
-template <int L, int R, int NS, int TPC, bool INV>
-__device__ __forceinline__ void fft_stage(float2* __restrict__ x, int t, const float2* __restrict__ tw) {
-  constexpr int NB = L / R;
-  constexpr int BPT = NB / TPC;
-  static_assert(BPT >= 1 && NB % TPC == 0, "butterflies per thread");
-  float2 v[BPT][R];
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int j = t + b * TPC;
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = x[lpad(j + r * NB)];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int j = t + b * TPC;
-    const int k = j % NS;
-    if constexpr (NS > 1) {
-      // W^(k r): three table reads, the other powers by products
-      constexpr int STEP = L / (NS * R);
-      float2 w1 = tw[k * STEP];
-      if (INV) w1.y = -w1.y;
-      if constexpr (R == 2) {
-        v[b][1] = cmul(v[b][1], w1);
-      } else {
-        float2 w2 = tw[2 * k * STEP];
-        if (INV) w2.y = -w2.y;
-        const float2 w3 = cmul(w1, w2);
-        v[b][1] = cmul(v[b][1], w1);
-        v[b][2] = cmul(v[b][2], w2);
-        v[b][3] = cmul(v[b][3], w3);
-        if constexpr (R == 8) {
-          float2 w4 = tw[4 * k * STEP];
-          if (INV) w4.y = -w4.y;
-          v[b][4] = cmul(v[b][4], w4);
-          v[b][5] = cmul(v[b][5], cmul(w1, w4));
-          v[b][6] = cmul(v[b][6], cmul(w2, w4));
-          v[b][7] = cmul(v[b][7], cmul(w3, w4));
-        }
-      }
-    }
-    dft<R, INV>(v[b]);
-    const int base = (j / NS) * NS * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) x[lpad(base + r * NS)] = v[b][r];
-  }
-  __syncthreads();
-}
-
-// Full in-place transform of x[lpad(0..L)) by the TPC threads with index t.
-// Must be reached by every thread of the workgroup (it contains barriers).
-// Caller synchronises before the first stage reads x.  Unnormalised in both
-// directions (INV only conjugates the kernel).
-template <int L, int TPC, bool INV, int NS = 1>
-__device__ __forceinline__ void fft_lds(float2* __restrict__ x, int t, const float2* __restrict__ tw) {
-  if constexpr (NS < L) {
-    constexpr int R = pick_radix(L / NS);
-    fft_stage<L, R, NS, TPC, INV>(x, t, tw);
-    fft_lds<L, TPC, INV, NS * R>(x, t, tw);
-  }
+// Lengths: 2^k (16 points per thread, radix-16 passes) and 3 * 2^k (12 points
+// per thread: one radix-12 pass, then radix-4 passes and a radix-2 for an odd
+// leftover).
+__host__ __device__ constexpr bool pow2c(int L) { return L > 0 && (L & (L - 1)) == 0; }
+__host__ __device__ constexpr bool fft_len_ok(int L) { return pow2c(L) || (L % 3 == 0 && pow2c(L / 3)); }
+__host__ __device__ constexpr int pt_of(int L) { return pow2c(L) ? 16 : 12; }
+__host__ __device__ constexpr int next_radix(int L, int NS) {
+  return pow2c(L) ? (L / NS >= 16 ? 16 : L / NS) : (NS == 1 ? 12 : ((L / NS) % 4 == 0 ? 4 : 2));
 }
 
 // ---- register-resident Stockham FFT -----------------------------------------
-// Thread t (0 <= t < TPC = L / 16) of a transform holds v[m] = x[t + m TPC],
-// m < 16, before and after the transform.  A radix-R pass runs the thread's
-// 16 / R butterflies j = t + b TPC on its own registers (butterfly j reads
-// x[j + s L / R] = v[b + s 16 / R]); its outputs go to LDS at their Stockham
+// Thread t (0 <= t < TPC = L / PT) of a transform holds v[m] = x[t + m TPC],
+// m < PT (PT = pt_of(L): 16, or 12 for 3 * 2^k), before and after the
+// transform.  A radix-R pass (R divides PT) runs the thread's PT / R
+// butterflies j = t + b TPC on its own registers (butterfly j reads
+// x[j + s L / R] = v[b + s PT / R]); its outputs go to LDS at their Stockham
 // positions and come back in natural order for the next pass.  The last
 // pass's Stockham positions are the natural ones, so its outputs stay in
-// registers: a length 16^a r transform (r in {1, 2, 4, 8}) costs a + [r > 1] - 1
-// LDS exchanges, against one LDS round trip per radix-8 stage plus the
-// global <-> LDS staging of fft_lds.  Global loads and stores of v are
-// therefore direct (element t + m TPC: consecutive t, consecutive addresses).
+// registers: a length 16^a r transform (r in {1, 2, 4, 8}) costs
+// a + [r > 1] - 1 LDS exchanges.  Global loads and stores of v are direct
+// (element t + m TPC: consecutive t, consecutive addresses).
 
-// Workgroup shape: C transforms of length L, stride S between them in LDS
-// (room for XTRA more values per transform, for kernels that stage more than L
-// inputs there).  S = 16 (mod 32) complex values, with the interleaved thread
-// mapping of map(), keeps the Stockham writes and natural reads free of bank
-// conflicts (two transforms per 32-lane half-wave at a 16-bank-pair offset).
-// The plain mapping (64 consecutive t of one transform per wave) reads rows
-// in longer contiguous runs; it leaves some 2-way conflicts on 64-bank LDS.
+// Workgroup shape: C transforms of length L (at most 1024 threads), stride S
+// between them in LDS (room for XTRA more values per transform, for kernels
+// that stage more than L inputs there).  S = 16 (mod 32) complex values, with
+// the interleaved thread mapping of map(), keeps the Stockham writes and
+// natural reads of power-of-two lengths free of bank conflicts (two
+// transforms per 32-lane half-wave at a 16-bank-pair offset).  The plain
+// mapping (64 consecutive t of one transform per wave) reads rows in longer
+// contiguous runs; it leaves some 2-way conflicts on 64-bank LDS.
 template <int L, int BATCH, int XTRA = 0, bool INTERLEAVE = true>
 struct RegShape {
-  static_assert(L >= 16 && L <= kMaxFFT && (L & (L - 1)) == 0, "power-of-two length in [16, 16384]");
-  static constexpr int C = BATCH > L ? BATCH / L : 1;
-  static constexpr int TPC = L / kPerThread;
+  static_assert(L >= 16 && L <= kMaxFFT && fft_len_ok(L), "length 2^k or 3 * 2^k in [16, 16384]");
+  static constexpr int PT = pt_of(L);
+  static constexpr int TPC = L / PT;
+  static constexpr int C0 = BATCH > L ? BATCH / L : 1;
+  static constexpr int C = C0 * TPC > 1024 ? 1024 / TPC : C0;
   static constexpr int NT = C * TPC;
   static constexpr int S0 = L + L / 16 + XTRA;  // > lpad(L - 1) + XTRA
   static constexpr int S = TPC < 16 ? S0 : S0 + ((16 - S0 % 32) + 32) % 32;  // S = 16 (mod 32)
@@ -266,40 +251,32 @@ __device__ __forceinline__ void twiddle(float2* u, int k, const float2* __restri
     if (INV) w.y = -w.y;
     return w;
   };
-  const float2 w1 = ld(1);
-  if constexpr (R == 2) {
-    u[1] = cmul(u[1], w1);
-  } else {
-    const float2 w2 = ld(2), w3 = cmul(w1, w2);
-    u[1] = cmul(u[1], w1);
-    u[2] = cmul(u[2], w2);
-    u[3] = cmul(u[3], w3);
-    if constexpr (R >= 8) {
-      const float2 w4 = ld(4), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
-      u[4] = cmul(u[4], w4);
-      u[5] = cmul(u[5], w5);
-      u[6] = cmul(u[6], w6);
-      u[7] = cmul(u[7], w7);
-      if constexpr (R == 16) {
-        const float2 w8 = ld(8);
-        u[8] = cmul(u[8], w8);
-        u[9] = cmul(u[9], cmul(w1, w8));
-        u[10] = cmul(u[10], cmul(w2, w8));
-        u[11] = cmul(u[11], cmul(w3, w8));
-        u[12] = cmul(u[12], cmul(w4, w8));
-        u[13] = cmul(u[13], cmul(w5, w8));
-        u[14] = cmul(u[14], cmul(w6, w8));
-        u[15] = cmul(u[15], cmul(w7, w8));
-      }
-    }
+  float2 w[16];
+  w[1] = ld(1);
+  if constexpr (R > 2) {
+    w[2] = ld(2);
+    w[3] = cmul(w[1], w[2]);
   }
+  if constexpr (R > 4) {
+    w[4] = ld(4);
+#pragma unroll
+    for (int e = 5; e < 8; ++e) w[e] = cmul(w[e - 4], w[4]);
+  }
+  if constexpr (R > 8) {
+    w[8] = ld(8);
+#pragma unroll
+    for (int e = 9; e < 16; ++e) w[e] = cmul(w[e - 8], w[8]);
+  }
+#pragma unroll
+  for (int e = 1; e < R; ++e) u[e] = cmul(u[e], w[e]);
 }
 
 // One radix-R pass of fft_reg (NS = product of the radices already applied).
-template <int L, int R, int NS, int TPC, bool INV>
-__device__ __forceinline__ void reg_pass(float2 (&v)[kPerThread], float2* __restrict__ x, int t,
+template <int L, int R, int NS, int TPC, bool INV, int PT>
+__device__ __forceinline__ void reg_pass(float2 (&v)[PT], float2* __restrict__ x, int t,
                                          const float2* __restrict__ tw) {
-  constexpr int BPT = kPerThread / R;
+  static_assert(PT % R == 0, "the radix divides the points per thread");
+  constexpr int BPT = PT / R;
   constexpr bool LAST = NS * R == L;
   float2 u[BPT][R];
 #pragma unroll
@@ -307,9 +284,11 @@ __device__ __forceinline__ void reg_pass(float2 (&v)[kPerThread], float2* __rest
     const int j = t + b * TPC;
 #pragma unroll
     for (int s = 0; s < R; ++s) u[b][s] = v[b + s * BPT];
-    if constexpr (NS > 1) twiddle<L, R, NS, INV>(u[b], j & (NS - 1), tw);
+    if constexpr (NS > 1) twiddle<L, R, NS, INV>(u[b], j % NS, tw);
     if constexpr (R == 16)
       dft16<INV>(u[b]);
+    else if constexpr (R == 12)
+      dft12<INV>(u[b]);
     else
       dft<R, INV>(u[b]);
   }
@@ -322,29 +301,58 @@ __device__ __forceinline__ void reg_pass(float2 (&v)[kPerThread], float2* __rest
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       const int j = t + b * TPC;
-      const int base = (j / NS) * NS * R + (j & (NS - 1));
+      const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
       for (int q = 0; q < R; ++q) x[lpad(base + q * NS)] = u[b][q];
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) v[m] = x[lpad(t + m * TPC)];
+    for (int m = 0; m < PT; ++m) v[m] = x[lpad(t + m * TPC)];
     __syncthreads();
   }
 }
 
-// Transform of the 16 points each of the TPC threads of one transform holds in
-// v (natural order in and out), x = that transform's LDS (RegShape stride).
-// Contains barriers when L > 16: every thread of the workgroup must call it.
+// Transform of the PT points each of the TPC threads of one transform holds
+// in v (natural order in and out), x = that transform's LDS (RegShape stride).
+// Contains barriers when L > PT: every thread of the workgroup must call it.
 // Unnormalised in both directions (INV only conjugates the kernel).
-template <int L, int TPC, bool INV, int NS = 1>
-__device__ __forceinline__ void fft_reg(float2 (&v)[kPerThread], float2* __restrict__ x, int t,
+template <int L, int TPC, bool INV, int NS = 1, int PT>
+__device__ __forceinline__ void fft_reg(float2 (&v)[PT], float2* __restrict__ x, int t,
                                         const float2* __restrict__ tw) {
+  static_assert(PT == pt_of(L) && TPC * PT == L, "register FFT shape");
   if constexpr (NS < L) {
-    constexpr int R = L / NS >= 16 ? 16 : L / NS;
-    reg_pass<L, R, NS, TPC, INV>(v, x, t, tw);
+    constexpr int R = next_radix(L, NS);
+    reg_pass<L, R, NS, TPC, INV, PT>(v, x, t, tw);
     fft_reg<L, TPC, INV, NS * R>(v, x, t, tw);
   }
 }
 
 }  // namespace mm
+
+// Runs CALL with constexpr int LL = L_RUNTIME for every FFT length the kernels
+// are built for (2^k in [16, 16384], 3 * 2^k in [48, 12288]); any other length
+// returns MM_ERR_UNSUPPORTED from the enclosing function.
+#define MM_DISPATCH_FFT(L_RUNTIME, CALL)                                                             \
+  switch (L_RUNTIME) {                                                                               \
+    case 16: { constexpr int LL = 16; CALL; } break;                                                 \
+    case 32: { constexpr int LL = 32; CALL; } break;                                                 \
+    case 64: { constexpr int LL = 64; CALL; } break;                                                 \
+    case 128: { constexpr int LL = 128; CALL; } break;                                               \
+    case 256: { constexpr int LL = 256; CALL; } break;                                               \
+    case 512: { constexpr int LL = 512; CALL; } break;                                               \
+    case 1024: { constexpr int LL = 1024; CALL; } break;                                             \
+    case 2048: { constexpr int LL = 2048; CALL; } break;                                             \
+    case 4096: { constexpr int LL = 4096; CALL; } break;                                             \
+    case 8192: { constexpr int LL = 8192; CALL; } break;                                             \
+    case 16384: { constexpr int LL = 16384; CALL; } break;                                           \
+    case 48: { constexpr int LL = 48; CALL; } break;                                                 \
+    case 96: { constexpr int LL = 96; CALL; } break;                                                 \
+    case 192: { constexpr int LL = 192; CALL; } break;                                               \
+    case 384: { constexpr int LL = 384; CALL; } break;                                               \
+    case 768: { constexpr int LL = 768; CALL; } break;                                               \
+    case 1536: { constexpr int LL = 1536; CALL; } break;                                             \
+    case 3072: { constexpr int LL = 3072; CALL; } break;                                             \
+    case 6144: { constexpr int LL = 6144; CALL; } break;                                             \
+    case 12288: { constexpr int LL = 12288; CALL; } break;                                           \
+    default: return ::mm::fail(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(L_RUNTIME) + " not built"); \
+  }

@@ -126,9 +126,9 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   const float* ra = obj + ((long long)z * n + y0) * n;
   const float* rb = ra + n;
   const float pya = valid ? phy[y0] : 0.f, pyb = valid ? phy[y0 + 1] : 0.f;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int xi = t + m * Sh::TPC - half;
     v[m] = make_float2(0.f, 0.f);
     if (valid && xi >= 0 && xi < n) {
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+  for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
   // unpack A = (Z_k + conj Z_{L-k})/2 (row y0), B = (Z_k - conj Z_{L-k})/(2i) (row y0+1);
   // thread order: column within block, then the 2C rows of this workgroup, then block
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
     float2* rp = rowp[r];
     if (col > n || rp == nullptr) continue;
     const float2* xc = lds + (r >> 1) * Sh::S;
-    const float2 zk = xc[lpad(col)], zm = xc[lpad((L - col) & (L - 1))];
+    const float2 zk = xc[lpad(col)], zm = xc[lpad(col == 0 ? 0 : L - col)];
     const float2 w = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
                              : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
     rp[(long long)blk * 4 * L + q] = w;  // hs_idx(row, col, L) = 4 row + 4 L (col >> 2) + (col & 3)
@@ -178,15 +178,15 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_fft(float2* __restrict__ 
   const int col = blockIdx.x * Sh::C + c;
   const bool live = col < ncols;
   float2* b = buf + (long long)blockIdx.y * zs;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     v[m] = (live && i >= in_lo && i < in_hi) ? b[hs_idx(i, col, L)] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, INV>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     if (live && i >= out_lo && i < out_hi) b[hs_idx(i, col, L)] = cscale(v[m], scale);
   }
@@ -230,6 +230,42 @@ __device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* 
 //    is one 16-byte read for both slices (9 per row, half the barriers per
 //    slice); lanes of a quarter-wave have distinct base mod 16.
 // Either way the 16 pieces of one read hit 16 distinct bank groups.
+// Staging descriptor idx of a gather item: the half-spectrum offset (within a
+// z plane) of patch element idx and its LDS slot << 1 | conj flag; slots
+// outside the patch read offset 0 into the sink slot PATCH_ELEMS.  Patch
+// columns and rows are raw fftfreq indices taken mod 2n (tiles near the top
+// and right edges reach past 2n for small n); columns k0 > n are the
+// conjugate of the Hermitian mirror (2n - k1, 2n - k0).
+__host__ __device__ __forceinline__ int wrap_mod(int v, int m) {
+  const int r = v % m;
+  return r < 0 ? r + m : r;
+}
+__host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int idx, int n, int& soff, int& sdst) {
+  const int pw = it.w + 2 * M, ph = it.h + 2 * M;
+  const int pp = patch_pitch(it.w);
+  const int px0 = it.x0 - M, py0 = it.y0 - M;
+  const int bx0 = px0 & ~3;  // first 4-column block touched by the patch
+  const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
+  const int twon = 2 * n;
+  soff = 0;
+  sdst = PATCH_ELEMS << 1;
+  if (idx >= nstage) return;
+  const int q = idx & 3, rest = idx >> 2;
+  const int pr = rest % ph, blk = rest / ph;
+  const int l0 = bx0 + 4 * blk + q;
+  const int pc = l0 - px0;
+  if (pc < 0 || pc >= pw) return;
+  const int k0 = wrap_mod(l0, twon), k1 = wrap_mod(py0 + pr, twon);
+  int fl = 0;
+  if (k0 <= n) {
+    soff = (int)hs_idx(k1, k0, twon);
+  } else {
+    soff = (int)hs_idx(k1 == 0 ? 0 : twon - k1, twon - k0, twon);
+    fl = 1;
+  }
+  sdst = ((pr * pp + pc) << 1) | fl;
+}
+
 template <int ZP>
 __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(ZP == 1 ? 8 : 4, ZP == 1 ? 8 : 4)))
 void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const Item* __restrict__ items,
@@ -242,40 +278,14 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
   const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
   const Item it = items[lw % gridDim.x];
   const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
-  const int pw = it.w + 2 * M, ph = it.h + 2 * M;
-  const int pp = patch_pitch(it.w);  // LDS row pitch >= pw
-  const int px0 = it.x0 - M, py0 = it.y0 - M;
-  const int bx0 = px0 & ~3;  // first 4-column block touched by the patch
-  const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
+  const int pp = patch_pitch(it.w);  // LDS row pitch >= w + 2M
   const int s = threadIdx.x;
   const bool active = s < it.count;
-  const int mask = 2 * n - 1;
   // staging descriptors: source offset in the z plane; LDS slot << 1 | conj.
   // Idle slots read offset 0 and write the sink, so the loads issue together.
   int soff[STG_MAX], sdst[STG_MAX];
 #pragma unroll
-  for (int e = 0; e < STG_MAX; ++e) {
-    const int idx = threadIdx.x + e * TILE_THREADS;
-    sdst[e] = PATCH_ELEMS << 1;
-    soff[e] = 0;
-    if (idx < nstage) {
-      const int q = idx & 3, rest = idx >> 2;
-      const int pr = rest % ph, blk = rest / ph;
-      const int l0 = bx0 + 4 * blk + q;
-      const int pc = l0 - px0;
-      if (pc >= 0 && pc < pw) {
-        const int k0 = l0 & mask, k1 = (py0 + pr) & mask;
-        int fl = 0;
-        if (k0 <= n) {
-          soff[e] = (int)hs_idx(k1, k0, 2 * n);
-        } else {
-          soff[e] = (int)hs_idx((2 * n - k1) & mask, 2 * n - k0, 2 * n);
-          fl = 1;
-        }
-        sdst[e] = ((pr * pp + pc) << 1) | fl;
-      }
-    }
-  }
+  for (int e = 0; e < STG_MAX; ++e) gather_stage_desc(it, threadIdx.x + e * TILE_THREADS, n, soff[e], sdst[e]);
   // ZP = 1: ex[j] weighs patch column (base & ~1) + j, the 9 taps shifted by
   // the parity of base (ex[9] = 0 or ex[0] = 0).  ZP = 2: ex[j] weighs base + j.
   constexpr int NX = ZP == 1 ? TAPS + 1 : TAPS;
@@ -415,9 +425,9 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r4_line_ifft(const float2* __re
   const int sg = valid ? sig[row / nz] : 1;
   const float2* src = gh + (valid ? row : 0) * H;
   constexpr int half = L / 2;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int r = t + m * Sh::TPC;
     const int d = r - half;
     int kk;
@@ -438,7 +448,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r4_line_ifft(const float2* __re
   if (valid) {
     float* dst = sino + row * L;
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) {
+    for (int m = 0; m < Sh::PT; ++m) {
       const int s = t + m * Sh::TPC;
       const float w = v[m].x * scale;
       dst[s] = (s & 1) ? w : -w;
@@ -466,9 +476,9 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
   const int sg = valid ? sig[rr / nz] : 1;
   float2* x = lds + c * Sh::S;
   constexpr int half = L / 2;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int r = t + m * Sh::TPC;
     float2 w;
     if (CPLX)
@@ -483,7 +493,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
     if (!valid) return;
     float2* dst = yre + rr * H;
 #pragma unroll
-    for (int m = 0; m < kPerThread; ++m) {
+    for (int m = 0; m < Sh::PT; ++m) {
       const int r = t + m * Sh::TPC;
       const float2 y = (r & 1) ? v[m] : make_float2(-v[m].x, -v[m].y);  // yhat(r)
       if (sg > 0) {
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
     return;
   }
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+  for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
   if (!valid) return;  // no barrier below
   auto yhat = [&](int r) {
@@ -522,7 +532,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
       }
     } else {
       const int r = sg > 0 ? half + k : half - k;
-      const float2 p = yhat(r), q = yhat((L - r) & (L - 1));
+      const float2 p = yhat(r), q = yhat(r == 0 ? 0 : L - r);
       vr = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
       vi = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
     }
@@ -626,7 +636,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   for (int jp = 0; jp < SL_PAIRS_B; ++jp) lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0u;
   __syncthreads();
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
-  const int mask = 2 * n - 1;
+  const int l1w = l1 < 0 ? l1 + 2 * n : l1;  // raw fftfreq row
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
     for (int e = p; e < it.ne; e += SNP) {
@@ -684,7 +694,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       for (int zz = 0; zz < nzz; ++zz) {
         const float2 val = make_float2(sgn * (float)ax[zz], sgn * (float)ay[zz]);
         if (it.slot < 0)
-          Hb[(long long)(zc + zz) * hzs + hs_idx(l1 & mask, l0, 2 * n)] = val;
+          Hb[(long long)(zc + zz) * hzs + hs_idx(l1w, l0, 2 * n)] = val;
         else
           partial[((long long)it.slot * nz + zc + zz) * SNP + pt] = val;
       }
@@ -708,7 +718,7 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
     acc.x += v.x;
     acc.y += v.y;
   }
-  Hb[(long long)z * hzs + hs_idx(l1 & (2 * n - 1), l0, 2 * n)] = acc;
+  Hb[(long long)z * hzs + hs_idx(l1 < 0 ? l1 + 2 * n : l1, l0, 2 * n)] = acc;
 }
 
 // Row pass of RT: C2R inverse transform of the half-spectrum rows n/2 + y
@@ -775,9 +785,9 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
   int c, t;
   Sh::map(threadIdx.x, c, t);
   float2* x = lds + c * Sh::S;
-  float2 v[kPerThread];
+  float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int kx = t + m * Sh::TPC;
     float2 a, b;
     if (kx <= n) {
@@ -806,7 +816,7 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
     y = 2 * (int)(unit - (long long)z * half);
   }
 #pragma unroll
-  for (int m = 0; m < kPerThread; ++m) {
+  for (int m = 0; m < Sh::PT; ++m) {
     const int xx = t + m * Sh::TPC - half;
     if (xx < 0 || xx >= n) continue;
     const float2 w = cscale(v[m], scale);
@@ -830,7 +840,10 @@ struct Rect {
   int c0, c1, r0, r1;  // inclusive, centred coordinates inside the half plane domain
 };
 
-static inline int wrapc(int v, int n) { return ((v + n) & (2 * n - 1)) - n; }
+static inline int wrapc(int v, int n) {  // into [-n, n)
+  const int r = (v + n) % (2 * n);
+  return (r < 0 ? r + 2 * n : r) - n;
+}
 
 // Splits rows [r0, r1] (centred, may leave [-n, n)) into wrapped pieces.
 static void push_wrapped(std::vector<Rect>& out, int c0, int c1, int r0, int r1, int n) {
@@ -1329,7 +1342,7 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
   using Sh = RowReg<L>;
   const long long units = (long long)nz * (p->n / 2);
   hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, obj,
-                     p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + L);
+                     p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + tw_offset(L));
 }
 
 template <int L, bool INV>
@@ -1338,7 +1351,7 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in
   using Sh = ColShape<L>;
   const int ncols = p->n + 1;
   hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                     0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + L);
+                     0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + tw_offset(L));
 }
 
 template <int L>
@@ -1346,7 +1359,7 @@ static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
   using Sh = RowReg<L>;
   const long long rows = (long long)p->ntheta * nz;
   hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
-                     sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + L);
+                     sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + tw_offset(L));
 }
 
 template <int L>
@@ -1356,10 +1369,10 @@ static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, h
   const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
   if (cplx)
     hipLaunchKernelGGL((rt1_line_fft<L, true>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H, rows,
-                       nz, p->C0s, p->tw + L);
+                       nz, p->C0s, p->tw + tw_offset(L));
   else
     hipLaunchKernelGGL((rt1_line_fft<L, false>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H,
-                       rows, nz, p->C0s, p->tw + L);
+                       rows, nz, p->C0s, p->tw + tw_offset(L));
 }
 
 template <int L>
@@ -1370,27 +1383,12 @@ static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStrea
   const float scale = 1.0f / (2.0f * p->n);
   if (cplx)
     hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(Sh::NT), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
-                       p->n, nz, p->zs, scale, p->tw + L);
+                       p->n, nz, p->zs, scale, p->tw + tw_offset(L));
   else
     hipLaunchKernelGGL((rt4_rows_c2r<L, false>), grid, dim3(Sh::NT), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
-                       p->n, nz, p->zs, scale, p->tw + L);
+                       p->n, nz, p->zs, scale, p->tw + tw_offset(L));
 }
 
-#define MM_DISPATCH_L(L_RUNTIME, CALL)              \
-  switch (L_RUNTIME) {                              \
-    case 16: { constexpr int LL = 16; CALL; } break;     \
-    case 32: { constexpr int LL = 32; CALL; } break;     \
-    case 64: { constexpr int LL = 64; CALL; } break;     \
-    case 128: { constexpr int LL = 128; CALL; } break;   \
-    case 256: { constexpr int LL = 256; CALL; } break;   \
-    case 512: { constexpr int LL = 512; CALL; } break;   \
-    case 1024: { constexpr int LL = 1024; CALL; } break; \
-    case 2048: { constexpr int LL = 2048; CALL; } break; \
-    case 4096: { constexpr int LL = 4096; CALL; } break; \
-    case 8192: { constexpr int LL = 8192; CALL; } break; \
-    case 16384: { constexpr int LL = 16384; CALL; } break; \
-    default: return fail(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(L_RUNTIME) + " not built"); \
-  }
 
 static int zsplit_for(int items, int nz) {
   if (items <= 0) return 1;
@@ -1450,8 +1448,8 @@ extern "C" {
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags) {
   MM_ARG_CHECK(plan != nullptr, "mm_radon_create: plan is NULL");
   *plan = nullptr;
-  MM_ARG_CHECK(is_pow2(n) && n >= 16 && 2 * n <= kMaxFFT,
-               "mm_radon_create: n must be a power of two in [16, 8192], got " + std::to_string(n));
+  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && 2 * n <= kMaxFFT,
+               "mm_radon_create: n must be 2^k in [16, 8192] or 3 * 2^k in [96, 6144], got " + std::to_string(n));
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
@@ -1546,12 +1544,12 @@ int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* strea
   const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 0, 0, st))) return rc;
-  MM_DISPATCH_L(L2, (launch_r1<LL>(p, obj, nz, st)));
+  MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
   if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
-  MM_DISPATCH_L(L2, (launch_col<LL, false>(p, p->fde, nz, p->n / 2, 3 * p->n / 2, 0, 2 * p->n, 1.0f, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, p->n / 2, 3 * p->n / 2, 0, 2 * p->n, 1.0f, st)));
   if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
   if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
-  MM_DISPATCH_L(L1, (launch_r4<LL>(p, sino, nz, st)));
+  MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
   if ((rc = check_launch("r4_line_ifft"))) return rc;
   return mark(p, 3, 1, st);
 }
@@ -1576,20 +1574,20 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
-  MM_DISPATCH_L(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
+  MM_DISPATCH_FFT(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
   if ((rc = check_launch("rt1_line_fft")) || (rc = mark(p, 4, 1, st)) || (rc = mark(p, 5, 0, st))) return rc;
   if ((rc = launch_scatter(p, p->gh, p->fde, nz, st))) return rc;
   if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
   const float s = 1.0f / (2.0f * p->n);
-  MM_DISPATCH_L(L2, (launch_col<LL, true>(p, p->fde, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
   if ((rc = check_launch("col_fft (RT)"))) return rc;
   if (cplx) {
-    MM_DISPATCH_L(L2, (launch_col<LL, true>(p, p->fde2, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
     if ((rc = check_launch("col_fft (RT im)"))) return rc;
   }
   if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
-  MM_DISPATCH_L(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
+  MM_DISPATCH_FFT(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
   if ((rc = check_launch("rt4_rows_c2r"))) return rc;
   return mark(p, 7, 1, st);
 }
@@ -1621,6 +1619,76 @@ int mm_radon_stage_ms(mm_radon* p, float* ms, int max_stages) {
     if (!p->ran[s]) continue;
     MM_HIP_TRY(hipEventSynchronize(p->ev[s][1]));
     MM_HIP_TRY(hipEventElapsedTime(&ms[s], p->ev[s][0], p->ev[s][1]));
+  }
+  return MM_OK;
+}
+
+// Host-only check of the plan tables against the index arithmetic of the
+// gather and scatter kernels (every staging offset inside a half-spectrum
+// plane, every patch element staged once, every tap read inside the patch,
+// every list slot naming a staged entry row).  Builds the tables like
+// mm_radon_create, touches no device.
+int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
+  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && 2 * n <= kMaxFFT,
+               "mm_radon_plan_check: unsupported n " + std::to_string(n));
+  MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_plan_check: need ntheta >= 1 angles");
+  HostGeom G;
+  G.n = n;
+  G.H = n / 2 + 1;
+  G.ntheta = ntheta;
+  G.dx.resize(ntheta);
+  G.dy.resize(ntheta);
+  for (int t = 0; t < ntheta; ++t) {
+    const double c = std::cos(theta_host[t]), sn = std::sin(theta_host[t]);
+    G.dx[t] = std::fabs(c);
+    G.dy[t] = -(c >= 0 ? 1 : -1) * sn;
+  }
+  const long long zs = (long long)(n / 4 + 1) * 2 * n * 4;
+  auto bad = [&](const std::string& what) { return fail(MM_ERR_ARG, "mm_radon_plan_check n=" + std::to_string(n) + ": " + what); };
+  for (int zp = 1; zp <= 2; ++zp) {
+    Binning B;
+    B.zp = zp;
+    build_gather(G, std::min(ROOT, n), B);
+    for (const Item& it : B.items) {
+      const int pw = it.w + 2 * M, ph = it.h + 2 * M, pp = patch_pitch(it.w);
+      if (pp < pw || ph * pp > PATCH_ELEMS) return bad("patch pitch");
+      std::vector<int> cover(PATCH_ELEMS + 1, 0);
+      for (int idx = 0; idx < STG_MAX * TILE_THREADS; ++idx) {
+        int so, sd;
+        gather_stage_desc(it, idx, n, so, sd);
+        if (so < 0 || so >= zs || (sd >> 1) > PATCH_ELEMS)
+          return bad("staging offset " + std::to_string(so) + " / slot " + std::to_string(sd >> 1) + " out of range");
+        ++cover[sd >> 1];
+      }
+      for (int r = 0; r < ph; ++r)
+        for (int c = 0; c < pw; ++c)
+          if (cover[r * pp + c] != 1) return bad("patch element staged " + std::to_string(cover[r * pp + c]) + " times");
+      for (int k = 0; k < it.count; ++k) {
+        int rx, ry;
+        G.centre(B.samples[it.start + k], rx, ry);
+        if (rx < it.x0 || rx >= it.x0 + it.w || ry < it.y0 || ry >= it.y0 + it.h) return bad("sample outside its tile");
+        const int b = (ry - it.y0) * pp + (rx - it.x0);
+        const int last = zp == 1 ? (b & ~1) + 8 * pp + 9 : b + 8 * pp + 8;
+        if (last >= PATCH_ELEMS + (zp == 1 ? 2 : 0)) return bad("tap read past the patch");
+        if (B.order[it.start + k] >= it.count) return bad("output order");
+      }
+    }
+  }
+  SBinning S;
+  build_scatter_lists(G, S);
+  for (const SItem& it : S.items) {
+    if (it.ne > SE_MAX) return bad("scatter entries per item");
+    if (it.loff + (long long)it.nl / 2 * 2 * SNP > (long long)S.lists.size()) return bad("list offset");
+    for (int w = 0; w < SWAVES; ++w) {
+      const int np = (int)((it.wnp >> (8 * w)) & 0xFFu);
+      if (2 * np > it.nl || 2 * np > SL_MAX) return bad("wave schedule length");
+      for (int j = 0; j < 2 * np; ++j)
+        for (int l = 64 * w; l < 64 * w + 64; ++l) {
+          const unsigned u = S.lists[it.loff + (long long)(j / 2) * 2 * SNP + 2 * l + (j & 1)];
+          if ((int)(u & 0x1FFu) >= std::max(it.ne, 1) || ((u >> 9) >= (unsigned)(TAPS * TAPS) && (u >> 9) != SL_IDLE_TAP))
+            return bad("list slot");
+        }
+    }
   }
   return MM_OK;
 }

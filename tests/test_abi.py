@@ -56,10 +56,12 @@ def test_null_plan_is_an_argument_error(lib):
 def test_bad_sizes_rejected_before_device_work(lib):
     p = C.c_void_p()
     th = (C.c_double * 4)(0, 0.1, 0.2, 0.3)
-    rc = lib.mm_radon_create(C.byref(p), 100, 1, 4, th, 0)    # not a power of two
-    assert rc == _lib.MM_ERR_ARG and not p
-    rc = lib.mm_fresnel_create(C.byref(p), 64, 48, 1, 1e-10, 1e-6, (C.c_double * 1)(0.1), 0)
-    assert rc == _lib.MM_ERR_ARG and not p
+    for n in (100, 48, 160, 12288):   # not 2^k / 3 * 2^k, 3 * 2^k below 96, 5 * 2^k, 2n above the largest FFT
+        rc = lib.mm_radon_create(C.byref(p), n, 1, 4, th, 0)
+        assert rc == _lib.MM_ERR_ARG and not p, n
+    for n, nz in ((64, 40), (20, 64), (64, 12)):   # 5 * 2^3, 5 * 2^2, 3 * 2^2 below 24
+        rc = lib.mm_fresnel_create(C.byref(p), n, nz, 1, 1e-10, 1e-6, (C.c_double * 1)(0.1), 0)
+        assert rc == _lib.MM_ERR_ARG and not p, (n, nz)
     rc = lib.mm_psi_from_proj(None, None, -1, 1.0, 1.0, None)
     assert rc == _lib.MM_ERR_ARG
 

@@ -76,6 +76,22 @@ def test_sizes_vs_fp64(tomo_cls, n, nt, span, nz, zp):
     assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
 
 
+@pytest.mark.parametrize("n,nt,nz", [(96, 37, 3), (192, 50, 2), (768, 129, 5), (1536, 64, 2)])
+def test_sizes_3x2k_vs_fp64(tomo_cls, n, nt, nz):
+    """Detector widths 3 * 2^k (the mosaic tiles' 1536 and 3072): radix-12 FFT passes, non-power-of-two
+    Fourier grid wrap-around in the gather and scatter."""
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(n + nt)
+    x = rng.standard_normal((nz, n, n)).astype(np.float32)
+    y = rng.standard_normal((nt, nz, n)).astype(np.float32)
+    t = tomo_cls(n, nz, th)
+    r = t.R(torch.from_numpy(x).cuda()).cpu().numpy()
+    b = t.RT(torch.from_numpy(y).cuda()).cpu().numpy()
+    er, eb = rel(r, usfft.tomo_R(x, th, np.float64)), rel(b, usfft.tomo_RT(y, th, dtype=np.float64))
+    print(f"n={n}: R {er:.2e}  RT {eb:.2e}")
+    assert er < 1e-5 and eb < 1e-5
+
+
 def test_random_angles_and_partial_nz(tomo_cls):
     n, nt = 64, 29
     th = np.sort(np.random.default_rng(5).uniform(0, 2 * np.pi, nt)).astype(np.float32)

@@ -48,3 +48,20 @@ def test_phantom_seeded_and_bounded():
     assert float(a.max()) <= 12 * 1e-6 and float(a.min()) >= 0 and float(a.max()) > 0
     th = phantom.angles(384)
     assert th.dtype == np.float32 and th[0] == 0 and th[-1] < np.pi
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 96, 128, 192, 256, 384, 768, 1536])
+def test_radon_plan_tables_match_kernel_indexing(n):
+    """Host-only: every gather staging offset lies inside a half-spectrum plane (tiles at the right
+    edge of small grids reach past 2n and must wrap), every patch element is staged once, every tap
+    read stays inside the LDS patch, every scatter slot names a staged entry."""
+    import ctypes as C
+
+    import numpy as np
+
+    from model_mosaic_amd import _lib
+    lib = _lib.load()
+    for nt, span in ((7, np.pi), (24, 2 * np.pi), (37, np.pi)):
+        th = np.linspace(0, span, nt, endpoint=False).astype(np.float32).astype(np.float64)
+        rc = lib.mm_radon_plan_check(n, nt, th.ctypes.data_as(C.POINTER(C.c_double)))
+        assert rc == _lib.MM_OK, _lib.last_error()
