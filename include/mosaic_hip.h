@@ -88,6 +88,13 @@ int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
 #define MM_RADON_TUNE_SCATTER_XCD_GROUP 2
 #define MM_RADON_TUNE_NKNOBS 3
 int mm_radon_tune(mm_radon* plan, int knob, int value);
+/* Current value of a knob (or MM_ERR_ARG). */
+int mm_radon_get_tune(const mm_radon* plan, int knob);
+/* Tile autotuner: times R (obj -> sino) for every gather XCD group size in
+ * {1, 2, 4, 8, 16} and keeps the fastest; with back != NULL also RT
+ * (sino -> back) for the scatter knob.  Synchronises the stream (not for use
+ * under graph capture).  obj is only read; sino and back are overwritten. */
+int mm_radon_autotune(mm_radon* plan, const float* obj_dev, float* sino_dev, float* back_dev, int nz, void* stream);
 
 /* Host-only self-check (no counterpart in the reference; for tests on machines
  * without a GPU): builds the gather / scatter tables of an (n, theta) plan and
@@ -132,6 +139,40 @@ int mm_psi_from_proj(const float* proj_dev, void* psi_dev, long long count, floa
                      void* stream);
 /* out = re^2 + im^2 (step3_propagation.py:105-110). */
 int mm_abs2(const void* z_dev, float* out_dev, long long count, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Uneven mosaic batch: for every tile t (detector width n[t], nz[t] rows,
+ * ntheta[t] angles theta_all[sum_{s<t} ntheta[s] ...]),
+ *   out[t] = |D(psi(R delta[t]))|^2,  psi(p) = exp(-k beta_ratio p) exp(i k p),
+ * at one Fresnel distance.  Replaces the tile loops of the reference pipeline:
+ * step2_radon.py:147,191 (R per tile chunk) followed by
+ * step3_propagation.py:175,223 (psi, D, |.|^2 per angle chunk, :94-110).
+ * Tiles of equal (n, angles) share a forward-only Radon plan, tiles of equal
+ * (n, nz) a Fresnel plan; all tiles share the workspaces (sized for the
+ * largest).  delta[t] (nz, n, n) float32, out[t] (ntheta, nz, n) float32,
+ * device pointers.
+ * mm_mosaic_run enqueues the whole batch as one HIP graph (captured on first
+ * use and again whenever the pointers change) on the batch's own stream,
+ * ordered after prior work on `stream` and before later work on it.
+ * flags: MM_MOSAIC_NO_GRAPH enqueues the launches directly (same results);
+ * MM_MOSAIC_AUTOTUNE runs mm_radon_autotune for every Radon plan on its
+ * first tile's input at the first run (synchronous). */
+typedef struct mm_mosaic mm_mosaic;
+#define MM_MOSAIC_NO_GRAPH 1u
+#define MM_MOSAIC_AUTOTUNE 2u
+#define MM_MOSAIC_INFO_RADON_PLANS 0
+#define MM_MOSAIC_INFO_FRESNEL_PLANS 1
+#define MM_MOSAIC_INFO_CAPTURES 2
+#define MM_MOSAIC_INFO_TILES 3
+int mm_mosaic_create(mm_mosaic** plan, int ntiles, const int* n, const int* nz, const int* ntheta,
+                     const double* theta_all, double wavelength, double voxelsize, double distance, float k,
+                     float beta_ratio, int pad_mode, unsigned flags);
+int mm_mosaic_run(mm_mosaic* plan, const float* const* delta_dev, float* const* out_dev, void* stream);
+int mm_mosaic_info(const mm_mosaic* plan, int what);
+/* Gather XCD group knob of tile t's Radon plan (after autotuning: the chosen one). */
+int mm_mosaic_tile_knob(const mm_mosaic* plan, int tile);
+size_t mm_mosaic_workspace_bytes(const mm_mosaic* plan);
+void mm_mosaic_destroy(mm_mosaic* plan);
 
 /* ---------------------------------------------------------------------------
  * Differentiable forward model (not in the reference):

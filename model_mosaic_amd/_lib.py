@@ -20,6 +20,12 @@ MM_RADON_FP64_TILES = 1
 MM_RADON_GATHER_ZP1 = 2
 MM_RADON_TUNE_GATHER_XCD_GROUP = 1
 MM_RADON_TUNE_SCATTER_XCD_GROUP = 2
+MM_MOSAIC_NO_GRAPH = 1
+MM_MOSAIC_AUTOTUNE = 2
+MM_MOSAIC_INFO_RADON_PLANS = 0
+MM_MOSAIC_INFO_FRESNEL_PLANS = 1
+MM_MOSAIC_INFO_CAPTURES = 2
+MM_MOSAIC_INFO_TILES = 3
 MM_RADON_NSTAGES = 8
 RADON_STAGES = ("r_rows_rfft", "r_col_fft", "r_gather", "r_line_ifft",
                 "rt_line_fft", "rt_scatter", "rt_col_ifft", "rt_rows_c2r")
@@ -34,6 +40,8 @@ _SIGS = {
     "mm_radon_adj": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "mm_radon_tune": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mm_radon_plan_check": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mm_radon_get_tune": (C.c_int, [_vp, C.c_int]),
+    "mm_radon_autotune": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "mm_radon_profile": (C.c_int, [_vp, C.c_int]),
     "mm_radon_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_int]),
     "mm_radon_workspace_bytes": (C.c_size_t, [_vp]),
@@ -46,6 +54,14 @@ _SIGS = {
     "mm_fresnel_destroy": (None, [_vp]),
     "mm_psi_from_proj": (C.c_int, [_vp, _vp, C.c_longlong, C.c_float, C.c_float, _vp]),
     "mm_abs2": (C.c_int, [_vp, _vp, C.c_longlong, _vp]),
+    "mm_mosaic_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_float, C.c_float,
+                                   C.c_int, C.c_uint]),
+    "mm_mosaic_run": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp]),
+    "mm_mosaic_info": (C.c_int, [_vp, C.c_int]),
+    "mm_mosaic_tile_knob": (C.c_int, [_vp, C.c_int]),
+    "mm_mosaic_workspace_bytes": (C.c_size_t, [_vp]),
+    "mm_mosaic_destroy": (None, [_vp]),
     "mm_model_create": (C.c_int, [C.POINTER(_vp), _vp, _vp, C.c_int, C.c_float]),
     "mm_model_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
     "mm_model_bwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, _vp, _vp, _vp]),

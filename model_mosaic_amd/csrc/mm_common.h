@@ -57,8 +57,15 @@ __host__ __device__ inline constexpr long tw_offset(int L) {
 // Returns the device pointer of the bank on the current device (built on first use).
 int twiddle_bank(const float2** out);
 
-// Plan dimensions for the modules that compose plans (model.hip).
+// Plan dimensions for the modules that compose plans (model.hip, mosaic.hip).
 void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max);
+// Radon plan creation with optional caller-owned half-spectrum (fde) and
+// representative (gh) buffers shared by plans that never run concurrently,
+// and optionally without the adjoint tables (radon.hip).
+int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags,
+                      float2* fde_ext, float2* gh_ext, bool fwd_only);
+size_t radon_fde_elems(int n, int nz_max);
+size_t radon_gh_elems(int n, int nz_max, int ntheta);
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }

@@ -1299,6 +1299,8 @@ struct mm_radon {
   int n_splits = 0, n_slots = 0;
   const float2* tw = nullptr;
   size_t bytes = 0;
+  bool own_ws = true;     // fde / gh allocated by the plan (false: a batch's shared workspace)
+  bool fwd_only = false;  // no adjoint scatter tables (mm_radon_adj unsupported)
   bool profile = false;
   bool ran[MM_RADON_NSTAGES] = {};
   hipEvent_t ev[MM_RADON_NSTAGES + 1][2] = {};
@@ -1317,6 +1319,7 @@ static void radon_free(mm_radon* p) {
   for (auto& e : p->ev)
     for (auto& x : e)
       if (x) (void)hipEventDestroy(x);
+  if (!p->own_ws) p->fde = p->gh = nullptr;
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
                   p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
                   p->splits};
@@ -1446,6 +1449,21 @@ void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max) {
 extern "C" {
 
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags) {
+  return radon_create_impl(plan, n, nz_max, ntheta, theta_host, flags, nullptr, nullptr, false);
+}
+
+}  // extern "C"
+
+namespace mm {
+size_t radon_fde_elems(int n, int nz_max) { return (size_t)nz_max * (size_t)(n / 4 + 1) * 2 * n * 4; }
+size_t radon_gh_elems(int n, int nz_max, int ntheta) { return (size_t)ntheta * nz_max * (n / 2 + 1); }
+
+// Plan creation.  fde_ext / gh_ext (both or neither): caller-owned half-spectrum
+// and representative buffers of at least radon_fde_elems / radon_gh_elems
+// complex values, shared by plans that never run concurrently (a mosaic
+// batch).  fwd_only skips the adjoint scatter tables.
+int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags,
+                      float2* fde_ext, float2* gh_ext, bool fwd_only) {
   MM_ARG_CHECK(plan != nullptr, "mm_radon_create: plan is NULL");
   *plan = nullptr;
   MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && 2 * n <= kMaxFFT,
@@ -1511,7 +1529,8 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   SBinning BS;
   const int root = std::min(ROOT, n);
   build_gather(G, root, BG);
-  build_scatter_lists(G, BS);
+  p->fwd_only = fwd_only;
+  if (!fwd_only) build_scatter_lists(G, BS);
 
   size_t& bytes = p->bytes;
   if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
@@ -1525,15 +1544,23 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
   p->n_sitems = (int)BS.items.size();
   p->n_splits = (int)BS.splits.size();
   p->n_slots = BS.nslots;
-  if ((rc = dmalloc((void**)&p->fde, sizeof(float2) * (size_t)nz_max * p->zs, bytes, "fde")) ||
-      (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives")))
+  if (fde_ext && gh_ext) {
+    p->own_ws = false;
+    p->fde = fde_ext;
+    p->gh = gh_ext;
+  } else if ((rc = dmalloc((void**)&p->fde, sizeof(float2) * (size_t)nz_max * p->zs, bytes, "fde")) ||
+             (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives"))) {
     return bail(rc);
+  }
   if (p->n_slots > 0 &&
       (rc = dmalloc((void**)&p->partial, sizeof(float2) * (size_t)p->n_slots * nz_max * SNP, bytes, "partial tiles")))
     return bail(rc);
   *plan = p;
   return MM_OK;
 }
+}  // namespace mm
+
+extern "C" {
 
 int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* stream) {
   MM_ARG_CHECK(p != nullptr, "mm_radon_fwd: plan is NULL");
@@ -1560,6 +1587,7 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
                "mm_radon_adj: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
   MM_ARG_CHECK(sino && obj, "mm_radon_adj: null pointer");
   MM_ARG_CHECK(sino_is_c64 == 0 || sino_is_c64 == 1, "mm_radon_adj: sino_is_c64 must be 0 or 1");
+  if (p->fwd_only) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj: forward-only plan (no scatter tables)");
   const bool cplx = sino_is_c64 != 0;
   if (cplx) {
     int rc;
@@ -1599,6 +1627,55 @@ int mm_radon_tune(mm_radon* p, int knob, int value) {
   MM_ARG_CHECK(value >= 1 && value <= 64, "mm_radon_tune: XCD group size must be in [1, 64]");
   p->tune[knob] = value;
   return MM_OK;
+}
+
+int mm_radon_autotune(mm_radon* p, const float* obj, float* sino, float* back, int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_autotune: plan is NULL");
+  MM_ARG_CHECK(obj && sino, "mm_radon_autotune: null pointer");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max, "mm_radon_autotune: nz outside [1, nz_max]");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MM_HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(MM_ERR_HIP, "mm_radon_autotune: hipEventCreate");
+  }
+  int rc = MM_OK;
+  // best of two timed calls for every XCD group size; the knob keeps the fastest
+  auto sweep = [&](int knob, auto&& call) {
+    const int cand[] = {1, 2, 4, 8, 16};
+    float best_ms = 1e30f;
+    int best = p->tune[knob];
+    if ((rc = call())) return;  // warm-up
+    for (int c : cand) {
+      p->tune[knob] = c;
+      for (int r = 0; r < 2; ++r) {
+        if (hipEventRecord(e0, st) != hipSuccess || (rc = call()) || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess) {
+          if (!rc) rc = fail(MM_ERR_HIP, "mm_radon_autotune: event timing");
+          return;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    }
+    p->tune[knob] = best;
+  };
+  sweep(MM_RADON_TUNE_GATHER_XCD_GROUP, [&]() { return mm_radon_fwd(p, obj, sino, nz, stream); });
+  if (!rc && back && !p->fwd_only)
+    sweep(MM_RADON_TUNE_SCATTER_XCD_GROUP, [&]() { return mm_radon_adj(p, sino, 0, back, nz, stream); });
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int mm_radon_get_tune(const mm_radon* p, int knob) {
+  if (!p || knob < 1 || knob >= MM_RADON_TUNE_NKNOBS) return MM_ERR_ARG;
+  return p->tune[knob];
 }
 
 int mm_radon_profile(mm_radon* p, int enable) {
