@@ -166,6 +166,65 @@ def run_model(torch, steps=3, warmup=1):
             "algorithmic_GB": alg / 1e9, "hbm_frac": alg / dt / 1e9 / PEAK_HBM_GBS}
 
 
+def mosaic_spec(seed=7):
+    """Uneven mosaic: 4 x-tiles x 9 z-stacks of 96 rows (8-row overlaps between stacks); the x-tiles' detector
+    widths are a seeded draw without replacement from {1536, 2048, 3072, 4096} (so the batch fits one GPU:
+    ~113 GB of inputs, ~57 GB of outputs), angle counts per tile drawn from {1029, 2058}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nx, nzs, rows, overlap = 4, 9, 96, 8
+    widths = rng.permutation([1536, 2048, 3072, 4096])[:nx]
+    tiles = []
+    for i in range(nx):
+        for s in range(nzs):
+            nt = int(rng.choice([1029, 2058]))
+            tiles.append(dict(x=i, s=s, n=int(widths[i]), nz=rows, ntheta=nt, rows=(s * (rows - overlap), s * (rows - overlap) + rows),
+                              ztotal=nzs * (rows - overlap) + overlap))
+    return tiles
+
+
+def run_mosaic(torch, reps=2):
+    """R, psi, Fresnel D (30 keV, 0.69 um, z = 0.2 m) and |.|^2 over the uneven mosaic as one HIP graph."""
+    import numpy as np
+
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.mosaic import MosaicBatch
+
+    spec = mosaic_spec()
+    dev = torch.device("cuda", 0)
+    deltas = [phantom.ellipsoids(t["ztotal"], t["n"], seed=100 + t["x"], rows=t["rows"], device=dev) for t in spec]
+    tiles = [(t["n"], t["nz"], phantom.angles(t["ntheta"])) for t in spec]
+    # one k for the batch: the largest phase k p about 2 rad (p ~ n max(delta) / (2 sqrt(n ntheta)))
+    pmax = max(float(d.max()) * 0.5 * np.sqrt(t["n"] / t["ntheta"]) for d, t in zip(deltas, spec))
+    k = 2.0 / pmax
+    t0 = time.perf_counter()
+    mb = MosaicBatch(tiles, 1.24e-9 / 30.0, 0.69e-6, 0.2, k, 1e-3, autotune=True)
+    plan_s = time.perf_counter() - t0
+    outs = mb.out_buffers()
+    mb.run(deltas, outs)                       # autotune + capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mb.run(deltas, outs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    info = mb.info()
+    alg = sum(t["nz"] * b_dir(t["n"], t["ntheta"]) + 56 * t["ntheta"] * t["nz"] * t["n"] for t in spec)
+    nproj = sum(t["ntheta"] for t in spec)
+    res = {"config": "4 x-tiles x 9 z-stacks, 96 rows each (8-row overlaps), widths per x-tile from {1536, 2048, 3072, "
+                     "4096}, ntheta per tile from {1029, 2058}, R then psi, Fresnel D (30 keV, 0.69 um, 0.2 m), |.|^2",
+           "widths": sorted({t["n"] for t in spec}), "tiles": len(spec), "projections": nproj,
+           "s_per_batch": round(dt, 5), "projections_per_s": round(nproj / dt, 1),
+           "algorithmic_GB": round(alg / 1e9, 3), "hbm_frac": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
+           "plan_s": round(plan_s, 2), "radon_plans": info["radon_plans"], "fresnel_plans": info["fresnel_plans"],
+           "graph_captures": info["graph_captures"], "gather_xcd_group": sorted(set(info["gather_xcd_group"])),
+           "workspace_GB": round(info["workspace_bytes"] / 1e9, 2)}
+    mb.free()
+    del deltas, outs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +237,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fresnel", action="store_true")
+    ap.add_argument("--no-mosaic", action="store_true")
+    ap.add_argument("--only-mosaic", action="store_true", help="run only the uneven mosaic leg (testing)")
     ap.add_argument("--gather-zp", type=int, default=2, choices=(1, 2),
                     help="z slices per gather patch element (TomoReal gather_zp)")
     ap.add_argument("--gather-xcd-group", type=int, default=0, help="override the gather XCD group knob")
@@ -199,6 +260,9 @@ def main():
     lib = _lib.load()
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
+    if a.only_mosaic:
+        print(json.dumps({"mosaic": run_mosaic(torch)}), flush=True)
+        return
     nz, n, nt = a.nz, a.n, a.ntheta
     if a.copy_calib_mib > 0:
         src = torch.ones(a.copy_calib_mib << 18, dtype=torch.float32, device=dev)
@@ -293,6 +357,13 @@ def main():
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    mosaic = None
+    if not a.no_mosaic:
+        print("[bench] uneven mosaic batch, one HIP graph", file=sys.stderr, flush=True)
+        mosaic = run_mosaic(torch)
+        print(f"[bench] mosaic {mosaic['s_per_batch'] * 1e3:.1f} ms per batch of {mosaic['tiles']} tiles, "
+              f"{mosaic['projections_per_s']:.0f} projections/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
@@ -323,6 +394,7 @@ def main():
         "batched_over_plain": None if plain is None else round(value / plain, 3),
         "model_fwd_bwd": model,
         "fresnel_D_DT": fres,
+        "mosaic": mosaic,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

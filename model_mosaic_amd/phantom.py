@@ -32,14 +32,18 @@ def ellipsoid_params(nz: int, n: int, count: int = 12, seed: int = 0, vrange=(1e
 
 
 def ellipsoids(nz: int, n: int, count: int = 12, seed: int = 0, noise: float = 0.0, device=None,
-               vrange=(1e-7, 1e-6)) -> torch.Tensor:
+               vrange=(1e-7, 1e-6), rows=None) -> torch.Tensor:
     """(nz, n, n) float32 sum of seeded ellipsoids; ``noise`` adds a Gaussian
-    texture of that relative strength (of the peak value) on the support."""
+    texture of that relative strength (of the peak value) on the support.
+    ``rows=(r0, r1)`` evaluates only rows [r0, r1) of the nz-row phantom (a
+    z-stack of a taller sample; neighbouring stacks may overlap)."""
     device = torch.device(device) if device is not None else torch.device("cpu")
-    vol = torch.zeros((nz, n, n), dtype=torch.float32, device=device)
+    r0, r1 = (0, nz) if rows is None else rows
+    vol = torch.zeros((r1 - r0, n, n), dtype=torch.float32, device=device)
     for e in ellipsoid_params(nz, n, count, seed, vrange):
         (cz, cy, cx), (az, ay, ax), v = e["c"], e["a"], e["v"]
-        z0, z1 = max(0, int(cz - az)), min(nz, int(cz + az) + 1)
+        cz -= r0
+        z0, z1 = max(0, int(cz - az)), min(r1 - r0, int(cz + az) + 1)
         y0, y1 = max(0, int(cy - ay)), min(n, int(cy + ay) + 1)
         x0, x1 = max(0, int(cx - ax)), min(n, int(cx + ax) + 1)
         if z0 >= z1 or y0 >= y1 or x0 >= x1:
