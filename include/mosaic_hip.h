@@ -175,6 +175,18 @@ size_t mm_mosaic_workspace_bytes(const mm_mosaic* plan);
 void mm_mosaic_destroy(mm_mosaic* plan);
 
 /* ---------------------------------------------------------------------------
+ * Reference-structured comparator (the benchmark's plain one-slice path; not
+ * used by the operators above): the per-sample 81-tap gather and atomic
+ * scatter of processing/kernels.py:274-349 and :16-93 on one slice, reading /
+ * adding taps directly in global memory.
+ * mm_plain_gather: F (2n, n+1) complex64 rfft2 of the padded slice ->
+ *   g (ntheta, n) complex64 (the value the reference's gather writes).
+ * mm_plain_scatter: g (ntheta, n) complex64 -> G (2n, 2n) complex64, centred
+ *   full grid, accumulated (the caller zeroes G). theta_dev: ntheta float32. */
+int mm_plain_gather(const void* F_dev, void* g_dev, const float* theta_dev, int n, int ntheta, void* stream);
+int mm_plain_scatter(const void* g_dev, void* G_dev, const float* theta_dev, int n, int ntheta, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Differentiable forward model (not in the reference):
  *   p = R delta, b = R beta  (ntheta, nz, n),
  *   psi = exp(i k p - k b), u = D_j psi, I = |u|^2,

@@ -62,6 +62,8 @@ _SIGS = {
     "mm_mosaic_tile_knob": (C.c_int, [_vp, C.c_int]),
     "mm_mosaic_workspace_bytes": (C.c_size_t, [_vp]),
     "mm_mosaic_destroy": (None, [_vp]),
+    "mm_plain_gather": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "mm_plain_scatter": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "mm_model_create": (C.c_int, [C.POINTER(_vp), _vp, _vp, C.c_int, C.c_float]),
     "mm_model_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
     "mm_model_bwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, _vp, _vp, _vp]),

@@ -207,3 +207,19 @@ def test_large_adjoint_and_validation():
     assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) < 1e-6
     with pytest.raises(AssertionError):
         tl.R(x, [24, nt, n])
+
+
+@pytest.mark.parametrize("n,nt", [(64, 24), (128, 50)])
+def test_plain_reference_structured_path(n, nt):
+    """The benchmark's plain one-slice comparator computes the same operator (FP64 oracle, 1e-5)."""
+    from model_mosaic_amd.plain_path import PlainTomo
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((1, n, n)).astype(np.float32)
+    y = rng.standard_normal((nt, 1, n)).astype(np.float32)
+    p = PlainTomo(n, th)
+    r = p.R(torch.from_numpy(x[0]).cuda()).cpu().numpy()
+    print(f"plain R rel-L2 vs FP64 {rel(r, usfft.tomo_R(x, th, np.float64)[:, 0]):.2e}")
+    b = p.RT(torch.from_numpy(y[:, 0]).cuda()).cpu().numpy()
+    assert rel(r, usfft.tomo_R(x, th, np.float64)[:, 0]) < 1e-5
+    assert rel(b, usfft.tomo_RT(y, th, dtype=np.float64)[0]) < 1e-5
