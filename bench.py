@@ -243,6 +243,7 @@ def main():
                     help="z slices per gather patch element (TomoReal gather_zp)")
     ap.add_argument("--gather-xcd-group", type=int, default=0, help="override the gather XCD group knob")
     ap.add_argument("--scatter-xcd-group", type=int, default=0, help="override the scatter XCD group knob")
+    ap.add_argument("--col-split", type=int, default=-1, help="override the column-FFT split knob (0 / 1)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB first (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -282,6 +283,8 @@ def main():
         _lib.check(lib.mm_radon_tune(tomo._plan, _lib.MM_RADON_TUNE_GATHER_XCD_GROUP, a.gather_xcd_group), "tune")
     if a.scatter_xcd_group:
         _lib.check(lib.mm_radon_tune(tomo._plan, _lib.MM_RADON_TUNE_SCATTER_XCD_GROUP, a.scatter_xcd_group), "tune")
+    if a.col_split >= 0:
+        _lib.check(lib.mm_radon_tune(tomo._plan, _lib.MM_RADON_TUNE_COL_SPLIT, a.col_split), "tune")
     sino = torch.empty((nt, nz, n), dtype=torch.float32, device=dev)
     back = torch.empty((nz, n, n), dtype=torch.float32, device=dev)
     print(f"[bench] plan {plan_s:.2f} s, workspace {tomo.workspace_bytes() / 2**30:.2f} GiB; warmup",

@@ -83,10 +83,14 @@ int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
  * MM_RADON_TUNE_GATHER_XCD_GROUP / MM_RADON_TUNE_SCATTER_XCD_GROUP: number G
  * of neighbouring gather / scatter work items that run on one XCD (one L2)
  * before the next XCD takes the next G; 1 = plain round-robin order.
+ * MM_RADON_TUNE_COL_SPLIT: 0 = column FFTs four columns per workgroup (one
+ * per CU), 1 = two columns per workgroup (two per CU, block halves paired on
+ * one XCD).
  * Results do not depend on the knobs, only the speed does. */
 #define MM_RADON_TUNE_GATHER_XCD_GROUP 1
 #define MM_RADON_TUNE_SCATTER_XCD_GROUP 2
-#define MM_RADON_TUNE_NKNOBS 3
+#define MM_RADON_TUNE_COL_SPLIT 3
+#define MM_RADON_TUNE_NKNOBS 4
 int mm_radon_tune(mm_radon* plan, int knob, int value);
 /* Current value of a knob (or MM_ERR_ARG). */
 int mm_radon_get_tune(const mm_radon* plan, int knob);

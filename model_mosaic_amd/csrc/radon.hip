@@ -101,19 +101,22 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   using Sh = RowReg<L>;
   constexpr int R2 = 2 * Sh::C;
   __shared__ float2 lds[Sh::LDS];
-  __shared__ float2* rowp[R2];  // destination row r of this workgroup in the half spectrum (null: none)
+  // offset of destination row r of this workgroup in the half spectrum (-1: none); offsets from the
+  // kernel argument keep the stores global (a pointer read back from LDS would make them flat
+  // stores, which the LDS reads of the next element then wait for)
+  __shared__ long long rowoff[R2];
   int c, t;
   Sh::map(threadIdx.x, c, t);
   const int half = n / 2;
   const long long npairs = (long long)nz * half;
   if (threadIdx.x < R2) {
     const long long pr = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
-    float2* rp = nullptr;
+    long long ro = -1;
     if (pr < npairs) {
       const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (threadIdx.x & 1);
-      rp = fde + (long long)zz * fzs + 4LL * (half + yy);  // hs_idx(half + yy, col) = this + col part
+      ro = (long long)zz * fzs + 4LL * (half + yy);  // hs_idx(half + yy, col) = this + col part
     }
-    rowp[threadIdx.x] = rp;
+    rowoff[threadIdx.x] = ro;
   }
   const long long pair = (long long)blockIdx.x * Sh::C + c;
   const bool valid = pair < npairs;
@@ -141,19 +144,24 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
   // unpack A = (Z_k + conj Z_{L-k})/2 (row y0), B = (Z_k - conj Z_{L-k})/(2i) (row y0+1);
-  // thread order: column within block, then the 2C rows of this workgroup, then block
-  const int nblk = n / 4 + 1;
-  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
+  // thread order: column within block, then the 2C rows of this workgroup, then block.
+  // Unrolled: every element's LDS reads are independent of the global stores.
+  constexpr int NBLK = L / 8 + 1;  // n / 4 + 1 column blocks
+  constexpr int TOT = NBLK * R2 * 4, IT = (TOT + Sh::NT - 1) / Sh::NT;
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = threadIdx.x + i * Sh::NT;
+    if (idx >= TOT) continue;
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
-    float2* rp = rowp[r];
-    if (col > n || rp == nullptr) continue;
+    const long long ro = rowoff[r];
+    if (col > n || ro < 0) continue;
     const float2* xc = lds + (r >> 1) * Sh::S;
     const float2 zk = xc[lpad(col)], zm = xc[lpad(col == 0 ? 0 : L - col)];
     const float2 w = (r & 1) ? make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x))
                              : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-    rp[(long long)blk * 4 * L + q] = w;  // hs_idx(row, col, L) = 4 row + 4 L (col >> 2) + (col & 3)
+    fde[ro + (long long)blk * 4 * L + q] = w;  // hs_idx(row, col, L) = 4 row + 4 L (col >> 2) + (col & 3)
   }
 }
 
@@ -166,18 +174,27 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
 template <int L>
 using ColShape = RegShape<L, kBatchElems>;
 
-template <int L, bool INV>
-__global__ __launch_bounds__(ColShape<L>::NT) void col_fft(float2* __restrict__ buf, int ncols,
-                                                                      long long zs, int in_lo, int in_hi,
-                                                                      int out_lo, int out_hi, float scale,
-                                                                      const float2* __restrict__ tw) {
-  using Sh = ColShape<L>;
+//
+// BATCH = kBatchElems: C = 4 columns (one 32-byte block row piece) per
+// workgroup, one workgroup per CU.  BATCH = kRowBatch: C = 2 (half a block),
+// two workgroups per CU, the two halves of a block kept on one XCD (xg = 2)
+// so that its L2 fetches each 128-byte line once.
+template <int L, int BATCH = kBatchElems>
+using ColShapeB = RegShape<L, BATCH>;
+
+template <int L, bool INV, int BATCH = kBatchElems>
+__global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __restrict__ buf, int ncols,
+                                                                   long long zs, int in_lo, int in_hi,
+                                                                   int out_lo, int out_hi, float scale,
+                                                                   const float2* __restrict__ tw, int xg) {
+  using Sh = ColShapeB<L, BATCH>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
-  const int col = blockIdx.x * Sh::C + c;
+  const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
+  const int col = (int)(lw % gridDim.x) * Sh::C + c;
   const bool live = col < ncols;
-  float2* b = buf + (long long)blockIdx.y * zs;
+  float2* b = buf + (long long)(lw / gridDim.x) * zs;
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
@@ -749,38 +766,53 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
   constexpr int R2 = 2 * Sh::C;
   static_assert(OFF + C2rOff<L>::A <= Sh::S, "staging fits the transform's LDS");
   __shared__ float2 lds[Sh::LDS];
-  __shared__ const float2* rowp[R2];  // source row r (transform r / 2, A/B = r & 1); null: none
+  // offset of source row r (transform r / 2, A/B = r & 1; complex output: B from Him) in its half
+  // plane (-1: none).  Offsets from the kernel arguments keep the loads global; a pointer read back
+  // from LDS made them flat loads that each waited before the next.
+  __shared__ long long rowoff[R2];
   const int half = n / 2;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
   if (threadIdx.x < R2) {
     const long long u = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
     const int which = threadIdx.x & 1;
-    const float2* rp = nullptr;
+    long long ro = -1;
     if (u < nunits) {
       int z, row;
       if (CPLX) {
         z = (int)(u / n);
         row = half + (int)(u - (long long)z * n);
-        rp = which ? Him : Hre;
       } else {
         z = (int)(u / half);
         row = half + 2 * (int)(u - (long long)z * half) + which;
-        rp = Hre;
       }
-      rp += (long long)z * hzs + 4LL * row;  // hs_idx(row, col) = this + col part
+      ro = (long long)z * hzs + 4LL * row;  // hs_idx(row, col) = this + col part
     }
-    rowp[threadIdx.x] = rp;
+    rowoff[threadIdx.x] = ro;
   }
   __syncthreads();
-  const int nblk = n / 4 + 1;
-  for (int idx = threadIdx.x; idx < nblk * R2 * 4; idx += Sh::NT) {
+  // all of this thread's staging loads are issued before its LDS stores
+  constexpr int NBLK = L / 8 + 1;  // n / 4 + 1 column blocks
+  constexpr int TOT = NBLK * R2 * 4, IT = (TOT + Sh::NT - 1) / Sh::NT;
+  float2 sv[IT];
+  int sd[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int idx = threadIdx.x + i * Sh::NT;
+    sd[i] = -1;
+    sv[i] = make_float2(0.f, 0.f);
+    if (idx >= TOT) continue;
     const int q = idx & 3, rest = idx >> 2;
     const int r = rest % R2, blk = rest / R2;
     const int col = 4 * blk + q;
-    const float2* rp = rowp[r];
-    if (col > n || rp == nullptr) continue;
-    lds[(r >> 1) * Sh::S + (r & 1) * OFF + lpad(col)] = rp[(long long)blk * 4 * L + q];
+    const long long ro = rowoff[r];
+    if (col > n || ro < 0) continue;
+    const float2* src = (CPLX && (r & 1)) ? Him : Hre;
+    sv[i] = src[ro + (long long)blk * 4 * L + q];
+    sd[i] = (r >> 1) * Sh::S + (r & 1) * OFF + lpad(col);
   }
+#pragma unroll
+  for (int i = 0; i < IT; ++i)
+    if (sd[i] >= 0) lds[sd[i]] = sv[i];
   __syncthreads();
   int c, t;
   Sh::map(threadIdx.x, c, t);
@@ -1289,7 +1321,7 @@ struct mm_radon {
   unsigned short* gorder = nullptr;
   int n_gitems = 0;
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
-  int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
+  int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   SItem* sitems = nullptr;
   unsigned char* sperm = nullptr;
   int* sentries = nullptr;
@@ -1351,10 +1383,17 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
 template <int L, bool INV>
 static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in_hi, int out_lo, int out_hi,
                        float scale, hipStream_t st) {
-  using Sh = ColShape<L>;
   const int ncols = p->n + 1;
-  hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                     0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + tw_offset(L));
+  if (p->tune[MM_RADON_TUNE_COL_SPLIT] && ColShapeB<L, kRowBatch>::C == 2) {
+    using Sh = ColShapeB<L, kRowBatch>;
+    hipLaunchKernelGGL((col_fft<L, INV, kRowBatch>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz),
+                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale,
+                       p->tw + tw_offset(L), 2);
+  } else {
+    using Sh = ColShape<L>;
+    hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
+                       0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + tw_offset(L), 1);
+  }
 }
 
 template <int L>
@@ -1622,9 +1661,11 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
 
 int mm_radon_tune(mm_radon* p, int knob, int value) {
   MM_ARG_CHECK(p != nullptr, "mm_radon_tune: plan is NULL");
-  MM_ARG_CHECK(knob == MM_RADON_TUNE_GATHER_XCD_GROUP || knob == MM_RADON_TUNE_SCATTER_XCD_GROUP,
-               "mm_radon_tune: unknown knob " + std::to_string(knob));
-  MM_ARG_CHECK(value >= 1 && value <= 64, "mm_radon_tune: XCD group size must be in [1, 64]");
+  MM_ARG_CHECK(knob >= 1 && knob < MM_RADON_TUNE_NKNOBS, "mm_radon_tune: unknown knob " + std::to_string(knob));
+  if (knob == MM_RADON_TUNE_COL_SPLIT)
+    MM_ARG_CHECK(value == 0 || value == 1, "mm_radon_tune: column split must be 0 or 1");
+  else
+    MM_ARG_CHECK(value >= 1 && value <= 64, "mm_radon_tune: XCD group size must be in [1, 64]");
   p->tune[knob] = value;
   return MM_OK;
 }
