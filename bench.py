@@ -166,6 +166,29 @@ def run_model(torch, steps=3, warmup=1):
             "algorithmic_GB": alg / 1e9, "hbm_frac": alg / dt / 1e9 / PEAK_HBM_GBS}
 
 
+def run_fbp(torch, reps=5):
+    """FBP filter (shepp) of a 2048-angle x 64-slice x 2048 sinogram in place: one read + one write."""
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter
+    nt, nz, n = 2048, 64, 2048
+    dev = torch.device("cuda", 0)
+    sino = torch.randn((nt, nz, n), device=dev, dtype=torch.float32)
+    f = FBPFilter(n)
+    w = f.calc_filter("shepp")
+    f.filter(sino, w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f.filter(sino, w)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = 2 * 4 * nt * nz * n
+    del sino
+    torch.cuda.empty_cache()
+    return {"config": f"shepp filter, sinogram {nt} x {nz} x {n} float32, 4x edge padding, in place",
+            "ms_per_call": round(dt * 1e3, 4), "achieved_GBs": round(nbytes / dt / 1e9, 1),
+            "hbm_frac": round(nbytes / dt / 1e9 / PEAK_HBM_GBS, 4)}
+
+
 def mosaic_spec(seed=7):
     """Uneven mosaic: 4 x-tiles x 9 z-stacks of 96 rows (8-row overlaps between stacks); the x-tiles' detector
     widths are a seeded draw without replacement from {1536, 2048, 3072, 4096} (so the batch fits one GPU:
@@ -238,6 +261,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fresnel", action="store_true")
     ap.add_argument("--no-mosaic", action="store_true")
+    ap.add_argument("--no-fbp", action="store_true")
     ap.add_argument("--only-mosaic", action="store_true", help="run only the uneven mosaic leg (testing)")
     ap.add_argument("--gather-zp", type=int, default=2, choices=(1, 2),
                     help="z slices per gather patch element (TomoReal gather_zp)")
@@ -375,6 +399,12 @@ def main():
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    fbp = None
+    if not a.no_fbp:
+        print("[bench] FBP filter", file=sys.stderr, flush=True)
+        fbp = run_fbp(torch)
+        print(f"[bench] FBP filter {fbp['ms_per_call']:.2f} ms per 2048x64x2048 sinogram", file=sys.stderr, flush=True)
+
     mosaic = None
     if not a.no_mosaic:
         print("[bench] uneven mosaic batch, one HIP graph", file=sys.stderr, flush=True)
@@ -417,6 +447,7 @@ def main():
         "model_fwd_bwd": model,
         "fresnel_D_DT": fres,
         "mosaic": mosaic,
+        "fbp_filter": fbp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

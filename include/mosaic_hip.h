@@ -136,6 +136,15 @@ size_t mm_fresnel_workspace_bytes(const mm_fresnel* plan);
 void mm_fresnel_destroy(mm_fresnel* plan);
 
 /* ---------------------------------------------------------------------------
+ * FBP filter.  Replaces processing/fbp_filter.py:128-210 (FBPFilter.filter):
+ * every row of sino (rows x n float32, in place) is edge-replicated to
+ * ne = 4 n, filtered with the real half-spectrum weights w (ne / 2 + 1
+ * float32, FBPFilter.calc_filter, :48-91) and cropped back:
+ * row <- crop(irfft(rfft(edge_pad(row)) * w, ne)).  4 n must be 2^k or 3 * 2^k
+ * in [64, 16384]. */
+int mm_fbp_filter(float* sino_dev, const float* w_dev, long long rows, int n, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Element-wise kernels of step3_propagation.py:94-110.
  * psi = exp(-p*inv_norm*inv_beta_ratio) * (cos(p*inv_norm) + i sin(p*inv_norm)).
  */

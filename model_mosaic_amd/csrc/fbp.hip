@@ -1,0 +1,80 @@
+// FBP filter of the sinogram rows (processing/fbp_filter.py:128-210): edge
+// replication to ne = 4 n, FFT, multiplication by real weights w (ne / 2 + 1
+// values), inverse FFT / ne, crop back to n -- in place, one kernel.
+//
+// Two real rows ride in one complex transform of length ne (row A in the real
+// part, row B in the imaginary part): the weights are real and even
+// (w[k] = w[ne - k] extends the half-spectrum weights), so the inverse
+// transform returns filtered A and filtered B as its real and imaginary
+// parts -- the same values irfft(rfft(row) w, ne) gives for each row.  A
+// workgroup reads its rows once into the register FFT and writes the cropped
+// results once: the sinogram makes one HBM round trip.
+#include "mm_fft.h"
+
+namespace mm {
+namespace fbp {
+
+template <int NE>
+__global__ __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict__ sino, const float* __restrict__ w,
+                                                             long long rows, int n, const float2* __restrict__ tw) {
+  using Sh = RowReg<NE>;
+  __shared__ float2 lds[Sh::LDS];
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const long long ra = 2 * ((long long)blockIdx.x * Sh::C + c), rb = ra + 1;
+  const bool has_a = ra < rows, has_b = rb < rows;
+  float* pa = sino + (has_a ? ra : 0) * n;
+  float* pb = sino + (has_b ? rb : 0) * n;
+  const int pad = (NE - n) / 2;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int i = t + m * Sh::TPC;
+    const int j = min(max(i - pad, 0), n - 1);  // edge replication
+    v[m] = make_float2(has_a ? pa[j] : 0.f, has_b ? pb[j] : 0.f);
+  }
+  fft_reg<NE, Sh::TPC, false>(v, lds + c * Sh::S, t, tw);
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int i = t + m * Sh::TPC;
+    v[m] = cscale(v[m], w[i <= NE / 2 ? i : NE - i]);
+  }
+  fft_reg<NE, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
+  const float s = 1.0f / (float)NE;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int i = t + m * Sh::TPC - pad;
+    if (i < 0 || i >= n) continue;
+    if (has_a) pa[i] = v[m].x * s;
+    if (has_b) pb[i] = v[m].y * s;
+  }
+}
+
+template <int NE>
+static void launch(float* sino, const float* w, long long rows, int n, const float2* tw, hipStream_t st) {
+  using Sh = RowReg<NE>;
+  const long long pairs = (rows + 1) / 2;
+  hipLaunchKernelGGL(filter_rows<NE>, dim3((unsigned)((pairs + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, sino, w,
+                     rows, n, tw + tw_offset(NE));
+}
+
+}  // namespace fbp
+}  // namespace mm
+
+extern "C" {
+
+int mm_fbp_filter(float* sino_dev, const float* w_dev, long long rows, int n, void* stream) {
+  MM_ARG_CHECK(rows >= 0, "mm_fbp_filter: rows < 0");
+  MM_ARG_CHECK(n >= 4 && mm::is_fft_len(4L * n) && 4 * n >= 64 && 4 * n <= mm::kMaxFFT,
+               "mm_fbp_filter: 4n must be an FFT length 2^k or 3 * 2^k in [64, 16384], got n=" + std::to_string(n));
+  if (rows == 0) return MM_OK;
+  MM_ARG_CHECK(sino_dev && w_dev, "mm_fbp_filter: null pointer");
+  MM_ARG_CHECK((rows + 1) / 2 < (1LL << 31), "mm_fbp_filter: too many rows");
+  const float2* tw = nullptr;
+  int rc = mm::twiddle_bank(&tw);
+  if (rc) return rc;
+  MM_DISPATCH_FFT(4 * n, (mm::fbp::launch<LL>(sino_dev, w_dev, rows, n, tw, (hipStream_t)stream)));
+  return mm::check_launch("fbp filter_rows");
+}
+
+}  // extern "C"

@@ -65,3 +65,13 @@ def test_radon_plan_tables_match_kernel_indexing(n):
         th = np.linspace(0, span, nt, endpoint=False).astype(np.float32).astype(np.float64)
         rc = lib.mm_radon_plan_check(n, nt, th.ctypes.data_as(C.POINTER(C.c_double)))
         assert rc == _lib.MM_OK, _lib.last_error()
+
+
+@pytest.mark.parametrize("n", [16, 64, 96, 2048])
+def test_fbp_filter_weights_equal_the_oracle(n):
+    """Host weights of the product (FBPFilter.calc_filter) = the oracle's restatement, bit for bit."""
+    from model_mosaic_amd.processing.fbp_filter import FILTERS, calc_filter_host
+
+    from oracle import fbp
+    for name in FILTERS:
+        assert np.array_equal(calc_filter_host(n, name), fbp.calc_filter(n, name)), name
