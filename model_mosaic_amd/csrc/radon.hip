@@ -626,11 +626,16 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
     const SItem* __restrict__ items, const unsigned char* __restrict__ perm, const int* __restrict__ entries,
     const unsigned short* __restrict__ lists, const double2* __restrict__ dirs, int n, int H, int nz, long long hzs,
-    int zchunk, float a, int xg) {
+    int zchunk, float a2, int xg) {
   __shared__ SRow R[SE_MAX];
+  __shared__ float2 TT[128];  // tap code t = 9 i1 + i0 -> (i0 - M, i1 - M); idle code: far away (weight 0)
   const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
   const unsigned item = lw % gridDim.x;
   const SItem it = items[item];
+  if (threadIdx.x < 128) {
+    const int t = threadIdx.x;
+    TT[t] = t < TAPS * TAPS ? make_float2((float)(t % TAPS - M), (float)(t / TAPS - M)) : make_float2(1e3f, 1e3f);
+  }
   const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
   const int p = threadIdx.x;                                 // lane: list lp[.. p ..]
   const int pt = perm[(long long)item * SNP + p];           // the tile point it owns
@@ -679,15 +684,14 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
     auto tap = [&](unsigned u) {
       const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
-      const int i1 = t / TAPS, i0 = t - TAPS * i1;
       const float4* row = reinterpret_cast<const float4*>(&R[e]);
-      // f read as a full 16-byte piece (one LDS cycle per 16 lanes, issued with the
-      // value pieces); the weight is computed for every slot and zeroed for idle ones
-      // (a select, not a branch around the read)
+      // f is read with the value pieces; the tap's integer offsets come from TT (no
+      // division by 9, no conversions), and an idle slot's offsets are so far off that
+      // its weight underflows to exactly 0 (no select); a2 = -a log2(e)
       const float4 ft = row[SZC / 2];  // f, tk
-      const float dx = (float)(i0 - M) - ft.x, dy = (float)(i1 - M) - ft.y;
-      const float we = __expf(-a * (dx * dx + dy * dy));
-      const float w = t == (int)SL_IDLE_TAP ? 0.f : we;
+      const float2 tb = TT[t];
+      const float dx = tb.x - ft.x, dy = tb.y - ft.y;
+      const float w = __builtin_amdgcn_exp2f(a2 * (dx * dx + dy * dy));
 #pragma unroll
       for (int q = 0; q < SZC / 2; ++q) {
         const float4 v = row[q];
@@ -1462,11 +1466,11 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   if (p->flags & MM_RADON_FP64_TILES)
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
-                       p->a, xg);
+                       (float)(-(double)p->a * M_LOG2E), xg);
   else
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
-                       p->a, xg);
+                       (float)(-(double)p->a * M_LOG2E), xg);
   int rc = check_launch("scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
