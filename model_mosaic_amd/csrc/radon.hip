@@ -658,7 +658,6 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   for (int jp = 0; jp < SL_PAIRS_B; ++jp) lb[jp] = jp + SL_PAIRS_A < npairs ? lp[(jp + SL_PAIRS_A) * SNP] : 0u;
   __syncthreads();
   const float sgn = ((l0 + l1) & 1) ? -1.f : 1.f;
-  const int l1w = l1 < 0 ? l1 + 2 * n : l1;  // raw fftfreq row
   for (int zc = z0; zc < z1; zc += SZC) {
     const int nzz = min(SZC, z1 - zc);
     for (int e = p; e < it.ne; e += SNP) {
@@ -711,13 +710,34 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       tap(u & 0xFFFFu);
       tap(u >> 16);
     }
-    if (l0 <= n) {
+    // Results leave through LDS: lane p owns point pt (lanes are ordered by list length, so a
+    // wave's points are scattered over the tile); staged in the value slots of the entry rows
+    // (index zz * SNP + point -> row / SZC, slice % SZC), then written in tile order -- a wave
+    // covers 16 rows x 4 columns of one 4-column block, i.e. one contiguous 512-byte run of the
+    // half plane per slice (partial tiles: 64 consecutive points).
+    __syncthreads();  // every lane is done reading the entry values
+    static_assert(SZC * SNP <= SE_MAX * SZC, "result staging fits the value slots");
+#pragma unroll
+    for (int zz = 0; zz < SZC; ++zz) {
+      const int i = zz * SNP + pt;
+      R[i / SZC].v[i % SZC] = make_float2(sgn * (float)ax[zz], sgn * (float)ay[zz]);
+    }
+    __syncthreads();
+    if (it.slot < 0) {
+      const int blk = p >> 6, r = (p >> 2) & 15, c4 = p & 3;
+      const int q = r * STS + 4 * blk + c4;  // tile point this lane writes
+      const int col = it.x0 + 4 * blk + c4, row = it.y0 + r;
+      if (col <= n) {
+        const long long o = hs_idx(row < 0 ? row + 2 * n : row, col, 2 * n);
+        for (int zz = 0; zz < nzz; ++zz) {
+          const int i = zz * SNP + q;
+          Hb[(long long)(zc + zz) * hzs + o] = R[i / SZC].v[i % SZC];
+        }
+      }
+    } else {
       for (int zz = 0; zz < nzz; ++zz) {
-        const float2 val = make_float2(sgn * (float)ax[zz], sgn * (float)ay[zz]);
-        if (it.slot < 0)
-          Hb[(long long)(zc + zz) * hzs + hs_idx(l1w, l0, 2 * n)] = val;
-        else
-          partial[((long long)it.slot * nz + zc + zz) * SNP + pt] = val;
+        const int i = zz * SNP + p;
+        partial[((long long)it.slot * nz + zc + zz) * SNP + p] = R[i / SZC].v[i % SZC];
       }
     }
     __syncthreads();
