@@ -63,11 +63,33 @@ constexpr int SMAX = TILE_THREADS;  // samples per tile work item
 // per cycle), which the LDS serves 16 lanes per cycle: the plan gives the 16
 // samples of each quarter-wave distinct (base / 2) mod 16, so the pieces of
 // one read never share a bank (64 banks = 16 groups of 16 bytes).
-__host__ __device__ __forceinline__ int patch_pitch(int w) {
+__host__ __device__ __forceinline__ constexpr int patch_pitch(int w) {
   return w >= 64 ? 76 : (w >= 32 ? 44 : (w >= 16 ? 28 : 20));
 }
-constexpr int PATCH_ELEMS = (ROOT + 2 * M) * 76;  // even: the 16-byte reads stay inside
-constexpr int STG_MAX = (((ROOT + 2 * M + 3) / 4 + 1) * 4 * (ROOT + 2 * M) + TILE_THREADS - 1) / TILE_THREADS;
+
+// Gather kernel shapes.  Root class (tiles wider than 32): 1024 lanes, one
+// sample each; its float4 patch (72 x 76) takes 87.5 KB of LDS, so one
+// workgroup holds a CU and the next round's staging loads overlap only its own
+// taps.  Leaf class (tiles of width <= 32): 512 lanes with two samples each
+// and a 40 x 44 patch (~44 KB of LDS with the output buffer), so two
+// workgroups share a CU and one's loads overlap the other's taps.
+//   PE: patch capacity (the sink slot for idle staging lanes is PE itself);
+//   STG: staging slots per lane (4-column blocks x rows over NT lanes).
+template <int WMAX, int NT_>
+struct GatherShape {
+  static constexpr int NT = NT_, W = WMAX;
+  static constexpr int SPT = SMAX / NT;
+  static constexpr int PE = (WMAX + 2 * M) * patch_pitch(WMAX);  // even: the 16-byte reads stay inside
+  static constexpr int STG = (((WMAX + 2 * M + 3) / 4 + 1) * 4 * (WMAX + 2 * M) + NT - 1) / NT;
+  static_assert(SMAX % NT == 0, "samples per lane");
+  // waves per SIMD the register budget is cut for: root ZP = 2 106 VGPRs at 4
+  // (one workgroup per CU), root ZP = 1 64 VGPRs at 8; leaf at 4 (<= 128
+  // VGPRs, two workgroups of 8 waves per CU; 6 would spill)
+  template <int ZP>
+  static constexpr int wpe() { return NT == TILE_THREADS ? (ZP == 1 ? 8 : 4) : 4; }
+};
+using GatherRoot = GatherShape<ROOT, TILE_THREADS>;
+using GatherLeaf = GatherShape<ROOT / 2, TILE_THREADS / 2>;
 
 struct Item {
   int x0, y0;        // tile origin (l0, l1), centred coordinates
@@ -249,7 +271,7 @@ __device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* 
 // Either way the 16 pieces of one read hit 16 distinct bank groups.
 // Staging descriptor idx of a gather item: the half-spectrum offset (within a
 // z plane) of patch element idx and its LDS slot << 1 | conj flag; slots
-// outside the patch read offset 0 into the sink slot PATCH_ELEMS.  Patch
+// outside the patch read offset 0 into the sink slot pe (the patch capacity).  Patch
 // columns and rows are raw fftfreq indices taken mod 2n (tiles near the top
 // and right edges reach past 2n for small n); columns k0 > n are the
 // conjugate of the Hermitian mirror (2n - k1, 2n - k0).
@@ -257,7 +279,8 @@ __host__ __device__ __forceinline__ int wrap_mod(int v, int m) {
   const int r = v % m;
   return r < 0 ? r + m : r;
 }
-__host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int idx, int n, int& soff, int& sdst) {
+__host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int idx, int n, int pe, int& soff,
+                                                           int& sdst) {
   const int pw = it.w + 2 * M, ph = it.h + 2 * M;
   const int pp = patch_pitch(it.w);
   const int px0 = it.x0 - M, py0 = it.y0 - M;
@@ -265,7 +288,7 @@ __host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int i
   const int nstage = (((px0 + pw - 1) - bx0) / 4 + 1) * ph * 4;
   const int twon = 2 * n;
   soff = 0;
-  sdst = PATCH_ELEMS << 1;
+  sdst = pe << 1;
   if (idx >= nstage) return;
   const int q = idx & 3, rest = idx >> 2;
   const int pr = rest % ph, blk = rest / ph;
@@ -283,74 +306,77 @@ __host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int i
   sdst = ((pr * pp + pc) << 1) | fl;
 }
 
-template <int ZP>
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(ZP == 1 ? 8 : 4, ZP == 1 ? 8 : 4)))
+template <int ZP, class S>
+__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(S::template wpe<ZP>(), S::template wpe<ZP>())))
 void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const Item* __restrict__ items,
                   const int* __restrict__ samples, const unsigned short* __restrict__ order,
                   const double2* __restrict__ dirs, int n, int H, int nz, long long fzs, int zchunk, float a,
                   float C0, int xg) {
   using PT = typename std::conditional<ZP == 1, float2, float4>::type;
-  __shared__ __align__(16) PT P[PATCH_ELEMS + 2];  // + a sink for idle staging slots
-  __shared__ PT O[TILE_THREADS];
+  constexpr int NT = S::NT, SPT = S::SPT, STG = S::STG, PE = S::PE;
+  __shared__ __align__(16) PT P[PE + 2];  // + a sink for idle staging slots
+  __shared__ PT O[SMAX];
   const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
   const Item it = items[lw % gridDim.x];
   const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
   const int pp = patch_pitch(it.w);  // LDS row pitch >= w + 2M
-  const int s = threadIdx.x;
-  const bool active = s < it.count;
   // staging descriptors: source offset in the z plane; LDS slot << 1 | conj.
   // Idle slots read offset 0 and write the sink, so the loads issue together.
-  int soff[STG_MAX], sdst[STG_MAX];
+  int soff[STG], sdst[STG];
 #pragma unroll
-  for (int e = 0; e < STG_MAX; ++e) gather_stage_desc(it, threadIdx.x + e * TILE_THREADS, n, soff[e], sdst[e]);
+  for (int e = 0; e < STG; ++e) gather_stage_desc(it, threadIdx.x + e * NT, n, PE, soff[e], sdst[e]);
+  // Lane threadIdx.x computes samples s_j = threadIdx.x + j NT (j < SPT).
   // ZP = 1: ex[j] weighs patch column (base & ~1) + j, the 9 taps shifted by
   // the parity of base (ex[9] = 0 or ex[0] = 0).  ZP = 2: ex[j] weighs base + j.
   constexpr int NX = ZP == 1 ? TAPS + 1 : TAPS;
-  float ex[NX], ey[TAPS];
-  int base = 0;
-  if (active) {
+  float ex[SPT][NX], ey[SPT][TAPS];
+  int base[SPT];
+  // output of sorted sample t_j = threadIdx.x + j NT
+  int src_lane[SPT];
+  long long out0[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    const int s = threadIdx.x + j * NT;
+    base[j] = 0;
+    src_lane[j] = 0;
+    out0[j] = 0;
+    if (s >= it.count) continue;
     const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
     const int b = (g.ry - it.y0) * pp + (g.rx - it.x0);
     const int par = ZP == 1 ? (b & 1) : 0;
-    base = b - par;
+    base[j] = b - par;
     // centring sign of tap (i0, i1): (-1)^(rx - M + i0) (-1)^(ry - M + i1)
     const float sx = ((g.rx - M) & 1) ? -1.f : 1.f, sy = ((g.ry - M) & 1) ? -C0 : C0;
 #pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      const int i = j - par;
+    for (int q = 0; q < NX; ++q) {
+      const int i = q - par;
       const float dx = (float)(i - M) - g.fx;
       const float sg = (i & 1) ? -sx : sx;
-      ex[j] = (i >= 0 && i < TAPS) ? sg * __expf(-a * dx * dx) : 0.f;
+      ex[j][q] = (i >= 0 && i < TAPS) ? sg * __expf(-a * dx * dx) : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < TAPS; ++i) {
       const float dy = (float)(i - M) - g.fy;
-      ey[i] = ((i & 1) ? -sy : sy) * __expf(-a * dy * dy);
+      ey[j][i] = ((i & 1) ? -sy : sy) * __expf(-a * dy * dy);
     }
-  }
-  // output of sorted sample t = threadIdx.x
-  const bool writer = threadIdx.x < it.count;
-  int src_lane = 0;
-  long long out0 = 0;
-  if (writer) {
-    src_lane = order[it.start + threadIdx.x];
-    const int id = samples[it.start + src_lane];
+    src_lane[j] = order[it.start + s];
+    const int id = samples[it.start + src_lane[j]];
     const int th = id / H;
-    out0 = (long long)th * nz * H + (id - th * H);
+    out0[j] = (long long)th * nz * H + (id - th * H);
   }
-  float2 sa[STG_MAX], sb[STG_MAX];
+  float2 sa[STG], sb[STG];
   auto fetch = [&](int z) {
     const float2* f0 = fde + (long long)z * fzs;
 #pragma unroll
-    for (int e = 0; e < STG_MAX; ++e) sa[e] = f0[soff[e]];
+    for (int e = 0; e < STG; ++e) sa[e] = f0[soff[e]];
     if constexpr (ZP == 2) {
       const bool two = z + 1 < z1;
       const float2* f1 = two ? f0 + fzs : f0;
 #pragma unroll
-      for (int e = 0; e < STG_MAX; ++e) sb[e] = f1[soff[e]];
+      for (int e = 0; e < STG; ++e) sb[e] = f1[soff[e]];
       if (!two)
 #pragma unroll
-        for (int e = 0; e < STG_MAX; ++e) sb[e] = make_float2(0.f, 0.f);
+        for (int e = 0; e < STG; ++e) sb[e] = make_float2(0.f, 0.f);
     }
   };
   // ZP = 2 prefetches the next round during the taps; ZP = 1 (64 VGPRs at two
@@ -359,7 +385,7 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
   for (int z = z0; z < z1; z += ZP) {
     if constexpr (ZP == 1) fetch(z);
 #pragma unroll
-    for (int e = 0; e < STG_MAX; ++e) {
+    for (int e = 0; e < STG; ++e) {
       float2 u = sa[e];
       if (sdst[e] & 1) u.y = -u.y;
       if constexpr (ZP == 1) {
@@ -372,55 +398,59 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
     }
     __syncthreads();
     if (ZP == 2 && z + ZP < z1) fetch(z + ZP);
-    if (active) {
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      if (threadIdx.x + j * NT >= it.count) continue;
       if constexpr (ZP == 1) {
         float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
         for (int i1 = 0; i1 < TAPS; ++i1) {
-          const float4* prow = reinterpret_cast<const float4*>(P + base + i1 * pp);  // base, pp even
+          const float4* prow = reinterpret_cast<const float4*>(P + base[j] + i1 * pp);  // base, pp even
           float2 row = make_float2(0.f, 0.f);
 #pragma unroll
           for (int c = 0; c < (TAPS + 1) / 2; ++c) {
             const float4 v = prow[c];
-            row.x = fmaf(ex[2 * c], v.x, row.x);
-            row.y = fmaf(ex[2 * c], v.y, row.y);
-            row.x = fmaf(ex[2 * c + 1], v.z, row.x);
-            row.y = fmaf(ex[2 * c + 1], v.w, row.y);
+            row.x = fmaf(ex[j][2 * c], v.x, row.x);
+            row.y = fmaf(ex[j][2 * c], v.y, row.y);
+            row.x = fmaf(ex[j][2 * c + 1], v.z, row.x);
+            row.y = fmaf(ex[j][2 * c + 1], v.w, row.y);
           }
-          acc.x = fmaf(ey[i1], row.x, acc.x);
-          acc.y = fmaf(ey[i1], row.y, acc.y);
+          acc.x = fmaf(ey[j][i1], row.x, acc.x);
+          acc.y = fmaf(ey[j][i1], row.y, acc.y);
         }
-        O[s] = acc;
+        O[threadIdx.x + j * NT] = acc;
       } else {
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int i1 = 0; i1 < TAPS; ++i1) {
-          const float4* prow = P + base + i1 * pp;
+          const float4* prow = P + base[j] + i1 * pp;
           float4 row = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
           for (int i0 = 0; i0 < TAPS; ++i0) {
             const float4 v = prow[i0];
-            row.x = fmaf(ex[i0], v.x, row.x);
-            row.y = fmaf(ex[i0], v.y, row.y);
-            row.z = fmaf(ex[i0], v.z, row.z);
-            row.w = fmaf(ex[i0], v.w, row.w);
+            row.x = fmaf(ex[j][i0], v.x, row.x);
+            row.y = fmaf(ex[j][i0], v.y, row.y);
+            row.z = fmaf(ex[j][i0], v.z, row.z);
+            row.w = fmaf(ex[j][i0], v.w, row.w);
           }
-          acc.x = fmaf(ey[i1], row.x, acc.x);
-          acc.y = fmaf(ey[i1], row.y, acc.y);
-          acc.z = fmaf(ey[i1], row.z, acc.z);
-          acc.w = fmaf(ey[i1], row.w, acc.w);
+          acc.x = fmaf(ey[j][i1], row.x, acc.x);
+          acc.y = fmaf(ey[j][i1], row.y, acc.y);
+          acc.z = fmaf(ey[j][i1], row.z, acc.z);
+          acc.w = fmaf(ey[j][i1], row.w, acc.w);
         }
-        O[s] = acc;
+        O[threadIdx.x + j * NT] = acc;
       }
     }
     __syncthreads();
-    if (writer) {
-      const PT o = O[src_lane];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      if (threadIdx.x + j * NT >= it.count) continue;
+      const PT o = O[src_lane[j]];
       if constexpr (ZP == 1) {
-        gh[out0 + (long long)z * H] = o;
+        gh[out0[j] + (long long)z * H] = o;
       } else {
-        gh[out0 + (long long)z * H] = make_float2(o.x, o.y);
-        if (z + 1 < z1) gh[out0 + (long long)(z + 1) * H] = make_float2(o.z, o.w);
+        gh[out0[j] + (long long)z * H] = make_float2(o.x, o.y);
+        if (z + 1 < z1) gh[out0[j] + (long long)(z + 1) * H] = make_float2(o.z, o.w);
       }
     }
   }
@@ -946,7 +976,8 @@ struct HostGeom {
 
 struct Binning {
   int zp = 2;                          // z slices per patch element of the gather kernel
-  std::vector<Item> items;
+  std::vector<Item> items;             // root class (w > GatherLeaf::W) first, then the leaf class
+  int n_root = 0;
   std::vector<int> samples;            // per item, in lane order
   std::vector<unsigned short> order;   // per item: lane of the t-th sample in ascending id order
 };
@@ -1042,6 +1073,10 @@ static void build_gather(const HostGeom& G, int root, Binning& B) {
   }
   for (int ty = 0; ty < nry; ++ty)
     for (int tx = 0; tx < nrx; ++tx) gather_node(G, tx * root, ty * root - n, root, roots[(size_t)ty * nrx + tx], B);
+  // neighbouring items stay neighbours inside each class (XCD grouping)
+  const auto mid = std::stable_partition(B.items.begin(), B.items.end(),
+                                         [](const Item& it) { return it.w > GatherLeaf::W; });
+  B.n_root = (int)(mid - B.items.begin());
 }
 
 // ---- scatter lists (host, at plan time) -------------------------------------
@@ -1344,6 +1379,7 @@ struct mm_radon {
   int* gsamples = nullptr;
   unsigned short* gorder = nullptr;
   int n_gitems = 0;
+  int n_groot = 0;  // gather items of the root class (the first n_groot of gitems)
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
   int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   SItem* sitems = nullptr;
@@ -1462,18 +1498,27 @@ static int zsplit_for(int items, int nz) {
   return std::max(1, std::min(zs, nz));
 }
 
-static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
-  if (p->n_gitems == 0) return MM_OK;
-  const int zsplit = zsplit_for(p->n_gitems, nz);
+template <class S>
+static void launch_gather_class(const mm_radon* p, const Item* items, int count, int nz, hipStream_t st) {
+  const int zsplit = zsplit_for(count, nz);
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
-  const dim3 grid((unsigned)p->n_gitems, (unsigned)zblocks);
+  const dim3 grid((unsigned)count, (unsigned)zblocks);
+  const int xg = p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP];
   if (p->gzp == 1)
-    hipLaunchKernelGGL(gather_tiles<1>, grid, dim3(TILE_THREADS), 0, st, p->fde, p->gh, p->gitems, p->gsamples,
-                       p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP]);
+    hipLaunchKernelGGL((gather_tiles<1, S>), grid, dim3(S::NT), 0, st, p->fde, p->gh, items, p->gsamples, p->gorder,
+                       p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, xg);
   else
-    hipLaunchKernelGGL(gather_tiles<2>, grid, dim3(TILE_THREADS), 0, st, p->fde, p->gh, p->gitems, p->gsamples,
-                       p->gorder, p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP]);
+    hipLaunchKernelGGL((gather_tiles<2, S>), grid, dim3(S::NT), 0, st, p->fde, p->gh, items, p->gsamples, p->gorder,
+                       p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, xg);
+}
+
+// Root-class items (tiles wider than 32) come first in the item table, then
+// the leaf class; each class is one launch.
+static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
+  if (p->n_groot > 0) launch_gather_class<GatherRoot>(p, p->gitems, p->n_groot, nz, st);
+  if (p->n_gitems > p->n_groot)
+    launch_gather_class<GatherLeaf>(p, p->gitems + p->n_groot, p->n_gitems - p->n_groot, nz, st);
   return check_launch("gather_tiles");
 }
 
@@ -1604,6 +1649,7 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
       (rc = upload(&p->splits, BS.splits, bytes)))
     return bail(rc);
   p->n_gitems = (int)BG.items.size();
+  p->n_groot = BG.n_root;
   p->n_sitems = (int)BS.items.size();
   p->n_splits = (int)BS.splits.size();
   p->n_slots = BS.nslots;
@@ -1791,14 +1837,20 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
     Binning B;
     B.zp = zp;
     build_gather(G, std::min(ROOT, n), B);
-    for (const Item& it : B.items) {
+    for (size_t i = 0; i < B.items.size(); ++i) {
+      const Item& it = B.items[i];
+      const bool root_class = (int)i < B.n_root;
+      if (root_class != (it.w > GatherLeaf::W)) return bad("item class order");
+      const int PE = root_class ? GatherRoot::PE : GatherLeaf::PE;
+      const int nslots = root_class ? GatherRoot::STG * GatherRoot::NT : GatherLeaf::STG * GatherLeaf::NT;
+      if (it.count > SMAX) return bad("samples per item");
       const int pw = it.w + 2 * M, ph = it.h + 2 * M, pp = patch_pitch(it.w);
-      if (pp < pw || ph * pp > PATCH_ELEMS) return bad("patch pitch");
-      std::vector<int> cover(PATCH_ELEMS + 1, 0);
-      for (int idx = 0; idx < STG_MAX * TILE_THREADS; ++idx) {
+      if (pp < pw || ph * pp > PE) return bad("patch pitch");
+      std::vector<int> cover(PE + 1, 0);
+      for (int idx = 0; idx < nslots; ++idx) {
         int so, sd;
-        gather_stage_desc(it, idx, n, so, sd);
-        if (so < 0 || so >= zs || (sd >> 1) > PATCH_ELEMS)
+        gather_stage_desc(it, idx, n, PE, so, sd);
+        if (so < 0 || so >= zs || (sd >> 1) > PE)
           return bad("staging offset " + std::to_string(so) + " / slot " + std::to_string(sd >> 1) + " out of range");
         ++cover[sd >> 1];
       }
@@ -1811,7 +1863,7 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
         if (rx < it.x0 || rx >= it.x0 + it.w || ry < it.y0 || ry >= it.y0 + it.h) return bad("sample outside its tile");
         const int b = (ry - it.y0) * pp + (rx - it.x0);
         const int last = zp == 1 ? (b & ~1) + 8 * pp + 9 : b + 8 * pp + 8;
-        if (last >= PATCH_ELEMS + (zp == 1 ? 2 : 0)) return bad("tap read past the patch");
+        if (last >= PE + (zp == 1 ? 2 : 0)) return bad("tap read past the patch");
         if (B.order[it.start + k] >= it.count) return bad("output order");
       }
     }
