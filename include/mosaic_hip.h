@@ -51,8 +51,9 @@ int mm_version(void);
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of
  *     MM_RADON_FP64_TILES  scatter tiles accumulate in FP64;
- *     MM_RADON_GATHER_ZP1  gather stages one z slice per round (default: two
- *                          slices interleaved per patch element).
+ *     MM_RADON_GATHER_ZP1  gather tiles of width <= 32 stage one z slice per
+ *                          round (default: two slices interleaved per patch
+ *                          element; wider tiles always stage one).
  */
 #define MM_RADON_FP64_TILES 1u
 #define MM_RADON_GATHER_ZP1 2u

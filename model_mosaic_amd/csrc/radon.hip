@@ -52,8 +52,8 @@ constexpr int M = 4;             // window half width (m of processing/tomo.py:4
 constexpr int TAPS = 2 * M + 1;  // 9
 constexpr int ROOT = 64;         // quadtree root tile
 constexpr int FLOOR = 8;         // quadtree floor tile
-constexpr int TILE_THREADS = 1024;
-constexpr int SMAX = TILE_THREADS;  // samples per tile work item
+
+constexpr int SMAX = 1024;  // samples per gather work item
 
 // Gather patch pitch (complex values per LDS row) for tile widths 64/32/16/8:
 // the smallest pitch >= w + 2M that is 4 (mod 8) with pitch / 4 odd, so the
@@ -67,12 +67,12 @@ __host__ __device__ __forceinline__ constexpr int patch_pitch(int w) {
   return w >= 64 ? 76 : (w >= 32 ? 44 : (w >= 16 ? 28 : 20));
 }
 
-// Gather kernel shapes.  Root class (tiles wider than 32): 1024 lanes, one
-// sample each; its float4 patch (72 x 76) takes 87.5 KB of LDS, so one
-// workgroup holds a CU and the next round's staging loads overlap only its own
-// taps.  Leaf class (tiles of width <= 32): 512 lanes with two samples each
-// and a 40 x 44 patch (~44 KB of LDS with the output buffer), so two
-// workgroups share a CU and one's loads overlap the other's taps.
+// Gather kernel shapes, both 512 lanes with two samples each and at most 128
+// VGPRs, so two workgroups (16 waves) share a CU and one's staging loads
+// overlap the other's taps.  Root class (tiles wider than 32): float2 patch of
+// one z slice (72 x 76, 43 KB: a float4 patch of two slices would take 87.5 KB
+// and hold the CU alone).  Leaf class (tiles of width <= 32): float4 patch of
+// two slices (40 x 44, 28 KB) unless the plan asks for one slice.
 //   PE: patch capacity (the sink slot for idle staging lanes is PE itself);
 //   STG: staging slots per lane (4-column blocks x rows over NT lanes).
 template <int WMAX, int NT_>
@@ -82,14 +82,14 @@ struct GatherShape {
   static constexpr int PE = (WMAX + 2 * M) * patch_pitch(WMAX);  // even: the 16-byte reads stay inside
   static constexpr int STG = (((WMAX + 2 * M + 3) / 4 + 1) * 4 * (WMAX + 2 * M) + NT - 1) / NT;
   static_assert(SMAX % NT == 0, "samples per lane");
-  // waves per SIMD the register budget is cut for: root ZP = 2 106 VGPRs at 4
-  // (one workgroup per CU), root ZP = 1 64 VGPRs at 8; leaf at 4 (<= 128
-  // VGPRs, two workgroups of 8 waves per CU; 6 would spill)
-  template <int ZP>
-  static constexpr int wpe() { return NT == TILE_THREADS ? (ZP == 1 ? 8 : 4) : 4; }
 };
-using GatherRoot = GatherShape<ROOT, TILE_THREADS>;
-using GatherLeaf = GatherShape<ROOT / 2, TILE_THREADS / 2>;
+using GatherRoot = GatherShape<ROOT, SMAX / 2>;
+using GatherLeaf = GatherShape<ROOT / 2, SMAX / 2>;
+// z slices per patch element of a gather item of width w (plan flag zp for the
+// leaf class); the lane colouring of the plan follows it
+__host__ __device__ __forceinline__ constexpr int gather_zp(int w, int zp_flag) {
+  return w > GatherLeaf::W ? 1 : zp_flag;
+}
 
 struct Item {
   int x0, y0;        // tile origin (l0, l1), centred coordinates
@@ -307,7 +307,7 @@ __host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int i
 }
 
 template <int ZP, class S>
-__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(S::template wpe<ZP>(), S::template wpe<ZP>())))
+__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const Item* __restrict__ items,
                   const int* __restrict__ samples, const unsigned short* __restrict__ order,
                   const double2* __restrict__ dirs, int n, int H, int nz, long long fzs, int zchunk, float a,
@@ -379,8 +379,9 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
         for (int e = 0; e < STG; ++e) sb[e] = make_float2(0.f, 0.f);
     }
   };
-  // ZP = 2 prefetches the next round during the taps; ZP = 1 (64 VGPRs at two
-  // workgroups per CU) has no registers to spare and fetches at the top.
+  // ZP = 2 prefetches the next round during the taps; ZP = 1 (11 staging slots
+  // per lane in the root class) has no registers to spare and fetches at the
+  // top.
   if (ZP == 2 && z0 < z1) fetch(z0);
   for (int z = z0; z < z1; z += ZP) {
     if constexpr (ZP == 1) fetch(z);
@@ -989,7 +990,8 @@ struct Binning {
 // colours first, so the tap reads are conflict-free wherever the colours allow.
 static void gather_lanes(const HostGeom& G, int x0, int y0, int w, const int* ids, int count, Binning& B) {
   const int pp = patch_pitch(w);
-  const int shift = B.zp == 1 ? 1 : 0;  // 16-byte pieces: two float2 (zp = 1) or one float4 (zp = 2) patch elements
+  // 16-byte pieces: two float2 (zp = 1) or one float4 (zp = 2) patch elements
+  const int shift = gather_zp(w, B.zp) == 1 ? 1 : 0;
   std::vector<int> bucket[16];
   for (int t = 0; t < count; ++t) {
     int rx, ry;
@@ -1505,12 +1507,16 @@ static void launch_gather_class(const mm_radon* p, const Item* items, int count,
   const int zblocks = (nz + zchunk - 1) / zchunk;
   const dim3 grid((unsigned)count, (unsigned)zblocks);
   const int xg = p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP];
-  if (p->gzp == 1)
-    hipLaunchKernelGGL((gather_tiles<1, S>), grid, dim3(S::NT), 0, st, p->fde, p->gh, items, p->gsamples, p->gorder,
-                       p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, xg);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(S::NT), 0, st, p->fde, p->gh, items, p->gsamples, p->gorder, p->dirs, p->n,
+                       p->H, nz, p->zs, zchunk, p->a, p->C0g, xg);
+  };
+  if constexpr (S::W > GatherLeaf::W)
+    go(gather_tiles<1, S>);
+  else if (p->gzp == 1)
+    go(gather_tiles<1, S>);
   else
-    hipLaunchKernelGGL((gather_tiles<2, S>), grid, dim3(S::NT), 0, st, p->fde, p->gh, items, p->gsamples, p->gorder,
-                       p->dirs, p->n, p->H, nz, p->zs, zchunk, p->a, p->C0g, xg);
+    go(gather_tiles<2, S>);
 }
 
 // Root-class items (tiles wider than 32) come first in the item table, then
@@ -1862,8 +1868,9 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
         G.centre(B.samples[it.start + k], rx, ry);
         if (rx < it.x0 || rx >= it.x0 + it.w || ry < it.y0 || ry >= it.y0 + it.h) return bad("sample outside its tile");
         const int b = (ry - it.y0) * pp + (rx - it.x0);
-        const int last = zp == 1 ? (b & ~1) + 8 * pp + 9 : b + 8 * pp + 8;
-        if (last >= PE + (zp == 1 ? 2 : 0)) return bad("tap read past the patch");
+        const int kzp = gather_zp(it.w, zp);
+        const int last = kzp == 1 ? (b & ~1) + 8 * pp + 9 : b + 8 * pp + 8;
+        if (last >= PE + (kzp == 1 ? 2 : 0)) return bad("tap read past the patch");
         if (B.order[it.start + k] >= it.count) return bad("output order");
       }
     }
