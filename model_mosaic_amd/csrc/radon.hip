@@ -37,6 +37,7 @@
 //  * FFTs are the LDS Stockham transforms of mm_fft.h: row transforms over
 //    contiguous rows, column transforms over C = 16384/L adjacent columns.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -189,7 +190,12 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
 
 // Column transform over C adjacent columns of one z plane of the half
 // spectrum (blockIdx.y = z).  Rows [in_lo, in_hi) are read (others are zero),
-// rows [out_lo, out_hi) are written back in place, times `scale`.  The
+// rows [out_lo, out_hi) are written back in place, times `scale`.  supp holds
+// per 4-column block the centred row range [lo, hi) (row i is l = i for i < n,
+// i - 2n otherwise) that carries spectrum data: the forward pass writes only
+// those rows (the gather reads nothing else), the inverse pass reads only
+// those (the scatter writes nothing else; the disc of radius n leaves the
+// corners of the plane empty).  The
 // register FFT's points t + m TPC of the 4 columns of a block are contiguous
 // (16 rows x 4 columns per wave and m), so loads and stores go straight
 // between HBM and registers.
@@ -207,7 +213,8 @@ using ColShapeB = RegShape<L, BATCH>;
 template <int L, bool INV, int BATCH = kBatchElems>
 __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __restrict__ buf, int ncols,
                                                                    long long zs, int in_lo, int in_hi,
-                                                                   int out_lo, int out_hi, float scale,
+                                                                   int out_lo, int out_hi,
+                                                                   const int2* __restrict__ supp, float scale,
                                                                    const float2* __restrict__ tw, int xg) {
   using Sh = ColShapeB<L, BATCH>;
   __shared__ float2 lds[Sh::LDS];
@@ -217,17 +224,23 @@ __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __r
   const int col = (int)(lw % gridDim.x) * Sh::C + c;
   const bool live = col < ncols;
   float2* b = buf + (long long)(lw / gridDim.x) * zs;
+  const int2 sp = live ? supp[col >> 2] : make_int2(0, 0);
+  auto carries = [&](int i) {
+    const int l = i < L / 2 ? i : i - L;
+    return l >= sp.x && l < sp.y;
+  };
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
-    v[m] = (live && i >= in_lo && i < in_hi) ? b[hs_idx(i, col, L)] : make_float2(0.f, 0.f);
+    const bool rd = live && i >= in_lo && i < in_hi && (!INV || carries(i));
+    v[m] = rd ? b[hs_idx(i, col, L)] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, INV>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
-    if (live && i >= out_lo && i < out_hi) b[hs_idx(i, col, L)] = cscale(v[m], scale);
+    if (live && i >= out_lo && i < out_hi && (INV || carries(i))) b[hs_idx(i, col, L)] = cscale(v[m], scale);
   }
 }
 
@@ -1343,6 +1356,79 @@ static void build_scatter_lists(const HostGeom& G, SBinning& B) {
   }
 }
 
+// ---- spectrum support (host, at plan time) ----------------------------------
+// Per 4-column block of the half spectrum, the centred row range [lo, hi) that
+// carries data between the column pass and the polar pass (hull of the rows
+// touched in the block's 4 columns; empty: lo = hi).
+
+static void support_add(std::vector<int2>& s, int col, int lo, int hi) {
+  int2& r = s[col >> 2];
+  if (r.x == r.y) {
+    r = make_int2(lo, hi);
+  } else {
+    r.x = std::min(r.x, lo);
+    r.y = std::max(r.y, hi);
+  }
+}
+
+// Rows the gather patches read: patch rows [y0 - M, y0 + h + M) in columns
+// [x0 - M, x0 + w + M); a column k0 > n is read as row -l of column 2n - k0.
+// A patch reaching past row -n or n (small n: rows wrap mod 2n) takes every row.
+static std::vector<int2> gather_support(const Binning& B, int n) {
+  const int twon = 2 * n;
+  std::vector<int2> s((size_t)(n / 4 + 1), make_int2(0, 0));
+  for (const Item& it : B.items) {
+    int lo = it.y0 - M, hi = it.y0 + it.h + M;
+    const bool all = lo <= -n || hi > n;
+    if (all) {
+      lo = -n;
+      hi = n;
+    }
+    for (int c = it.x0 - M; c < it.x0 + it.w + M; ++c) {
+      const int k0 = wrap_mod(c, twon);
+      if (k0 <= n)
+        support_add(s, k0, lo, hi);
+      else if (all)
+        support_add(s, twon - k0, -n, n);
+      else
+        support_add(s, twon - k0, 1 - hi, 1 - lo);
+    }
+  }
+  return s;
+}
+
+// Rows the scatter writes.  Per tile column, the tiles between the lowest and
+// the highest tile that has entries stay work items (empty ones write zeros);
+// empty tiles outside that range are dropped, and the column pass reads the
+// range only.  Returns the support; compacts B.items and B.perm.
+static std::vector<int2> trim_scatter(SBinning& B, int n) {
+  const int ntx = n / STS + 1;
+  std::vector<int> tmin(ntx, INT_MAX), tmax(ntx, INT_MIN);
+  for (const SItem& it : B.items)
+    if (it.ne > 0) {
+      const int tx = it.x0 / STS, ty = (it.y0 + n) / STS;
+      tmin[tx] = std::min(tmin[tx], ty);
+      tmax[tx] = std::max(tmax[tx], ty);
+    }
+  std::vector<SItem> items;
+  std::vector<unsigned char> perm;
+  for (size_t i = 0; i < B.items.size(); ++i) {
+    const SItem& it = B.items[i];
+    const int tx = it.x0 / STS, ty = (it.y0 + n) / STS;
+    if (it.ne == 0 && (ty < tmin[tx] || ty > tmax[tx])) continue;
+    items.push_back(it);
+    perm.insert(perm.end(), B.perm.begin() + (long long)i * SNP, B.perm.begin() + (long long)(i + 1) * SNP);
+  }
+  B.items.swap(items);
+  B.perm.swap(perm);
+  std::vector<int2> s((size_t)(n / 4 + 1), make_int2(0, 0));
+  for (int tx = 0; tx < ntx; ++tx)
+    if (tmin[tx] <= tmax[tx])
+      for (int c = tx * STS; c < std::min(tx * STS + STS, n + 1); c += 4)
+        support_add(s, c, tmin[tx] * STS - n, (tmax[tx] + 1) * STS - n);
+  return s;
+}
+
 template <typename T>
 static int upload(T** dst, const std::vector<T>& src, size_t& bytes) {
   *dst = nullptr;
@@ -1382,6 +1468,8 @@ struct mm_radon {
   unsigned short* gorder = nullptr;
   int n_gitems = 0;
   int n_groot = 0;  // gather items of the root class (the first n_groot of gitems)
+  int2* gsupp = nullptr;  // per 4-column block: centred rows the gather reads
+  int2* ssupp = nullptr;  // per 4-column block: centred rows the scatter writes
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
   int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   SItem* sitems = nullptr;
@@ -1416,7 +1504,7 @@ static void radon_free(mm_radon* p) {
   if (!p->own_ws) p->fde = p->gh = nullptr;
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
                   p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
-                  p->splits};
+                  p->splits, p->gsupp, p->ssupp};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1446,15 +1534,17 @@ template <int L, bool INV>
 static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in_hi, int out_lo, int out_hi,
                        float scale, hipStream_t st) {
   const int ncols = p->n + 1;
+  const int2* supp = INV ? p->ssupp : p->gsupp;
   if (p->tune[MM_RADON_TUNE_COL_SPLIT] && ColShapeB<L, kRowBatch>::C == 2) {
     using Sh = ColShapeB<L, kRowBatch>;
     hipLaunchKernelGGL((col_fft<L, INV, kRowBatch>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz),
-                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale,
+                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, supp, scale,
                        p->tw + tw_offset(L), 2);
   } else {
     using Sh = ColShape<L>;
     hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                       0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, scale, p->tw + tw_offset(L), 1);
+                       0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, supp, scale, p->tw + tw_offset(L),
+                       1);
   }
 }
 
@@ -1644,7 +1734,12 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   const int root = std::min(ROOT, n);
   build_gather(G, root, BG);
   p->fwd_only = fwd_only;
-  if (!fwd_only) build_scatter_lists(G, BS);
+  const std::vector<int2> gsupp = gather_support(BG, n);
+  std::vector<int2> ssupp;
+  if (!fwd_only) {
+    build_scatter_lists(G, BS);
+    ssupp = trim_scatter(BS, n);
+  }
 
   size_t& bytes = p->bytes;
   if ((rc = upload(&p->phx, phx, bytes)) || (rc = upload(&p->phy, phy, bytes)) || (rc = upload(&p->dirs, dirs, bytes)) ||
@@ -1652,7 +1747,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
       (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->gorder, BG.order, bytes)) ||
       (rc = upload(&p->sitems, BS.items, bytes)) || (rc = upload(&p->sperm, BS.perm, bytes)) ||
       (rc = upload(&p->sentries, BS.entries, bytes)) || (rc = upload(&p->slists, BS.lists, bytes)) ||
-      (rc = upload(&p->splits, BS.splits, bytes)))
+      (rc = upload(&p->splits, BS.splits, bytes)) || (rc = upload(&p->gsupp, gsupp, bytes)) ||
+      (rc = upload(&p->ssupp, ssupp, bytes)))
     return bail(rc);
   p->n_gitems = (int)BG.items.size();
   p->n_groot = BG.n_root;
@@ -1843,6 +1939,7 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
     Binning B;
     B.zp = zp;
     build_gather(G, std::min(ROOT, n), B);
+    const std::vector<int2> gs = gather_support(B, n);
     for (size_t i = 0; i < B.items.size(); ++i) {
       const Item& it = B.items[i];
       const bool root_class = (int)i < B.n_root;
@@ -1858,6 +1955,11 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
         gather_stage_desc(it, idx, n, PE, so, sd);
         if (so < 0 || so >= zs || (sd >> 1) > PE)
           return bad("staging offset " + std::to_string(so) + " / slot " + std::to_string(sd >> 1) + " out of range");
+        if ((sd >> 1) < PE) {  // a patch element: its row is one the forward column pass writes
+          const int blk = (int)(so / (8LL * n)), row = (int)((so / 4) % (2 * n));
+          const int l = row < n ? row : row - 2 * n;
+          if (l < gs[blk].x || l >= gs[blk].y) return bad("patch row outside the column-pass support");
+        }
         ++cover[sd >> 1];
       }
       for (int r = 0; r < ph; ++r)
@@ -1877,6 +1979,23 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
   }
   SBinning S;
   build_scatter_lists(G, S);
+  const std::vector<int2> ss = trim_scatter(S, n);
+  {  // every row the inverse column pass reads is written by one owning item or one split tile
+    const int ntx = n / STS + 1, nty = 2 * n / STS;
+    std::vector<int> wr((size_t)ntx * nty, 0);
+    if (S.perm.size() != S.items.size() * SNP) return bad("scatter permutation table");
+    for (const SItem& it : S.items)
+      if (it.slot < 0) ++wr[(size_t)((it.y0 + n) / STS) * ntx + it.x0 / STS];
+    for (const SplitTile& t : S.splits) ++wr[(size_t)((t.y0 + n) / STS) * ntx + t.x0 / STS];
+    for (size_t i = 0; i < wr.size(); ++i)
+      if (wr[i] > 1) return bad("scatter tile written twice");
+    for (int c = 0; c <= n; ++c)
+      for (int l = ss[c >> 2].x; l < ss[c >> 2].y; ++l)
+        if (wr[(size_t)((l + n) / STS) * ntx + c / STS] != 1) return bad("support row not written by the scatter");
+    for (const SItem& it : S.items)
+      if (it.ne > 0 && (it.y0 < ss[it.x0 >> 2].x || it.y0 + STS > ss[it.x0 >> 2].y))
+        return bad("scatter tile outside the support");
+  }
   for (const SItem& it : S.items) {
     if (it.ne > SE_MAX) return bad("scatter entries per item");
     if (it.loff + (long long)it.nl / 2 * 2 * SNP > (long long)S.lists.size()) return bad("list offset");

@@ -54,14 +54,15 @@ def test_phantom_seeded_and_bounded():
 def test_radon_plan_tables_match_kernel_indexing(n):
     """Host-only: every gather staging offset lies inside a half-spectrum plane (tiles at the right
     edge of small grids reach past 2n and must wrap), every patch element is staged once, every tap
-    read stays inside the LDS patch, every scatter slot names a staged entry."""
+    read stays inside the LDS patch, every scatter slot names a staged entry, the column passes cover every row the
+    gather reads and read no row the scatter leaves unwritten (dense and sparse angle sets)."""
     import ctypes as C
 
     import numpy as np
 
     from model_mosaic_amd import _lib
     lib = _lib.load()
-    for nt, span in ((7, np.pi), (24, 2 * np.pi), (37, np.pi)):
+    for nt, span in ((7, np.pi), (24, 2 * np.pi), (37, np.pi), (400, np.pi)):
         th = np.linspace(0, span, nt, endpoint=False).astype(np.float32).astype(np.float64)
         rc = lib.mm_radon_plan_check(n, nt, th.ctypes.data_as(C.POINTER(C.c_double)))
         assert rc == _lib.MM_OK, _lib.last_error()
