@@ -152,12 +152,14 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   const float* ra = obj + ((long long)z * n + y0) * n;
   const float* rb = ra + n;
   const float pya = valid ? phy[y0] : 0.f, pyb = valid ? phy[y0 + 1] : 0.f;
+  // the row (n = L/2 values) occupies points [L/4, 3L/4): m in [PT/4, 3PT/4)
+  static_assert(Sh::PT % 4 == 0 && Sh::PT * Sh::TPC == L, "L/4 is a multiple of TPC");
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
-    const int xi = t + m * Sh::TPC - half;
+    const int xi = t + (m - Sh::PT / 4) * Sh::TPC;
     v[m] = make_float2(0.f, 0.f);
-    if (valid && xi >= 0 && xi < n) {
+    if (valid && m >= Sh::PT / 4 && m < 3 * Sh::PT / 4) {
       const float p = phx[xi];
       v[m] = make_float2(ra[xi] * pya * p, rb[xi] * pyb * p);
     }
@@ -189,16 +191,17 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
 }
 
 // Column transform over C adjacent columns of one z plane of the half
-// spectrum (blockIdx.y = z).  Rows [in_lo, in_hi) are read (others are zero),
-// rows [out_lo, out_hi) are written back in place, times `scale`.  supp holds
-// per 4-column block the centred row range [lo, hi) (row i is l = i for i < n,
-// i - 2n otherwise) that carries spectrum data: the forward pass writes only
-// those rows (the gather reads nothing else), the inverse pass reads only
-// those (the scatter writes nothing else; the disc of radius n leaves the
-// corners of the plane empty).  The
-// register FFT's points t + m TPC of the 4 columns of a block are contiguous
-// (16 rows x 4 columns per wave and m), so loads and stores go straight
-// between HBM and registers.
+// spectrum (blockIdx.y = z), in place.  Forward: the object occupies rows
+// [L/4, 3L/4) (the others are zero padding and are not read); inverse: only
+// rows [L/4, 3L/4) are written back (the row pass crops to them), times
+// `scale`.  Thread t holds rows t + m TPC, and L/4 is a multiple of TPC
+// (PT = 16 or 12), so both ranges are m in [PT/4, 3PT/4), fixed at compile
+// time: the zero inputs fold out of the first radix pass and the unused
+// outputs out of the last.  supp holds per 4-column block the centred row
+// range [lo, hi) (row i is l = i for i < L/2, i - L otherwise) that carries
+// spectrum data: the forward pass writes only those rows (the gather reads
+// nothing else), the inverse pass reads only those (the scatter writes nothing
+// else; the disc of radius n leaves the corners of the plane empty).  The
 template <int L>
 using ColShape = RegShape<L, kBatchElems>;
 
@@ -212,10 +215,9 @@ using ColShapeB = RegShape<L, BATCH>;
 
 template <int L, bool INV, int BATCH = kBatchElems>
 __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __restrict__ buf, int ncols,
-                                                                   long long zs, int in_lo, int in_hi,
-                                                                   int out_lo, int out_hi,
-                                                                   const int2* __restrict__ supp, float scale,
-                                                                   const float2* __restrict__ tw, int xg) {
+                                                                   long long zs, const int2* __restrict__ supp,
+                                                                   float scale, const float2* __restrict__ tw,
+                                                                   int xg) {
   using Sh = ColShapeB<L, BATCH>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
@@ -229,18 +231,24 @@ __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __r
     const int l = i < L / 2 ? i : i - L;
     return l >= sp.x && l < sp.y;
   };
+  constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4;  // rows [L/4, 3L/4)
+  static_assert(Sh::PT % 4 == 0, "L/4 is a multiple of TPC");
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
-    const bool rd = live && i >= in_lo && i < in_hi && (!INV || carries(i));
+    const bool rd = INV ? (live && carries(i)) : (live && m >= M0 && m < M1);
     v[m] = rd ? b[hs_idx(i, col, L)] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, INV>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
-    if (live && i >= out_lo && i < out_hi && (INV || carries(i))) b[hs_idx(i, col, L)] = cscale(v[m], scale);
+    if (INV) {
+      if (live && m >= M0 && m < M1) b[hs_idx(i, col, L)] = cscale(v[m], scale);
+    } else if (live && carries(i)) {
+      b[hs_idx(i, col, L)] = v[m];
+    }
   }
 }
 
@@ -890,10 +898,10 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
   for (int m = 0; m < Sh::PT; ++m) {
     const int kx = t + m * Sh::TPC;
     float2 a, b;
-    if (kx <= n) {
+    if (kx <= L / 2) {  // n = L/2
       a = x[lpad(kx)];
       b = x[OFF + lpad(kx)];
-      if (kx == 0 || kx == n) {
+      if (kx == 0 || kx == L / 2) {
         a.y = 0.f;
         b.y = 0.f;
       }
@@ -915,10 +923,12 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
     z = (int)(unit / half);
     y = 2 * (int)(unit - (long long)z * half);
   }
+  // centre crop: points [L/4, 3L/4) = m in [PT/4, 3PT/4)
+  static_assert(Sh::PT % 4 == 0 && Sh::PT * Sh::TPC == L, "L/4 is a multiple of TPC");
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
-    const int xx = t + m * Sh::TPC - half;
-    if (xx < 0 || xx >= n) continue;
+    if (m < Sh::PT / 4 || m >= 3 * Sh::PT / 4) continue;
+    const int xx = t + (m - Sh::PT / 4) * Sh::TPC;
     const float2 w = cscale(v[m], scale);
     const float px = phx[xx];
     if (CPLX) {
@@ -1531,20 +1541,17 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
 }
 
 template <int L, bool INV>
-static void launch_col(const mm_radon* p, float2* buf, int nz, int in_lo, int in_hi, int out_lo, int out_hi,
-                       float scale, hipStream_t st) {
+static void launch_col(const mm_radon* p, float2* buf, int nz, float scale, hipStream_t st) {
   const int ncols = p->n + 1;
   const int2* supp = INV ? p->ssupp : p->gsupp;
   if (p->tune[MM_RADON_TUNE_COL_SPLIT] && ColShapeB<L, kRowBatch>::C == 2) {
     using Sh = ColShapeB<L, kRowBatch>;
     hipLaunchKernelGGL((col_fft<L, INV, kRowBatch>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz),
-                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, supp, scale,
-                       p->tw + tw_offset(L), 2);
+                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, supp, scale, p->tw + tw_offset(L), 2);
   } else {
     using Sh = ColShape<L>;
     hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                       0, st, buf, ncols, p->zs, in_lo, in_hi, out_lo, out_hi, supp, scale, p->tw + tw_offset(L),
-                       1);
+                       0, st, buf, ncols, p->zs, supp, scale, p->tw + tw_offset(L), 1);
   }
 }
 
@@ -1784,7 +1791,7 @@ int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* strea
   if ((rc = mark(p, 0, 0, st))) return rc;
   MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
   if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, p->n / 2, 3 * p->n / 2, 0, 2 * p->n, 1.0f, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, 1.0f, st)));
   if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
   if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
   MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
@@ -1819,10 +1826,10 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
   const float s = 1.0f / (2.0f * p->n);
-  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, s, st)));
   if ((rc = check_launch("col_fft (RT)"))) return rc;
   if (cplx) {
-    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, 0, 2 * p->n, p->n / 2, 3 * p->n / 2, s, st)));
+    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, s, st)));
     if ((rc = check_launch("col_fft (RT im)"))) return rc;
   }
   if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
