@@ -38,12 +38,23 @@ __global__ void chirp_table(float2* __restrict__ table, const double* __restrict
   table[idx] = make_float2(c * inv, -s * inv);
 }
 
-// Source row of padded row i for a frame of `len` rows (len/2 on each side).
-__device__ __forceinline__ int pad_src(int i, int len, bool mirror) {
-  const int h = len / 2;
-  if (i < h) return mirror ? h - i - 1 : -1;
-  if (i >= len + h) return mirror ? 2 * len - i + h - 1 : -1;
+// Source row of padded point i = t + m TPC of a length-L transform over a
+// frame of len = L/2 rows (len/2 padding on each side; -1: zero).  The frame
+// is m in [PT/4, 3PT/4) (L/4 is a multiple of TPC), so after unrolling over m
+// the frame/padding split is known at compile time.
+template <class Sh>
+__device__ __forceinline__ int pad_src(int m, int t, bool mirror) {
+  static_assert(Sh::PT % 4 == 0, "L/4 is a multiple of TPC");
+  constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4, h = M0 * Sh::TPC, len = 2 * h;
+  const int i = t + m * Sh::TPC;
+  if (m < M0) return mirror ? h - i - 1 : -1;
+  if (m >= M1) return mirror ? 2 * len - i + h - 1 : -1;
   return i - h;
+}
+// Frame row of point t + m TPC (-1: padding).
+template <class Sh>
+__device__ __forceinline__ int frame_row(int m, int t) {
+  return (m >= Sh::PT / 4 && m < 3 * Sh::PT / 4) ? t + (m - Sh::PT / 4) * Sh::TPC : -1;
 }
 
 template <int L>
@@ -74,8 +85,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
-    const int i = t + m * Sh::TPC;
-    const int sr = ADJ ? ((i >= h && i < h + nz) ? i - h : -1) : pad_src(i, nz, mirror);
+    const int sr = ADJ ? frame_row<Sh>(m, t) : pad_src<Sh>(m, t, mirror);
     v[m] = (live && sr >= 0) ? src[(long long)sr * n] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
@@ -99,8 +109,8 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
     if (!live) return;
 #pragma unroll
     for (int m = 0; m < Sh::PT; ++m) {
-      const int r = t + m * Sh::TPC - h;
-      if (r >= 0 && r < nz) dst[(long long)r * n] = v[m];
+      const int r = frame_row<Sh>(m, t);
+      if (r >= 0) dst[(long long)r * n] = v[m];
     }
   }
 }
@@ -122,8 +132,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
-    const int i = t + m * Sh::TPC;
-    const int sx = ADJ ? ((i >= h && i < h + n) ? i - h : -1) : pad_src(i, n, mirror);
+    const int sx = ADJ ? frame_row<Sh>(m, t) : pad_src<Sh>(m, t, mirror);
     v[m] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
@@ -148,8 +157,8 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
     if (!valid) return;
 #pragma unroll
     for (int m = 0; m < Sh::PT; ++m) {
-      const int r = t + m * Sh::TPC - h;
-      if (r >= 0 && r < n) dst[r] = v[m];
+      const int r = frame_row<Sh>(m, t);
+      if (r >= 0) dst[r] = v[m];
     }
   }
 }

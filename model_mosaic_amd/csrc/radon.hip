@@ -193,8 +193,8 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
 // Column transform over C adjacent columns of one z plane of the half
 // spectrum (blockIdx.y = z), in place.  Forward: the object occupies rows
 // [L/4, 3L/4) (the others are zero padding and are not read); inverse: only
-// rows [L/4, 3L/4) are written back (the row pass crops to them), times
-// `scale`.  Thread t holds rows t + m TPC, and L/4 is a multiple of TPC
+// rows [L/4, 3L/4) are written back (the row pass crops to them; it also
+// applies this pass's 1/L).  Thread t holds rows t + m TPC, and L/4 is a multiple of TPC
 // (PT = 16 or 12), so both ranges are m in [PT/4, 3PT/4), fixed at compile
 // time: the zero inputs fold out of the first radix pass and the unused
 // outputs out of the last.  supp holds per 4-column block the centred row
@@ -216,8 +216,7 @@ using ColShapeB = RegShape<L, BATCH>;
 template <int L, bool INV, int BATCH = kBatchElems>
 __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __restrict__ buf, int ncols,
                                                                    long long zs, const int2* __restrict__ supp,
-                                                                   float scale, const float2* __restrict__ tw,
-                                                                   int xg) {
+                                                                   const float2* __restrict__ tw, int xg) {
   using Sh = ColShapeB<L, BATCH>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
@@ -245,7 +244,7 @@ __global__ __launch_bounds__((ColShapeB<L, BATCH>::NT)) void col_fft(float2* __r
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     if (INV) {
-      if (live && m >= M0 && m < M1) b[hs_idx(i, col, L)] = cscale(v[m], scale);
+      if (live && m >= M0 && m < M1) b[hs_idx(i, col, L)] = v[m];
     } else if (live && carries(i)) {
       b[hs_idx(i, col, L)] = v[m];
     }
@@ -923,21 +922,22 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
     z = (int)(unit / half);
     y = 2 * (int)(unit - (long long)z * half);
   }
-  // centre crop: points [L/4, 3L/4) = m in [PT/4, 3PT/4)
+  // centre crop: points [L/4, 3L/4) = m in [PT/4, 3PT/4); `scale` (both
+  // inverse passes' 1/L) rides in the row weights
   static_assert(Sh::PT % 4 == 0 && Sh::PT * Sh::TPC == L, "L/4 is a multiple of TPC");
+  const float pya = phy[y] * scale, pyb = CPLX ? 0.f : phy[y + 1] * scale;
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     if (m < Sh::PT / 4 || m >= 3 * Sh::PT / 4) continue;
     const int xx = t + (m - Sh::PT / 4) * Sh::TPC;
-    const float2 w = cscale(v[m], scale);
     const float px = phx[xx];
     if (CPLX) {
-      const float p = phy[y] * px;
-      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(w.x * p, w.y * p);
+      const float p = pya * px;
+      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(v[m].x * p, v[m].y * p);
     } else {
       float* o = (float*)out_v + ((long long)z * n + y) * n;
-      o[xx] = w.x * phy[y] * px;
-      o[n + xx] = w.y * phy[y + 1] * px;
+      o[xx] = v[m].x * (pya * px);
+      o[n + xx] = v[m].y * (pyb * px);
     }
   }
 }
@@ -1541,17 +1541,17 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
 }
 
 template <int L, bool INV>
-static void launch_col(const mm_radon* p, float2* buf, int nz, float scale, hipStream_t st) {
+static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
   const int ncols = p->n + 1;
   const int2* supp = INV ? p->ssupp : p->gsupp;
   if (p->tune[MM_RADON_TUNE_COL_SPLIT] && ColShapeB<L, kRowBatch>::C == 2) {
     using Sh = ColShapeB<L, kRowBatch>;
     hipLaunchKernelGGL((col_fft<L, INV, kRowBatch>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz),
-                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, supp, scale, p->tw + tw_offset(L), 2);
+                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, supp, p->tw + tw_offset(L), 2);
   } else {
     using Sh = ColShape<L>;
     hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                       0, st, buf, ncols, p->zs, supp, scale, p->tw + tw_offset(L), 1);
+                       0, st, buf, ncols, p->zs, supp, p->tw + tw_offset(L), 1);
   }
 }
 
@@ -1581,7 +1581,7 @@ static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStrea
   using Sh = C2rShape<L>;
   const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
   const dim3 grid((unsigned)((units + Sh::C - 1) / Sh::C));
-  const float scale = 1.0f / (2.0f * p->n);
+  const float scale = (float)(1.0 / (4.0 * (double)p->n * p->n));  // column and row inverse passes, 1/(2n) each
   if (cplx)
     hipLaunchKernelGGL((rt4_rows_c2r<L, true>), grid, dim3(Sh::NT), 0, st, p->fde, p->fde2, out, p->phx, p->phy,
                        p->n, nz, p->zs, scale, p->tw + tw_offset(L));
@@ -1791,7 +1791,7 @@ int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* strea
   if ((rc = mark(p, 0, 0, st))) return rc;
   MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
   if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, 1.0f, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, st)));
   if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
   if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
   MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
@@ -1825,11 +1825,10 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   if ((rc = launch_scatter(p, p->gh, p->fde, nz, st))) return rc;
   if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
-  const float s = 1.0f / (2.0f * p->n);
-  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, s, st)));
+  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, st)));
   if ((rc = check_launch("col_fft (RT)"))) return rc;
   if (cplx) {
-    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, s, st)));
+    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, st)));
     if ((rc = check_launch("col_fft (RT im)"))) return rc;
   }
   if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
