@@ -37,6 +37,25 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "usfft_R_RT_n2048_nz64_hbm_traffic.json")
+
+
+def measured_traffic(stage, n, nt, nz):
+    """HBM bytes per launch of a stage, measured offline with two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE) over this workload and scaled by the counts of a plain 1 GiB device copy; None when no
+    measurement of this workload is on file."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != {"n": n, "ntheta": nt, "nz": nz}:
+        return None
+    st = d.get("stages", {}).get(stage)
+    return None if st is None else int(st["traffic_bytes"])
+
+
 def b_dir(n, nt):
     return 4 * n * n + 32 * n * (n + 1) + 4 * nt * n
 
@@ -433,7 +452,10 @@ def main():
         "config": {"workload": "batched FP32 USFFT R+RT, seeded ellipsoids + 1% noise",
                    "nz": nz, "n": n, "ntheta": nt, "oversampling": 2, "m": 4, "eps": 1e-3},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_ach, 2), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(dom_ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(dom_ach / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(dom, n, nt, nz),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch, copy-calibrated "
+                                       "(profiles/usfft_R_RT_n2048_nz64_hbm_traffic.md)",
                      "algorithmic_bytes": dom_bytes, "avg_ms": round(stage_ms[dom], 4)},
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved": round(step_ach, 2), "unit": "GB/s",
                           "frac": round(step_ach / PEAK_HBM_GBS, 4)},
