@@ -98,6 +98,9 @@ def test_random_angles_and_partial_nz(tomo_cls):
     x = np.random.default_rng(6).standard_normal((3, n, n)).astype(np.float32)
     t = tomo_cls(n, 8, th)   # plan for 8 slices, call with 3
     assert rel(t.R(torch.from_numpy(x).cuda()).cpu().numpy(), usfft.tomo_R(x, th, np.float64)) < 1e-5
+    # RT with 5 of 8 slices: the second z quad of the staging round is partial
+    y = np.random.default_rng(7).standard_normal((nt, 5, n)).astype(np.float32)
+    assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
 
 
 @pytest.mark.parametrize("fp64_tiles", [False, True])
