@@ -144,6 +144,15 @@ void mm_fresnel_destroy(mm_fresnel* plan);
  * row <- crop(irfft(rfft(edge_pad(row)) * w, ne)).  4 n must be 2^k or 3 * 2^k
  * in [64, 16384]. */
 int mm_fbp_filter(float* sino_dev, const float* w_dev, long long rows, int n, void* stream);
+/* The same filter on transforms of length 2 n (no separate counterpart in the
+ * reference; same result as mm_fbp_filter, half the transform length).  With
+ * h = irfft(w, 4 n):  row[i] <- sum_j row[j] h[i - j] + row[0] A[i] + row[n-1] B[i]
+ * (i, j in [0, n)); the host passes G = rfft(g) / (2 n) (n + 1 float32, real
+ * because g is even), g = h on the lags |l| < n laid out mod 2 n, and A[i] /
+ * B[i] (n float32 each) = the sums of h over the lags of the left / right edge
+ * padding.  2 n must be 2^k or 3 * 2^k in [16, 16384]. */
+int mm_fbp_filter_2n(float* sino_dev, const float* G_dev, const float* A_dev, const float* B_dev, long long rows,
+                     int n, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Element-wise kernels of step3_propagation.py:94-110.

@@ -50,6 +50,66 @@ __global__ __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict_
   }
 }
 
+// The same filter on transforms of length L = 2 n.  With h = irfft(w, 4 n)
+// (real and even) the padded convolution splits into
+//   y[i] = sum_j x[j] h[i - j] + x[0] A[i] + x[n - 1] B[i],   i, j in [0, n),
+// A[i] = sum of h over the lags of the left padding, B[i] over the right
+// padding.  The first term only sees lags |i - j| < n, so it is a circular
+// convolution of length 2 n with g = h restricted to those lags, whose
+// spectrum G is real (g is even): the rows occupy points [0, n) (m < PT/2,
+// the zero half folds out of the first radix pass and the unused half of the
+// outputs out of the last), two rows ride in one complex transform as above.
+// G (with the 1/L), A and B come from the host in FP64.
+template <int L>
+__global__ __launch_bounds__(RowReg<L>::NT) void filter_rows_2n(float* __restrict__ sino,
+                                                                const float* __restrict__ G,
+                                                                const float* __restrict__ A,
+                                                                const float* __restrict__ B, long long rows,
+                                                                const float2* __restrict__ tw) {
+  using Sh = RowReg<L>;
+  constexpr int n = L / 2, MH = Sh::PT / 2;  // points [0, n) = m < PT/2
+  static_assert(Sh::PT % 2 == 0 && Sh::PT * Sh::TPC == L, "n is a multiple of TPC");
+  __shared__ float2 lds[Sh::LDS];
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const long long ra = 2 * ((long long)blockIdx.x * Sh::C + c), rb = ra + 1;
+  const bool has_a = ra < rows, has_b = rb < rows;
+  float* pa = sino + (has_a ? ra : 0) * n;
+  float* pb = sino + (has_b ? rb : 0) * n;
+  // edge values, read before any lane of the workgroup writes
+  const float a0 = has_a ? pa[0] : 0.f, a1 = has_a ? pa[n - 1] : 0.f;
+  const float b0 = has_b ? pb[0] : 0.f, b1 = has_b ? pb[n - 1] : 0.f;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int i = t + m * Sh::TPC;
+    v[m] = m < MH ? make_float2(has_a ? pa[i] : 0.f, has_b ? pb[i] : 0.f) : make_float2(0.f, 0.f);
+  }
+  fft_reg<L, Sh::TPC, false>(v, lds + c * Sh::S, t, tw);
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int i = t + m * Sh::TPC;
+    v[m] = cscale(v[m], G[i <= L / 2 ? i : L - i]);
+  }
+  fft_reg<L, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
+#pragma unroll
+  for (int m = 0; m < MH; ++m) {
+    const int i = t + m * Sh::TPC;
+    const float ai = A[i], bi = B[i];
+    if (has_a) pa[i] = fmaf(a1, bi, fmaf(a0, ai, v[m].x));
+    if (has_b) pb[i] = fmaf(b1, bi, fmaf(b0, ai, v[m].y));
+  }
+}
+
+template <int L>
+static void launch_2n(float* sino, const float* G, const float* A, const float* B, long long rows, const float2* tw,
+                      hipStream_t st) {
+  using Sh = RowReg<L>;
+  const long long pairs = (rows + 1) / 2;
+  hipLaunchKernelGGL(filter_rows_2n<L>, dim3((unsigned)((pairs + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, sino, G,
+                     A, B, rows, tw + tw_offset(L));
+}
+
 template <int NE>
 static void launch(float* sino, const float* w, long long rows, int n, const float2* tw, hipStream_t st) {
   using Sh = RowReg<NE>;
@@ -75,6 +135,21 @@ int mm_fbp_filter(float* sino_dev, const float* w_dev, long long rows, int n, vo
   if (rc) return rc;
   MM_DISPATCH_FFT(4 * n, (mm::fbp::launch<LL>(sino_dev, w_dev, rows, n, tw, (hipStream_t)stream)));
   return mm::check_launch("fbp filter_rows");
+}
+
+int mm_fbp_filter_2n(float* sino_dev, const float* G_dev, const float* A_dev, const float* B_dev, long long rows,
+                     int n, void* stream) {
+  MM_ARG_CHECK(rows >= 0, "mm_fbp_filter_2n: rows < 0");
+  MM_ARG_CHECK(n >= 8 && mm::is_fft_len(2L * n) && 2 * n >= 16 && 2 * n <= mm::kMaxFFT,
+               "mm_fbp_filter_2n: 2n must be an FFT length 2^k or 3 * 2^k in [16, 16384], got n=" + std::to_string(n));
+  if (rows == 0) return MM_OK;
+  MM_ARG_CHECK(sino_dev && G_dev && A_dev && B_dev, "mm_fbp_filter_2n: null pointer");
+  MM_ARG_CHECK((rows + 1) / 2 < (1LL << 31), "mm_fbp_filter_2n: too many rows");
+  const float2* tw = nullptr;
+  int rc = mm::twiddle_bank(&tw);
+  if (rc) return rc;
+  MM_DISPATCH_FFT(2 * n, (mm::fbp::launch_2n<LL>(sino_dev, G_dev, A_dev, B_dev, rows, tw, (hipStream_t)stream)));
+  return mm::check_launch("fbp filter_rows_2n");
 }
 
 }  // extern "C"

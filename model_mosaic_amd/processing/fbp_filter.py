@@ -9,6 +9,12 @@ back as the real and imaginary parts), with the edge padding, the weights and
 the crop fused into one kernel (mm_fbp_filter): one read and one write of the
 sinogram.  The weights are computed on the host exactly as the reference does
 (calc_filter with the 12-point quadrature _wint, :48-126).
+
+By default the filter runs on transforms of length 2 n instead of 4 n
+(mm_fbp_filter_2n): with h = irfft(w, 4 n) the edge-padded convolution is a
+circular convolution of length 2 n with h cut to the lags |l| < n, plus the
+two edge values times fixed vectors A, B (the sums of h over the padding).
+``FBPFilter(n, method="4n")`` keeps the padded transforms of the reference.
 """
 from __future__ import annotations
 
@@ -46,6 +52,25 @@ def wint(order: int, t: np.ndarray) -> np.ndarray:
     return w
 
 
+def two_n_tables(w: np.ndarray, n: int):
+    """(G, A, B) float32 of the length-2n form of the filter with half-spectrum weights w (4n // 2 + 1):
+    h = irfft(w, 4n) is real and even; G = rfft(g) / (2n) (real, n + 1 values) for g = h on the lags
+    |l| < n laid out mod 2n; A[i] = sum_{d = i+1}^{pad+i} h[d] (left padding, pad = 3n/2 values of row[0])
+    and B[i] = A[n - 1 - i] (right padding, row[n-1]).  Computed in FP64."""
+    ne, L = 4 * n, 2 * n
+    pad = (ne - n) // 2
+    h = np.fft.irfft(np.asarray(w, dtype=np.float64), ne)
+    g = np.zeros(L)
+    lags = np.arange(-(n - 1), n)
+    g[lags % L] = h[lags % ne]
+    G = np.fft.rfft(g).real / L
+    cs = np.cumsum(h)
+    i = np.arange(n)
+    A = cs[pad + i] - cs[i]
+    B = A[::-1].copy()
+    return G.astype(np.float32), A.astype(np.float32), B.astype(np.float32)
+
+
 def calc_filter_host(n: int, name: str) -> np.ndarray:
     """(4n // 2 + 1,) float32 weights of the named filter (processing/fbp_filter.py:48-91)."""
     d = 0.5
@@ -79,14 +104,18 @@ class FBPFilter:
     """Same interface as processing/fbp_filter.py:39 (``FBPFilter(n, batch_chunk)``, ``calc_filter``,
     ``filter``, ``filter_host``); carriers are torch ROCm tensors."""
 
-    def __init__(self, n: int, batch_chunk: int = 256):
+    def __init__(self, n: int, batch_chunk: int = 256, method: str = "2n"):
         require_gpu()
+        if method not in ("2n", "4n"):
+            raise ValueError(f"method must be '2n' or '4n', got {method!r}")
         self._lib = _lib.load()
         self.n = int(n)
         self.ne = 4 * self.n
         self.pad = (self.ne - self.n) // 2
         self.batch_chunk = int(batch_chunk)
+        self.method = method
         self._cache: dict[str, torch.Tensor] = {}
+        self._tables: dict[tuple, tuple] = {}
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     def calc_filter(self, name: str) -> torch.Tensor:
@@ -107,9 +136,23 @@ class FBPFilter:
         if w.numel() != self.ne // 2 + 1:
             raise ValueError(f"filter weights have {w.numel()} values, expected {self.ne // 2 + 1}")
         rows = sino.numel() // self.n
-        _lib.check(self._lib.mm_fbp_filter(sino.data_ptr(), w.data_ptr(), rows, self.n, _lib.stream_handle()),
-                   "FBPFilter.filter")
+        if self.method == "4n":
+            _lib.check(self._lib.mm_fbp_filter(sino.data_ptr(), w.data_ptr(), rows, self.n, _lib.stream_handle()),
+                       "FBPFilter.filter")
+            return sino
+        G, A, B = self._two_n(w)
+        _lib.check(self._lib.mm_fbp_filter_2n(sino.data_ptr(), G.data_ptr(), A.data_ptr(), B.data_ptr(), rows,
+                                              self.n, _lib.stream_handle()), "FBPFilter.filter")
         return sino
+
+    def _two_n(self, w: torch.Tensor):
+        """Device (G, A, B) of the length-2n form for weights w, built once per weight tensor (keyed by its
+        storage and version counter, so an in-place change of w rebuilds them)."""
+        key = (w.data_ptr(), w._version, w.numel())
+        if key not in self._tables:
+            G, A, B = two_n_tables(w.detach().cpu().numpy(), self.n)
+            self._tables[key] = tuple(torch.from_numpy(a).to(w.device) for a in (G, A, B))
+        return self._tables[key]
 
     def filter_host(self, sino_h: np.ndarray, w: torch.Tensor, batch_chunk: int | None = None) -> np.ndarray:
         """Host sinogram filtered in place, in slabs of ``batch_chunk`` rows (processing/fbp_filter.py:212)."""

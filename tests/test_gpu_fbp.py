@@ -13,11 +13,12 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+@pytest.mark.parametrize("method", ["2n", "4n"])
 @pytest.mark.parametrize("n,rows,name", [(64, 7, "shepp"), (96, 10, "ramp"), (256, 33, "parzen"),
                                          (2048, 9, "shepp"), (1536, 4, "hann"), (16, 3, "cosine")])
-def test_filter_vs_fp64(n, rows, name):
+def test_filter_vs_fp64(n, rows, name, method):
     from model_mosaic_amd.processing.fbp_filter import FBPFilter
-    f = FBPFilter(n)
+    f = FBPFilter(n, method=method)
     w = f.calc_filter(name)
     x = np.random.default_rng(n + rows).standard_normal((rows, n)).astype(np.float32)
     d = torch.from_numpy(x).cuda()
@@ -25,15 +26,16 @@ def test_filter_vs_fp64(n, rows, name):
     assert out.data_ptr() == d.data_ptr()          # in place, like the reference
     ref = fbp.filter(x, w.cpu().numpy().astype(np.float64))
     err = rel(d.cpu().numpy(), ref)
-    print(f"fbp {name} n={n} rows={rows}: rel-L2 vs FP64 {err:.2e}")
+    print(f"fbp {method} {name} n={n} rows={rows}: rel-L2 vs FP64 {err:.2e}")
     assert err < 1e-5
 
 
-def test_none_filter_closed_form_and_3d_sinogram():
+@pytest.mark.parametrize("method", ["2n", "4n"])
+def test_none_filter_closed_form_and_3d_sinogram(method):
     from model_mosaic_amd.processing.fbp_filter import FBPFilter
     n = 128
     ne = 4 * n
-    f = FBPFilter(n)
+    f = FBPFilter(n, method=method)
     x = np.random.default_rng(3).standard_normal((5, 3, n)).astype(np.float32)   # (ntheta, nz, n)
     d = torch.from_numpy(x).cuda()
     f.filter(d, f.calc_filter("none"))
@@ -56,3 +58,16 @@ def test_filter_host_and_errors():
         f.filter(torch.zeros((4, 2 * n), device="cuda")[:, :n], w)   # not contiguous
     with pytest.raises(ValueError):
         f.calc_filter("gauss")
+
+
+def test_weights_changed_in_place_rebuild_the_2n_tables():
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter
+    n = 64
+    f = FBPFilter(n)
+    w = f.calc_filter("ramp").clone()
+    x = np.random.default_rng(5).standard_normal((4, n)).astype(np.float32)
+    f.filter(torch.from_numpy(x).cuda(), w)
+    w.mul_(0.5)
+    d = torch.from_numpy(x).cuda()
+    f.filter(d, w)
+    assert rel(d.cpu().numpy(), fbp.filter(x, w.cpu().numpy().astype(np.float64))) < 1e-5

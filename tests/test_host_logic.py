@@ -76,3 +76,21 @@ def test_fbp_filter_weights_equal_the_oracle(n):
     from oracle import fbp
     for name in FILTERS:
         assert np.array_equal(calc_filter_host(n, name), fbp.calc_filter(n, name)), name
+
+
+@pytest.mark.parametrize("n,name", [(16, "ramp"), (64, "shepp"), (96, "hann"), (512, "parzen"), (2048, "shepp")])
+def test_fbp_two_n_tables_reproduce_the_padded_filter(n, name):
+    """Host-only: the length-2n form (two_n_tables: G, A, B) applied in FP64 with numpy FFTs equals the
+    oracle's edge-padded length-4n filter (the reference's algorithm) to rounding."""
+    from model_mosaic_amd.processing.fbp_filter import calc_filter_host, two_n_tables
+
+    from oracle import fbp
+    w = calc_filter_host(n, name)
+    G, A, B = (a.astype(np.float64) for a in two_n_tables(w, n))
+    x = np.random.default_rng(n).standard_normal((5, n))
+    L = 2 * n
+    core = np.fft.irfft(np.fft.rfft(x, L) * (G * L), L)[:, :n]
+    y = core + x[:, :1] * A + x[:, -1:] * B
+    ref = fbp.filter(x, w.astype(np.float64))
+    # G, A, B are handed over in float32: 1e-6 relative
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-6
