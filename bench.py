@@ -206,7 +206,39 @@ def run_fbp(torch, reps=5):
         dt = (time.perf_counter() - t0) / reps
         out[method] = {"ms_per_call": round(dt * 1e3, 4), "achieved_GBs": round(nbytes / dt / 1e9, 1),
                        "hbm_frac": round(nbytes / dt / 1e9 / PEAK_HBM_GBS, 4)}
-    del sino
+    # FBP: filter then RT (two passes over the sinogram) vs RT_filtered (filter inside RT's line pass)
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.processing.tomo import TomoReal
+    tomo = TomoReal(n, nz, phantom.angles(nt))
+    f = FBPFilter(n)
+    w = f.calc_filter("shepp")
+    rec = torch.empty((nz, n, n), device=dev, dtype=torch.float32)
+    work = torch.empty_like(sino)
+
+    def two_pass():
+        work.copy_(sino)  # the filter is in place; RT_filtered leaves its input alone
+        tomo.RT(f.filter(work, w), out=rec)
+
+    def fused():
+        tomo.RT_filtered(sino, f, w, out=rec)
+
+    copy_s = None
+    for name, fn in (("copy", lambda: work.copy_(sino)), ("filter_then_RT", two_pass), ("RT_filtered", fused)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        if name == "copy":
+            copy_s = dt
+            continue
+        if name == "filter_then_RT":
+            dt -= copy_s  # the copy only keeps the input for the next repetition
+        out[name + "_ms"] = round(dt * 1e3, 4)
+    tomo.free()
+    del sino, work, rec
     torch.cuda.empty_cache()
     return out
 
@@ -426,7 +458,8 @@ def main():
         print("[bench] FBP filter", file=sys.stderr, flush=True)
         fbp = run_fbp(torch)
         print(f"[bench] FBP filter {fbp['2n']['ms_per_call']:.2f} ms (2n transforms), {fbp['4n']['ms_per_call']:.2f} ms "
-              f"(4n) per 2048x64x2048 sinogram", file=sys.stderr, flush=True)
+              f"(4n) per 2048x64x2048 sinogram; filter + RT {fbp['filter_then_RT_ms']:.2f} ms, RT_filtered "
+              f"{fbp['RT_filtered_ms']:.2f} ms", file=sys.stderr, flush=True)
 
     mosaic = None
     if not a.no_mosaic:

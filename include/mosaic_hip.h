@@ -68,6 +68,14 @@ int mm_radon_fwd(mm_radon* plan, const float* obj_dev, float* sino_dev, int nz, 
  * Same operator as processing/tomo.py:153-188 (TomoReal.RT): real part for a
  * float32 sinogram, the full complex result for a complex64 one. */
 int mm_radon_adj(mm_radon* plan, const void* sino_dev, int sino_is_c64, void* obj_dev, int nz, void* stream);
+/* RT of the FBP-filtered real sinogram in one pass over it: replaces
+ * FBPFilter.filter (processing/fbp_filter.py:128-210) followed by RT
+ * (processing/tomo.py:153-188), the pair step8_fbp.py:204-205 calls.  The
+ * filter runs in the length-2n form of mm_fbp_filter_2n (G, A, B as there)
+ * inside the line transform; sino_dev (ntheta, nz, n) float32 is not
+ * modified; obj_dev (nz, n, n) float32. */
+int mm_radon_adj_fbp(mm_radon* plan, const float* sino_dev, const float* G_dev, const float* A_dev,
+                     const float* B_dev, float* obj_dev, int nz, void* stream);
 
 /* Stage timing (no counterpart in the reference; for benchmarks).
  * mm_radon_profile(plan, 1) makes every later R / RT call record HIP events

@@ -55,6 +55,7 @@ _SIGS = {
     "mm_fresnel_destroy": (None, [_vp]),
     "mm_fbp_filter": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp]),
     "mm_fbp_filter_2n": (C.c_int, [_vp, _vp, _vp, _vp, C.c_longlong, C.c_int, _vp]),
+    "mm_radon_adj_fbp": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "mm_psi_from_proj": (C.c_int, [_vp, _vp, C.c_longlong, C.c_float, C.c_float, _vp]),
     "mm_abs2": (C.c_int, [_vp, _vp, C.c_longlong, _vp]),
     "mm_mosaic_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -529,6 +529,31 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r4_line_ifft(const float2* __re
 // sinogram yields the two channels of its real and imaginary parts (which pair
 // yhat(r) with yhat(L - r), so that output passes through LDS; the real one is
 // written straight from registers).
+// Representatives of one real line from its transform (thread t holds r = t + m TPC):
+// yhat(r) = c1d(r) v, k = r - L/2 (sigma > 0; k = L/2 from r = 0, conjugated) or
+// k = L/2 - r (sigma < 0), fold weights 1/2 at k = 0 and k = L/2.
+template <class Sh>
+__device__ __forceinline__ void rt1_emit_real(const float2 (&v)[Sh::PT], int t, int sg, float2* __restrict__ dst,
+                                              float scale) {
+  constexpr int half = Sh::PT * Sh::TPC / 2;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int r = t + m * Sh::TPC;
+    const float2 y = (r & 1) ? v[m] : make_float2(-v[m].x, -v[m].y);  // yhat(r)
+    if (sg > 0) {
+      if (r >= half) {
+        const int k = r - half;
+        dst[k] = cscale(y, (k == 0 ? 0.5f : 1.0f) * scale);
+      } else if (r == 0) {
+        dst[half] = cscale(cconj(y), 0.5f * scale);
+      }
+    } else if (r <= half) {
+      const int k = half - r;
+      dst[k] = cscale(y, ((k == 0 || k == half) ? 0.5f : 1.0f) * scale);
+    }
+  }
+}
+
 template <int L, bool CPLX>
 __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
                                                            float2* __restrict__ yim, const int* __restrict__ sig, int H,
@@ -558,24 +583,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
   if (!CPLX) {
-    if (!valid) return;
-    float2* dst = yre + rr * H;
-#pragma unroll
-    for (int m = 0; m < Sh::PT; ++m) {
-      const int r = t + m * Sh::TPC;
-      const float2 y = (r & 1) ? v[m] : make_float2(-v[m].x, -v[m].y);  // yhat(r)
-      if (sg > 0) {
-        if (r >= half) {
-          const int k = r - half;
-          dst[k] = cscale(y, (k == 0 ? 0.5f : 1.0f) * scale);
-        } else if (r == 0) {
-          dst[half] = cscale(cconj(y), 0.5f * scale);
-        }
-      } else if (r <= half) {
-        const int k = half - r;
-        dst[k] = cscale(y, ((k == 0 || k == half) ? 0.5f : 1.0f) * scale);
-      }
-    }
+    if (valid) rt1_emit_real<Sh>(v, t, sg, yre + rr * H, scale);
     return;
   }
 #pragma unroll
@@ -607,6 +615,87 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __rest
     yre[rr * H + k] = cscale(vr, f);
     yim[rr * H + k] = cscale(vi, f);
   }
+}
+
+// Line pass of RT behind the FBP filter (processing/fbp_filter.py:128-210, then
+// tomo.py:160-170), for real sinograms: each row is filtered in the length-2n
+// form of the edge-padded filter (G, A, B: see mm_fbp_filter_2n; two rows per
+// complex transform of length 2L), handed through LDS to the line transform of
+// length L, and its representatives are written as by rt1_line_fft.  The
+// filtered sinogram never goes to HBM and the input is not modified.  Both
+// stages use the same threads: C2 filter transforms = C1 = 2 C2 lines.
+template <int L>
+struct FbpLineShape {  // C2 filter transforms of length 2L and C1 = 2 C2 lines of length L per workgroup
+  static constexpr int C2 = kRowBatch / (2 * L) > 0 ? kRowBatch / (2 * L) : 1;
+  using S1 = RegShape<L, C2 * 2 * L, 0, false>;
+  using S2 = RegShape<2 * L, C2 * 2 * L, 0, false>;
+};
+
+template <int L>
+__global__ __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1_fbp_line_fft(const float* __restrict__ sino,
+                                                                  float2* __restrict__ yre,
+                                                                  const int* __restrict__ sig, int H,
+                                                                  long long nrows, int nz, float scale,
+                                                                  const float* __restrict__ G,
+                                                                  const float* __restrict__ A,
+                                                                  const float* __restrict__ B,
+                                                                  const float2* __restrict__ tw1,
+                                                                  const float2* __restrict__ tw2) {
+  using S1 = typename FbpLineShape<L>::S1;
+  using S2 = typename FbpLineShape<L>::S2;
+  static_assert(S1::NT == S2::NT && S1::C == 2 * S2::C && S1::PT == S2::PT, "one thread layout for both stages");
+  static_assert(S2::PT % 2 == 0, "the row is points m < PT/2 of the filter transform");
+  constexpr int LDSN = S1::LDS > S2::LDS ? S1::LDS : S2::LDS;
+  constexpr int MH = S2::PT / 2;
+  __shared__ float2 lds[LDSN];
+  const long long row0 = (long long)blockIdx.x * S1::C;
+  // stage 1: filter rows 2 c2 and 2 c2 + 1 of the workgroup (real / imaginary part)
+  int c2, t2;
+  S2::map(threadIdx.x, c2, t2);
+  const long long ra = row0 + 2 * c2, rb = ra + 1;
+  const bool has_a = ra < nrows, has_b = rb < nrows;
+  const float* pa = sino + (has_a ? ra : 0) * L;
+  const float* pb = sino + (has_b ? rb : 0) * L;
+  const float a0 = has_a ? pa[0] : 0.f, a1 = has_a ? pa[L - 1] : 0.f;
+  const float b0 = has_b ? pb[0] : 0.f, b1 = has_b ? pb[L - 1] : 0.f;
+  float2 v[S2::PT];
+#pragma unroll
+  for (int m = 0; m < S2::PT; ++m) {
+    const int i = t2 + m * S2::TPC;
+    v[m] = m < MH ? make_float2(has_a ? pa[i] : 0.f, has_b ? pb[i] : 0.f) : make_float2(0.f, 0.f);
+  }
+  fft_reg<2 * L, S2::TPC, false>(v, lds + c2 * S2::S, t2, tw2);
+#pragma unroll
+  for (int m = 0; m < S2::PT; ++m) {
+    const int i = t2 + m * S2::TPC;
+    v[m] = cscale(v[m], G[i <= L ? i : 2 * L - i]);
+  }
+  fft_reg<2 * L, S2::TPC, true>(v, lds + c2 * S2::S, t2, tw2);
+  __syncthreads();  // the filter's last exchange is read out
+#pragma unroll
+  for (int m = 0; m < MH; ++m) {
+    const int i = t2 + m * S2::TPC;
+    const float ai = A[i], bi = B[i];
+    lds[(2 * c2) * S1::S + lpad(i)] = make_float2(fmaf(a1, bi, fmaf(a0, ai, v[m].x)), 0.f);
+    lds[(2 * c2 + 1) * S1::S + lpad(i)] = make_float2(fmaf(b1, bi, fmaf(b0, ai, v[m].y)), 0.f);
+  }
+  __syncthreads();
+  // stage 2: line c1 = row row0 + c1, as rt1_line_fft
+  int c1, t1;
+  S1::map(threadIdx.x, c1, t1);
+  const long long row = row0 + c1;
+  const bool valid = row < nrows;
+  const int sg = valid ? sig[row / nz] : 1;
+  float2 w[S1::PT];
+#pragma unroll
+  for (int m = 0; m < S1::PT; ++m) {
+    const int r = t1 + m * S1::TPC;
+    const float2 u = lds[c1 * S1::S + lpad(r)];
+    w[m] = (r & 1) ? u : make_float2(-u.x, -u.y);
+  }
+  __syncthreads();  // the transform's exchanges reuse the staging area
+  fft_reg<L, S1::TPC, false>(w, lds + c1 * S1::S, t1, tw1);
+  if (valid) rt1_emit_real<S1>(w, t1, sg, yre + row * H, scale);
 }
 
 // ---- adjoint scatter: per-output-point contribution lists (owner computes) --
@@ -1577,6 +1666,17 @@ static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, h
 }
 
 template <int L>
+static void launch_rt1_fbp(const mm_radon* p, const float* sino, const float* G, const float* A, const float* B,
+                           int nz, hipStream_t st) {
+  if constexpr (2 * L <= kMaxFFT) {
+    using Sh = typename FbpLineShape<L>::S1;
+    const long long rows = (long long)p->ntheta * nz;
+    hipLaunchKernelGGL(rt1_fbp_line_fft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, sino,
+                       p->gh, p->sig, p->H, rows, nz, p->C0s, G, A, B, p->tw + tw_offset(L), p->tw + tw_offset(2 * L));
+  }
+}
+
+template <int L>
 static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
   using Sh = C2rShape<L>;
   const long long units = cplx ? (long long)nz * p->n : (long long)nz * (p->n / 2);
@@ -1799,6 +1899,8 @@ int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* strea
   return mark(p, 3, 1, st);
 }
 
+static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t st);
+
 int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int nz, void* stream) {
   MM_ARG_CHECK(p != nullptr, "mm_radon_adj: plan is NULL");
   MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
@@ -1817,11 +1919,19 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
       return rc;
   }
   hipStream_t st = (hipStream_t)stream;
-  const int L2 = 2 * p->n, L1 = p->n;
+  const int L1 = p->n;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
   MM_DISPATCH_FFT(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
-  if ((rc = check_launch("rt1_line_fft")) || (rc = mark(p, 4, 1, st)) || (rc = mark(p, 5, 0, st))) return rc;
+  if ((rc = check_launch("rt1_line_fft")) || (rc = mark(p, 4, 1, st))) return rc;
+  return radon_adj_tail(p, cplx, obj, nz, st);
+}
+
+// RT after the line pass: scatter, column IFFT, C2R rows.
+static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t st) {
+  const int L2 = 2 * p->n;
+  int rc;
+  if ((rc = mark(p, 5, 0, st))) return rc;
   if ((rc = launch_scatter(p, p->gh, p->fde, nz, st))) return rc;
   if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
@@ -1835,6 +1945,21 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
   MM_DISPATCH_FFT(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
   if ((rc = check_launch("rt4_rows_c2r"))) return rc;
   return mark(p, 7, 1, st);
+}
+
+int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float* A, const float* B, float* obj,
+                     int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_adj_fbp: plan is NULL");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
+               "mm_radon_adj_fbp: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
+  MM_ARG_CHECK(sino && G && A && B && obj, "mm_radon_adj_fbp: null pointer");
+  if (p->fwd_only) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: forward-only plan (no scatter tables)");
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = mark(p, 4, 0, st))) return rc;
+  MM_DISPATCH_FFT(p->n, (launch_rt1_fbp<LL>(p, sino, G, A, B, nz, st)));
+  if ((rc = check_launch("rt1_fbp_line_fft")) || (rc = mark(p, 4, 1, st))) return rc;
+  return radon_adj_tail(p, false, obj, nz, st);
 }
 
 int mm_radon_tune(mm_radon* p, int knob, int value) {

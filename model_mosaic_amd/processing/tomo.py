@@ -103,3 +103,26 @@ class TomoReal:
         _lib.check(self._lib.mm_radon_adj(self._plan, y.data_ptr(), 0 if is_real else 1, out.data_ptr(), nz,
                                           _lib.stream_handle()), "TomoReal.RT")
         return out
+
+    def RT_filtered(self, data, fbp, w, out: torch.Tensor | None = None) -> torch.Tensor:
+        """RT of the FBP-filtered real sinogram, ``fbp.filter(data, w)`` then ``RT`` (step8_fbp.py:204-205),
+        as one pass over the sinogram: the filter (length-2n form, ``FBPFilter``) runs inside the line
+        transform of RT, so the filtered sinogram never goes to HBM and ``data`` is left unchanged."""
+        self._alive()
+        y = as_device_tensor(data, torch.float32, self.device).contiguous()
+        if y.ndim != 3 or y.shape[0] != self.ntheta or y.shape[2] != self.n:
+            raise ValueError(f"TomoReal.RT_filtered: data shape {tuple(y.shape)} != ({self.ntheta}, nz, {self.n})")
+        if fbp.n != self.n:
+            raise ValueError(f"TomoReal.RT_filtered: filter n={fbp.n} != {self.n}")
+        nz = y.shape[1]
+        if not 1 <= nz <= self._nz:
+            raise ValueError(f"TomoReal.RT_filtered: nz={nz} outside [1, {self._nz}]")
+        w = as_device_tensor(w, torch.float32, self.device).contiguous()
+        if w.numel() != fbp.ne // 2 + 1:
+            raise ValueError(f"filter weights have {w.numel()} values, expected {fbp.ne // 2 + 1}")
+        G, A, B = fbp._two_n(w)
+        if out is None:
+            out = torch.empty((nz, self.n, self.n), dtype=torch.float32, device=self.device)
+        _lib.check(self._lib.mm_radon_adj_fbp(self._plan, y.data_ptr(), G.data_ptr(), A.data_ptr(), B.data_ptr(),
+                                              out.data_ptr(), nz, _lib.stream_handle()), "TomoReal.RT_filtered")
+        return out

@@ -71,3 +71,30 @@ def test_weights_changed_in_place_rebuild_the_2n_tables():
     d = torch.from_numpy(x).cuda()
     f.filter(d, w)
     assert rel(d.cpu().numpy(), fbp.filter(x, w.cpu().numpy().astype(np.float64))) < 1e-5
+
+
+@pytest.mark.parametrize("n,nt,nz", [(64, 24, 3), (96, 40, 2), (256, 100, 4), (2048, 64, 2)])
+def test_RT_filtered_equals_filter_then_RT(n, nt, nz):
+    """TomoReal.RT_filtered (filter fused into RT's line pass) = FBPFilter.filter then TomoReal.RT to FP32
+    rounding (same transforms; the compiler may contract the fused kernel's arithmetic differently), and
+    both = the FP64 oracle chain; the input is untouched."""
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter
+    from model_mosaic_amd.processing.tomo import TomoReal
+
+    from oracle import usfft
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    t = TomoReal(n, nz, th)
+    f = FBPFilter(n)
+    w = f.calc_filter("shepp")
+    y = np.random.default_rng(n).standard_normal((nt, nz, n)).astype(np.float32)
+    d = torch.from_numpy(y).cuda()
+    fused = t.RT_filtered(d, f, w).cpu().numpy()
+    assert np.array_equal(d.cpu().numpy(), y)
+    two = t.RT(f.filter(d.clone(), w)).cpu().numpy()
+    e2 = rel(fused, two)
+    print(f"RT_filtered n={n}: rel-L2 vs filter + RT {e2:.2e}")
+    assert e2 < 1e-6
+    ref = usfft.tomo_RT(fbp.filter(y, w.cpu().numpy().astype(np.float64)), th, dtype=np.float64)
+    err = rel(fused, ref)
+    print(f"RT_filtered n={n}: rel-L2 vs FP64 {err:.2e}")
+    assert err < 1e-5
