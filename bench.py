@@ -138,6 +138,32 @@ def run_fresnel(torch, reps=3):
     return res
 
 
+def run_paganin(torch, reps=5):
+    """Paganin retrieval of 128 intensity frames of 2048 x 2048 (30 keV, 0.69 um, 0.8 m, alpha 1e-3).  Algorithmic
+    bytes = the float32 frame read once and written once; the three-pass kernels move 24 bytes per pixel."""
+    from model_mosaic_amd.processing.paganin import Paganin
+    b, nz, n = 128, 2048, 2048
+    dev = torch.device("cuda", 0)
+    x = torch.rand((b, nz, n), device=dev, dtype=torch.float32).mul_(0.5).add_(0.5)
+    out = torch.empty_like(x)
+    pg = Paganin(n, nz, b, 1.24e-9 / 30.0, 0.69e-6, 0.8, 1e-3)
+    pg.retrieve(x, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pg.retrieve(x, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    alg = 8 * b * nz * n
+    pg.free()
+    del x, out
+    torch.cuda.empty_cache()
+    return {"config": f"{b} frames {nz}x{n} float32, 30keV, pixel 0.69um, z=0.8 m, alpha=1e-3",
+            "ms_per_call": round(dt * 1e3, 4), "frames_per_s": round(b / dt, 1),
+            "algorithmic_GBs": round(alg / dt / 1e9, 1), "hbm_frac": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
+            "moved_GBs": round(3 * alg / dt / 1e9, 1)}
+
+
 def run_model(torch, steps=3, warmup=1):
     import numpy as np
 
@@ -316,6 +342,7 @@ def main():
     ap.add_argument("--no-fresnel", action="store_true")
     ap.add_argument("--no-mosaic", action="store_true")
     ap.add_argument("--no-fbp", action="store_true")
+    ap.add_argument("--no-paganin", action="store_true")
     ap.add_argument("--only-mosaic", action="store_true", help="run only the uneven mosaic leg (testing)")
     ap.add_argument("--gather-zp", type=int, default=2, choices=(1, 2),
                     help="z slices per gather patch element (TomoReal gather_zp)")
@@ -453,6 +480,13 @@ def main():
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    pag = None
+    if not a.no_paganin:
+        print("[bench] Paganin retrieval, 128 frames 2048^2", file=sys.stderr, flush=True)
+        pag = run_paganin(torch)
+        print(f"[bench] Paganin {pag['ms_per_call']:.2f} ms per 128 frames ({pag['moved_GBs']:.0f} GB/s moved)",
+              file=sys.stderr, flush=True)
+
     fbp = None
     if not a.no_fbp:
         print("[bench] FBP filter", file=sys.stderr, flush=True)
@@ -505,6 +539,7 @@ def main():
         "batched_over_same_kernels_one_slice": None if plain is None else round(value / plain, 3),
         "model_fwd_bwd": model,
         "fresnel_D_DT": fres,
+        "paganin": pag,
         "mosaic": mosaic,
         "fbp_filter": fbp,
         "cpu_baseline": cpu,

@@ -163,6 +163,22 @@ int mm_fbp_filter_2n(float* sino_dev, const float* G_dev, const float* A_dev, co
                      int n, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Paganin phase retrieval.  mm_paganin_create replaces Paganin.__init__
+ * (processing/paganin.py:37-68: n, nz, ntheta frames, wavelength, voxelsize,
+ * distance, alpha); mm_paganin_retrieve replaces Paganin.retrieve
+ * (:70-92): every (nz, n) float32 frame of in_dev (nframes, nz, n) becomes
+ * -log(max(irfft2(H rfft2(frame)), 1e-6)) in out_dev (same shape; may equal
+ * in_dev), H = alpha / (wavelength distance (kx^2 + ky^2) / (4 pi) + alpha)
+ * on the fftfreq grids of spacing voxelsize.  n and nz: 2^k or 3 * 2^k in
+ * [16, 16384]; nframes <= nframes_max. */
+typedef struct mm_paganin mm_paganin;
+int mm_paganin_create(mm_paganin** plan, int n, int nz, int nframes_max, double wavelength, double voxelsize,
+                      double distance, double alpha);
+int mm_paganin_retrieve(mm_paganin* plan, const float* in_dev, float* out_dev, int nframes, void* stream);
+size_t mm_paganin_workspace_bytes(const mm_paganin* plan);
+void mm_paganin_destroy(mm_paganin* plan);
+
+/* ---------------------------------------------------------------------------
  * Element-wise kernels of step3_propagation.py:94-110.
  * psi = exp(-p*inv_norm*inv_beta_ratio) * (cos(p*inv_norm) + i sin(p*inv_norm)).
  */

@@ -53,6 +53,11 @@ _SIGS = {
     "mm_fresnel_adj": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "mm_fresnel_workspace_bytes": (C.c_size_t, [_vp]),
     "mm_fresnel_destroy": (None, [_vp]),
+    "mm_paganin_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double]),
+    "mm_paganin_retrieve": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "mm_paganin_workspace_bytes": (C.c_size_t, [_vp]),
+    "mm_paganin_destroy": (None, [_vp]),
     "mm_fbp_filter": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp]),
     "mm_fbp_filter_2n": (C.c_int, [_vp, _vp, _vp, _vp, C.c_longlong, C.c_int, _vp]),
     "mm_radon_adj_fbp": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),

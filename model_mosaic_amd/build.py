@@ -20,7 +20,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJDIR = HERE / "_build"
 LIB = HERE / "libmosaic_hip.so"
-SOURCES = ["common.hip", "radon.hip", "fresnel.hip", "model.hip", "mosaic.hip", "plain.hip", "fbp.hip"]
+SOURCES = ["common.hip", "radon.hip", "fresnel.hip", "model.hip", "mosaic.hip", "plain.hip", "fbp.hip",
+           "paganin.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -1,0 +1,82 @@
+"""PaganinLarge: the host-array interface of Paganin phase retrieval.
+
+Mirrors ``PaganinLarge`` of processing/paganin_large.py:42-284: constructor
+``PaganinLarge(n, nz, wavelength, voxelsize, distance, alpha)``,
+``retrieve(intensity, chunks)`` with intensity a host (ntheta, nz, n) or
+(nz, n) float32 array and ``chunks = [CHUNK_NZ, CHUNK_N]`` (validated with
+the reference's assertions, :124-134), returning a host float32 array of the
+same shape (an internal pinned buffer, overwritten by the next call), and
+``free()``.
+
+The reference streams strips of a host spectrum through a 40 GB card; here
+the frames move to HBM in batches bounded by ``gpu_budget_bytes`` and the
+device kernels of ``Paganin`` run on them (the chunk sizes only bound the
+reference's GPU strips and do not change the result).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from model_mosaic_amd._carriers import pinned_empty, require_gpu
+from model_mosaic_amd.processing.paganin import Paganin
+
+DEFAULT_GPU_BUDGET = 96 << 30
+
+
+class PaganinLarge:
+    """Paganin phase retrieval for host-resident intensity stacks."""
+
+    def __init__(self, n, nz, wavelength, voxelsize, distance, alpha, gpu_budget_bytes: int = DEFAULT_GPU_BUDGET):
+        require_gpu()
+        self.n = int(n)
+        self.nz = int(nz)
+        self.wavelength = float(wavelength)
+        self.voxelsize = float(voxelsize)
+        self.distance = float(distance)
+        self.alpha = float(alpha)
+        self.gpu_budget_bytes = int(gpu_budget_bytes)
+        self._pg: Paganin | None = None
+        self._out = None
+
+    def _batch(self, ntheta: int) -> int:
+        per = 3 * self.nz * self.n * 4  # input, output and half a packed complex workspace frame: 12 B / pixel
+        return int(max(1, min(ntheta, self.gpu_budget_bytes // per)))
+
+    def _get(self, b: int) -> Paganin:
+        if self._pg is None or self._pg._ntheta < b:
+            if self._pg is not None:
+                self._pg.free()
+            self._pg = Paganin(self.n, self.nz, b, self.wavelength, self.voxelsize, self.distance, self.alpha)
+        return self._pg
+
+    def free(self):
+        if self._pg is not None:
+            self._pg.free()
+        self._pg = None
+        self._out = None
+
+    def retrieve(self, intensity, chunks):
+        """Host (ntheta, nz, n) or (nz, n) float32 -> host phase of the same shape (paganin_large.py:115)."""
+        x = intensity if isinstance(intensity, torch.Tensor) else torch.as_tensor(np.asarray(intensity))
+        added_dim = x.ndim == 2
+        if added_dim:
+            x = x[None]
+        ntheta, nz_in, n_in = x.shape
+        assert nz_in == self.nz and n_in == self.n, \
+            f"intensity shape (…, {nz_in}, {n_in}) mismatches ({self.nz}, {self.n})"
+        chunk_nz, chunk_n = (int(c) for c in chunks)
+        assert chunk_nz >= 1 and self.nz % chunk_nz == 0, f"CHUNK_NZ={chunk_nz} must divide nz={self.nz}"
+        assert chunk_n >= 1 and self.n % chunk_n == 0, f"CHUNK_N={chunk_n} must divide n={self.n}"
+        b = self._batch(ntheta)
+        pg = self._get(b)
+        shape = (ntheta, self.nz, self.n)
+        if self._out is None or tuple(self._out.shape) != shape:
+            self._out = pinned_empty(shape, torch.float32)
+        for t0 in range(0, ntheta, b):
+            t1 = min(ntheta, t0 + b)
+            d = x[t0:t1].to(pg.device, dtype=torch.float32).contiguous()
+            self._out[t0:t1].copy_(pg.retrieve(d))
+        torch.cuda.synchronize(pg.device)
+        res = self._out.numpy()
+        return res[0] if added_dim else res
