@@ -9,8 +9,9 @@ same shape (an internal pinned buffer, overwritten by the next call), and
 ``free()``.
 
 The reference streams strips of a host spectrum through a 40 GB card; here
-the frames move to HBM in batches bounded by ``gpu_budget_bytes`` and the
-device kernels of ``Paganin`` run on them (the chunk sizes only bound the
+the frames move to HBM in batches bounded by ``gpu_budget_bytes`` (several
+batches go through a ``StreamPipe``, copies overlapping the retrieval) and
+the device kernels of ``Paganin`` run on them (the chunk sizes only bound the
 reference's GPU strips and do not change the result).
 """
 from __future__ import annotations
@@ -20,6 +21,7 @@ import torch
 
 from model_mosaic_amd._carriers import pinned_empty, require_gpu
 from model_mosaic_amd.processing.paganin import Paganin
+from model_mosaic_amd.processing.pipeline import StreamPipe
 
 DEFAULT_GPU_BUDGET = 96 << 30
 
@@ -73,10 +75,26 @@ class PaganinLarge:
         shape = (ntheta, self.nz, self.n)
         if self._out is None or tuple(self._out.shape) != shape:
             self._out = pinned_empty(shape, torch.float32)
-        for t0 in range(0, ntheta, b):
-            t1 = min(ntheta, t0 + b)
-            d = x[t0:t1].to(pg.device, dtype=torch.float32).contiguous()
-            self._out[t0:t1].copy_(pg.retrieve(d))
-        torch.cuda.synchronize(pg.device)
+        nb = (ntheta + b - 1) // b
+        if nb == 1:
+            self._out.copy_(pg.retrieve(x.to(pg.device, dtype=torch.float32).contiguous()))
+            torch.cuda.synchronize(pg.device)
+        else:  # frame ranges streamed: uploads and downloads overlap the retrieval of the range between
+            shape = (b, self.nz, self.n)
+            pipe = StreamPipe(pg.device, shape, torch.float32, shape, torch.float32)
+            out = self._out
+
+            def load(k, pin):
+                t0, t1 = k * b, min(ntheta, (k + 1) * b)
+                pin[: t1 - t0].copy_(x[t0:t1])
+                return pin[: t1 - t0]
+
+            def compute(k, d, res):
+                return pg.retrieve(d, out=res[: d.shape[0]])
+
+            def store(k, r):
+                out[k * b: k * b + r.shape[0]].copy_(r)
+
+            pipe.run(nb, load, compute, store)
         res = self._out.numpy()
         return res[0] if added_dim else res

@@ -10,8 +10,9 @@ internal cached pinned output, overwritten by the next call.  ``DT`` (new)
 applies the exact adjoint with the same interface.
 
 The reference streams strips through a 40 GB card; here the frames move to
-HBM in batches bounded by ``gpu_budget_bytes`` and the separable two-pass
-kernels of libmosaic_hip.so run on them.
+HBM in batches bounded by ``gpu_budget_bytes`` (several batches go through a
+``StreamPipe``, copies overlapping the propagation) and the separable
+two-pass kernels of libmosaic_hip.so run on them.
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ import numpy as np
 import torch
 
 from model_mosaic_amd._carriers import pinned_empty, require_gpu
+from model_mosaic_amd.processing.pipeline import StreamPipe
 from model_mosaic_amd.processing.propagation import Propagation
 
 DEFAULT_GPU_BUDGET = 96 << 30
@@ -91,12 +93,27 @@ class PropagationLarge:
         b = self._batch(ntheta)
         prop = self._get_prop(b)
         out = self._get_out(ntheta)
-        for t0 in range(0, ntheta, b):
-            t1 = min(ntheta, t0 + b)
-            x = psi_t[t0:t1].to(prop.device, dtype=torch.complex64).contiguous()
-            y = prop.DT(x, j) if adj else prop.D(x, j)
-            out[t0:t1].copy_(y)
-        torch.cuda.synchronize(prop.device)
+        op = prop.DT if adj else prop.D
+        nb = (ntheta + b - 1) // b
+        if nb == 1:
+            out.copy_(op(psi_t.to(prop.device, dtype=torch.complex64).contiguous(), j))
+            torch.cuda.synchronize(prop.device)
+        else:  # frame ranges streamed: uploads and downloads overlap the propagation of the range between
+            shape = (b, self.nz, self.n)
+            pipe = StreamPipe(prop.device, shape, torch.complex64, shape, torch.complex64)
+
+            def load(k, pin):
+                t0, t1 = k * b, min(ntheta, (k + 1) * b)
+                pin[: t1 - t0].copy_(psi_t[t0:t1])
+                return pin[: t1 - t0]
+
+            def compute(k, x, res):
+                return op(x, j, out=res[: x.shape[0]])
+
+            def store(k, r):
+                out[k * b: k * b + r.shape[0]].copy_(r)
+
+            pipe.run(nb, load, compute, store)
         res = out.numpy()
         return res[0] if added_dim else res
 

@@ -10,8 +10,10 @@ pinned object buffer.
 
 On the reference the chunks bound GPU memory of a 40 GB card.  With 288 GB of
 HBM the MI355X path is device resident: the whole z range is moved once per
-call (in z batches only when one batch would exceed ``gpu_budget_bytes``) and
-computed by the same kernels as ``TomoReal``.  The chunk arguments are still
+call and computed by the same kernels as ``TomoReal``; when the volume
+exceeds ``gpu_budget_bytes`` it is streamed in z slabs through a
+``StreamPipe`` (upload of slab k+1 and download of slab k-1 overlap the
+transform of slab k).  The chunk arguments are still
 validated as in the reference (AssertionError for a CHUNK_N that does not
 divide n, processing/tomo_large.py:430), and a CHUNK_THETA that does not divide
 ntheta is rejected too (the reference would silently skip the tail angles of
@@ -29,6 +31,7 @@ import numpy as np
 import torch
 
 from model_mosaic_amd._carriers import pinned_empty, require_gpu
+from model_mosaic_amd.processing.pipeline import StreamPipe
 from model_mosaic_amd.processing.tomo import TomoReal
 
 DEFAULT_GPU_BUDGET = 96 << 30  # bytes of HBM one call may hold
@@ -122,15 +125,27 @@ class TomoLargeReal:
         zb = self._zbatch(nz)
         tomo = self._get_tomo(zb)
         sino = self._get_sino(nz)
-        for z0 in range(0, nz, zb):
-            z1 = min(nz, z0 + zb)
-            x = obj_t[z0:z1].to(tomo.device, dtype=torch.float32)
-            s = tomo.R(x)
-            if z0 == 0 and z1 == nz:
-                sino.copy_(s)
-            else:
-                sino[:, z0:z1].copy_(s)
-        torch.cuda.synchronize(tomo.device)
+        nb = (nz + zb - 1) // zb
+        if nb == 1:
+            sino.copy_(tomo.R(obj_t.to(tomo.device, dtype=torch.float32)))
+            torch.cuda.synchronize(tomo.device)
+            return sino.numpy()
+        # z slabs streamed: upload of slab k+1 and download of slab k-1 overlap the transform of slab k
+        pipe = StreamPipe(tomo.device, (zb, self.n, self.n), torch.float32, (self.ntheta, zb, self.n), torch.float32)
+
+        def load(k, pin):
+            z0, z1 = k * zb, min(nz, (k + 1) * zb)
+            v = pin[: z1 - z0]
+            v.copy_(obj_t[z0:z1])
+            return v
+
+        def compute(k, x, out):
+            return tomo.R(x, out=out.view(-1)[: self.ntheta * x.shape[0] * self.n].view(self.ntheta, x.shape[0], self.n))
+
+        def store(k, res):
+            sino[:, k * zb: k * zb + res.shape[1]].copy_(res)
+
+        pipe.run(nb, load, compute, store)
         return sino.numpy()
 
     def RT(self, sino, chunks):
@@ -145,9 +160,24 @@ class TomoLargeReal:
         zb = self._zbatch(nz, cplx=not is_real)
         tomo = self._get_tomo(zb)
         out = self._get_obj_rt(nz, dt)
-        for z0 in range(0, nz, zb):
-            z1 = min(nz, z0 + zb)
-            y = s_np[:, z0:z1].to(tomo.device, dtype=dt).contiguous()
-            out[z0:z1].copy_(tomo.RT(y))
-        torch.cuda.synchronize(tomo.device)
+        nb = (nz + zb - 1) // zb
+        if nb == 1:
+            out.copy_(tomo.RT(s_np.to(tomo.device, dtype=dt).contiguous()))
+            torch.cuda.synchronize(tomo.device)
+            return out.numpy()
+        pipe = StreamPipe(tomo.device, (self.ntheta, zb, self.n), dt, (zb, self.n, self.n), dt)
+
+        def load(k, pin):
+            z0, z1 = k * zb, min(nz, (k + 1) * zb)
+            v = pin.view(-1)[: self.ntheta * (z1 - z0) * self.n].view(self.ntheta, z1 - z0, self.n)
+            v.copy_(s_np[:, z0:z1])
+            return v
+
+        def compute(k, y, res):
+            return tomo.RT(y, out=res.view(-1)[: y.shape[1] * self.n * self.n].view(y.shape[1], self.n, self.n))
+
+        def store(k, res):
+            out[k * zb: k * zb + res.shape[0]].copy_(res)
+
+        pipe.run(nb, load, compute, store)
         return out.numpy()

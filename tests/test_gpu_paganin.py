@@ -70,3 +70,16 @@ def test_paganin_vs_large():
         lrg.retrieve(x, [5, n])
     with pytest.raises(AssertionError):
         lrg.retrieve(x, [nz, 7])
+
+
+def test_large_streams_frame_batches():
+    """A GPU budget of two frames makes PaganinLarge stream 5 frames in 3 batches through the three-stream
+    pipeline; frames are paired differently than in one batch, so the match is to FP32 rounding."""
+    from model_mosaic_amd.processing.paganin import Paganin
+    from model_mosaic_amd.processing.paganin_large import PaganinLarge
+    n, nz = 64, 32
+    x = (0.5 + 0.5 * np.random.default_rng(2).random((5, nz, n))).astype(np.float32)
+    lrg = PaganinLarge(n, nz, WL, VOX, 0.1, ALPHA, gpu_budget_bytes=2 * 3 * nz * n * 4 + 1)
+    assert lrg._batch(5) == 2
+    ref = Paganin(n, nz, 5, WL, VOX, 0.1, ALPHA).retrieve(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert rel(lrg.retrieve(x, [nz, n]), ref) < 1e-6
