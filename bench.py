@@ -164,6 +164,33 @@ def run_paganin(torch, reps=5):
             "moved_GBs": round(3 * alg / dt / 1e9, 1)}
 
 
+def run_large(torch, reps=2):
+    """TomoLargeReal.R + RT of a host (numpy) 64 x 2048 x 2048 volume at 2048 angles, PCIe included: one
+    device batch, and 4 z slabs streamed through the three-stream pipeline (budget of 16 slices)."""
+    import numpy as np
+
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.processing.tomo_large import TomoLargeReal
+    nz, n, nt = 64, 2048, 2048
+    th = phantom.angles(nt)
+    obj = np.random.default_rng(0).standard_normal((nz, n, n), dtype=np.float32)
+    res = {"config": f"host float32 volume {nz}x{n}x{n}, {nt} angles, R then RT, PCIe-inclusive"}
+    for name, zb in (("one_batch", nz), ("streamed_16", 16)):
+        probe = TomoLargeReal(n, th, n)
+        tl = TomoLargeReal(n, th, n, gpu_budget_bytes=zb * probe._slice_bytes(False) + 1)
+        sino = tl.R(obj, [n, nt, n]).copy()
+        tl.RT(sino, [n, nt, n])
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sino = tl.R(obj, [n, nt, n])
+            tl.RT(sino, [n, nt, n])
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = {"s_per_RT_R": round(dt, 4), "slices_per_s": round(nz / dt, 1)}
+        tl.free()
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_model(torch, steps=3, warmup=1):
     import numpy as np
 
@@ -343,6 +370,7 @@ def main():
     ap.add_argument("--no-mosaic", action="store_true")
     ap.add_argument("--no-fbp", action="store_true")
     ap.add_argument("--no-paganin", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--only-mosaic", action="store_true", help="run only the uneven mosaic leg (testing)")
     ap.add_argument("--gather-zp", type=int, default=2, choices=(1, 2),
                     help="z slices per gather patch element (TomoReal gather_zp)")
@@ -480,6 +508,13 @@ def main():
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
 
+    large = None
+    if not a.no_large:
+        print("[bench] TomoLargeReal R+RT from host arrays", file=sys.stderr, flush=True)
+        large = run_large(torch)
+        print(f"[bench] host R+RT {large['one_batch']['slices_per_s']:.0f} slices/s one batch, "
+              f"{large['streamed_16']['slices_per_s']:.0f} slices/s in 16-slice slabs", file=sys.stderr, flush=True)
+
     pag = None
     if not a.no_paganin:
         print("[bench] Paganin retrieval, 128 frames 2048^2", file=sys.stderr, flush=True)
@@ -540,6 +575,7 @@ def main():
         "model_fwd_bwd": model,
         "fresnel_D_DT": fres,
         "paganin": pag,
+        "large_host_R_RT": large,
         "mosaic": mosaic,
         "fbp_filter": fbp,
         "cpu_baseline": cpu,
