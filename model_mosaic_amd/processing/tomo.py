@@ -25,7 +25,8 @@ class TomoReal:
     projection angles in radians (stored as float32, like the reference).
     ``fp64_tiles`` accumulates the adjoint scatter tiles in FP64.
     ``gather_zp`` (1 or 2) is the number of z slices the forward gather stages
-    per patch element (2, the default, is the faster variant on gfx950).
+    per patch element for tiles of width <= 32 (2, the default, is the faster
+    variant on gfx950; wider tiles always stage one slice).
     """
 
     def __init__(self, n, nz, theta, fp64_tiles: bool = False, gather_zp: int = 2):
