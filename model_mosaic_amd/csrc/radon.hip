@@ -68,12 +68,14 @@ __host__ __device__ __forceinline__ constexpr int patch_pitch(int w) {
   return w >= 64 ? 76 : (w >= 32 ? 44 : (w >= 16 ? 28 : 20));
 }
 
-// Gather kernel shapes, both 512 lanes with two samples each and at most 128
-// VGPRs, so two workgroups (16 waves) share a CU and one's staging loads
-// overlap the other's taps.  Root class (tiles wider than 32): float2 patch of
-// one z slice (72 x 76, 43 KB: a float4 patch of two slices would take 87.5 KB
-// and hold the CU alone).  Leaf class (tiles of width <= 32): float4 patch of
-// two slices (40 x 44, 28 KB) unless the plan asks for one slice.
+// Gather kernel shapes; each holds two workgroups per CU, so that one's
+// staging loads overlap the other's taps.  Root class (tiles wider than 32):
+// 512 lanes with two samples each (<= 128 VGPRs, 8 waves) and a float2 patch
+// of one z slice (72 x 76, 43 KB: a float4 patch of two slices would take
+// 87.5 KB and hold the CU alone).  Leaf class (tiles of width <= 32): 1024
+// lanes with one sample each (<= 64 VGPRs, 16 waves: twice the waves per CU
+// of the root shape) and a float4 patch of two slices (40 x 44, 28 KB) unless
+// the plan asks for one slice.
 //   PE: patch capacity (the sink slot for idle staging lanes is PE itself);
 //   STG: staging slots per lane (4-column blocks x rows over NT lanes).
 template <int WMAX, int NT_>
@@ -83,9 +85,10 @@ struct GatherShape {
   static constexpr int PE = (WMAX + 2 * M) * patch_pitch(WMAX);  // even: the 16-byte reads stay inside
   static constexpr int STG = (((WMAX + 2 * M + 3) / 4 + 1) * 4 * (WMAX + 2 * M) + NT - 1) / NT;
   static_assert(SMAX % NT == 0, "samples per lane");
+  static constexpr int WAVES_PER_SIMD = NT == SMAX ? 8 : 4;  // the register budget: 64 or 128 VGPRs
 };
 using GatherRoot = GatherShape<ROOT, SMAX / 2>;
-using GatherLeaf = GatherShape<ROOT / 2, SMAX / 2>;
+using GatherLeaf = GatherShape<ROOT / 2, SMAX>;
 // z slices per patch element of a gather item of width w (plan flag zp for the
 // leaf class); the lane colouring of the plan follows it
 __host__ __device__ __forceinline__ constexpr int gather_zp(int w, int zp_flag) {
@@ -327,7 +330,7 @@ __host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int i
 }
 
 template <int ZP, class S>
-__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(S::WAVES_PER_SIMD, S::WAVES_PER_SIMD)))
 void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const Item* __restrict__ items,
                   const int* __restrict__ samples, const unsigned short* __restrict__ order,
                   const double2* __restrict__ dirs, int n, int H, int nz, long long fzs, int zchunk, float a,
