@@ -131,12 +131,15 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
   // kernel argument keep the stores global (a pointer read back from LDS would make them flat
   // stores, which the LDS reads of the next element then wait for)
   __shared__ long long rowoff[R2];
+  // with one transform (two rows) per workgroup, two workgroups share each 128-byte line of a block
+  // (4 rows x 32 bytes): keep them adjacent in the XCD-grouped order, so the line is merged in one L2
+  const unsigned wg = Sh::C == 1 ? xcd_group(blockIdx.x, gridDim.x, 2) : blockIdx.x;
   int c, t;
   Sh::map(threadIdx.x, c, t);
   const int half = n / 2;
   const long long npairs = (long long)nz * half;
   if (threadIdx.x < R2) {
-    const long long pr = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
+    const long long pr = (long long)wg * Sh::C + (threadIdx.x >> 1);
     long long ro = -1;
     if (pr < npairs) {
       const int zz = (int)(pr / half), yy = 2 * (int)(pr - (long long)zz * half) + (threadIdx.x & 1);
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __res
     }
     rowoff[threadIdx.x] = ro;
   }
-  const long long pair = (long long)blockIdx.x * Sh::C + c;
+  const long long pair = (long long)wg * Sh::C + c;
   const bool valid = pair < npairs;
   int z = 0, y0 = 0;
   if (valid) {
@@ -937,10 +940,13 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
   // plane (-1: none).  Offsets from the kernel arguments keep the loads global; a pointer read back
   // from LDS made them flat loads that each waited before the next.
   __shared__ long long rowoff[R2];
+  // with one transform (two rows) per workgroup, two workgroups share each 128-byte line of a block
+  // (4 rows x 32 bytes): keep them adjacent in the XCD-grouped order, so the line is merged in one L2
+  const unsigned wg = Sh::C == 1 ? xcd_group(blockIdx.x, gridDim.x, 2) : blockIdx.x;
   const int half = n / 2;
   const long long nunits = CPLX ? (long long)nz * n : (long long)nz * half;
   if (threadIdx.x < R2) {
-    const long long u = (long long)blockIdx.x * Sh::C + (threadIdx.x >> 1);
+    const long long u = (long long)wg * Sh::C + (threadIdx.x >> 1);
     const int which = threadIdx.x & 1;
     long long ro = -1;
     if (u < nunits) {
@@ -1004,7 +1010,7 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
   }
   __syncthreads();  // the transform's exchanges reuse the staging area
   fft_reg<L, Sh::TPC, true>(v, x, t, tw);
-  const long long unit = (long long)blockIdx.x * Sh::C + c;
+  const long long unit = (long long)wg * Sh::C + c;
   if (unit >= nunits) return;
   int z, y;
   if (CPLX) {
@@ -1632,19 +1638,32 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
                      p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + tw_offset(L));
 }
 
-template <int L, bool INV>
-static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
+// Workgroups of fewer than four columns (long transforms, or the split knob)
+// read part of every 32-byte row piece of a block: the 4 / C workgroups of a
+// block are kept adjacent in the XCD-grouped order (xg = 4 / C, the number of
+// workgroups per z plane rounded up to a multiple of it, the spare ones idle)
+// so that they meet in one L2 and each 128-byte line is fetched once.
+template <int L, bool INV, int BATCH>
+static void launch_col_shape(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
+  using Sh = ColShapeB<L, BATCH>;
+  static_assert(Sh::C == 1 || Sh::C == 2 || Sh::C % 4 == 0, "workgroups cover whole blocks or block parts");
+  constexpr int G = Sh::C < 4 ? 4 / Sh::C : 1;
   const int ncols = p->n + 1;
   const int2* supp = INV ? p->ssupp : p->gsupp;
-  if (p->tune[MM_RADON_TUNE_COL_SPLIT] && ColShapeB<L, kRowBatch>::C == 2) {
-    using Sh = ColShapeB<L, kRowBatch>;
-    hipLaunchKernelGGL((col_fft<L, INV, kRowBatch>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz),
-                       dim3(Sh::NT), 0, st, buf, ncols, p->zs, supp, p->tw + tw_offset(L), 2);
-  } else {
-    using Sh = ColShape<L>;
-    hipLaunchKernelGGL((col_fft<L, INV>), dim3((unsigned)((ncols + Sh::C - 1) / Sh::C), (unsigned)nz), dim3(Sh::NT),
-                       0, st, buf, ncols, p->zs, supp, p->tw + tw_offset(L), 1);
+  const int gx = ((ncols + Sh::C - 1) / Sh::C + G - 1) / G * G;
+  hipLaunchKernelGGL((col_fft<L, INV, BATCH>), dim3((unsigned)gx, (unsigned)nz), dim3(Sh::NT), 0, st, buf, ncols,
+                     p->zs, supp, p->tw + tw_offset(L), G);
+}
+
+template <int L, bool INV>
+static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
+  if constexpr (ColShapeB<L, kRowBatch>::C == 2) {
+    if (p->tune[MM_RADON_TUNE_COL_SPLIT]) {
+      launch_col_shape<L, INV, kRowBatch>(p, buf, nz, st);
+      return;
+    }
   }
+  launch_col_shape<L, INV, kBatchElems>(p, buf, nz, st);
 }
 
 template <int L>

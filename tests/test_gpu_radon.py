@@ -76,7 +76,9 @@ def test_sizes_vs_fp64(tomo_cls, n, nt, span, nz, zp):
     assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
 
 
-@pytest.mark.parametrize("n,nt,nz", [(96, 37, 3), (192, 50, 2), (768, 129, 5), (1536, 64, 2)])
+@pytest.mark.parametrize("n,nt,nz", [(96, 37, 3), (192, 50, 2), (768, 129, 5), (1536, 64, 2),
+                                     # long transforms: two-column / one-pair workgroups kept paired per XCD
+                                     (3072, 24, 1), (4096, 16, 1)])
 def test_sizes_3x2k_vs_fp64(tomo_cls, n, nt, nz):
     """Detector widths 3 * 2^k (the mosaic tiles' 1536 and 3072): radix-12 FFT passes, non-power-of-two
     Fourier grid wrap-around in the gather and scatter."""
