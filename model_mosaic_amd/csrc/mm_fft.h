@@ -128,19 +128,21 @@ __host__ __device__ __forceinline__ void dft(float2* v) {
   }
 }
 
-// Lengths: 2^k (16 points per thread, radix-16 passes) and 3 * 2^k (12 points
-// per thread: one radix-12 pass, then radix-4 passes and a radix-2 for an odd
-// leftover).
+// Lengths: 2^k (16 points per thread, radix-16 passes) and 3 * 2^k (24 points
+// per thread: one radix-12 pass of two butterflies per thread, then radix-8
+// passes and a radix-4 or radix-2 for the leftover; a 6144-point transform
+// makes three LDS exchanges).
 __host__ __device__ constexpr bool pow2c(int L) { return L > 0 && (L & (L - 1)) == 0; }
 __host__ __device__ constexpr bool fft_len_ok(int L) { return pow2c(L) || (L % 3 == 0 && pow2c(L / 3)); }
-__host__ __device__ constexpr int pt_of(int L) { return pow2c(L) ? 16 : 12; }
+__host__ __device__ constexpr int pt_of(int L) { return pow2c(L) ? 16 : 24; }
 __host__ __device__ constexpr int next_radix(int L, int NS) {
-  return pow2c(L) ? (L / NS >= 16 ? 16 : L / NS) : (NS == 1 ? 12 : ((L / NS) % 4 == 0 ? 4 : 2));
+  return pow2c(L) ? (L / NS >= 16 ? 16 : L / NS)
+                  : (NS == 1 ? 12 : ((L / NS) % 8 == 0 ? 8 : ((L / NS) % 4 == 0 ? 4 : 2)));
 }
 
 // ---- register-resident Stockham FFT -----------------------------------------
 // Thread t (0 <= t < TPC = L / PT) of a transform holds v[m] = x[t + m TPC],
-// m < PT (PT = pt_of(L): 16, or 12 for 3 * 2^k), before and after the
+// m < PT (PT = pt_of(L): 16, or 24 for 3 * 2^k), before and after the
 // transform.  A radix-R pass (R divides PT) runs the thread's PT / R
 // butterflies j = t + b TPC on its own registers (butterfly j reads
 // x[j + s L / R] = v[b + s PT / R]); its outputs go to LDS at their Stockham
@@ -163,7 +165,9 @@ struct RegShape {
   static_assert(L >= 16 && L <= kMaxFFT && fft_len_ok(L), "length 2^k or 3 * 2^k in [16, 16384]");
   static constexpr int PT = pt_of(L);
   static constexpr int TPC = L / PT;
-  static constexpr int C0 = BATCH > L ? BATCH / L : 1;
+  // interleaved shapes take whole groups of 4 transforms (the map below; column kernels: whole blocks)
+  static constexpr int CB = BATCH > L ? BATCH / L : 1;
+  static constexpr int C0 = INTERLEAVE && L / PT >= 16 && CB >= 4 ? CB / 4 * 4 : CB;
   static constexpr int C = C0 * TPC > 1024 ? 1024 / TPC : C0;
   static constexpr int NT = C * TPC;
   static constexpr int S0 = L + L / 16 + XTRA;  // > lpad(L - 1) + XTRA

@@ -1646,7 +1646,6 @@ static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t s
 template <int L, bool INV, int BATCH>
 static void launch_col_shape(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
   using Sh = ColShapeB<L, BATCH>;
-  static_assert(Sh::C == 1 || Sh::C == 2 || Sh::C % 4 == 0, "workgroups cover whole blocks or block parts");
   constexpr int G = Sh::C < 4 ? 4 / Sh::C : 1;
   const int ncols = p->n + 1;
   const int2* supp = INV ? p->ssupp : p->gsupp;
