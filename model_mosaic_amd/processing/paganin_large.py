@@ -42,7 +42,7 @@ class PaganinLarge:
         self._out = None
 
     def _batch(self, ntheta: int) -> int:
-        per = 3 * self.nz * self.n * 4  # input, output and half a packed complex workspace frame: 12 B / pixel
+        per = 5 * self.nz * self.n * 4  # input and output (two of each when streamed), half a complex workspace frame
         return int(max(1, min(ntheta, self.gpu_budget_bytes // per)))
 
     def _get(self, b: int) -> Paganin:

@@ -53,7 +53,7 @@ class PropagationLarge:
         self._psi = None
 
     def _batch(self, ntheta: int) -> int:
-        per = 2 * self.nz * self.n * 8
+        per = 4 * self.nz * self.n * 8  # input and output frame, two of each when streamed
         return int(max(1, min(ntheta, self.gpu_budget_bytes // per)))
 
     def _get_prop(self, b: int) -> Propagation:

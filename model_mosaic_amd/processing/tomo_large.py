@@ -69,9 +69,9 @@ class TomoLargeReal:
     def _slice_bytes(self, cplx: bool) -> int:
         n, nt = self.n, self.ntheta
         per = 2 * n * (n + 2) * 8 + nt * (n // 2 + 1) * 8   # plan workspace per slice
-        per += 4 * n * n + 4 * nt * n                        # obj + sino slabs
+        per += 2 * (4 * n * n + 4 * nt * n)                  # obj + sino slabs, two of each when streamed
         if cplx:
-            per += 2 * n * (n + 2) * 8 + nt * (n // 2 + 1) * 8 + 4 * n * n + 4 * nt * n
+            per += 2 * n * (n + 2) * 8 + nt * (n // 2 + 1) * 8 + 2 * (4 * n * n + 4 * nt * n)
         return per
 
     def _zbatch(self, nz: int, cplx: bool = False) -> int:

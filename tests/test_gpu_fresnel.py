@@ -154,7 +154,7 @@ def test_large_streams_frame_batches_bitwise(g):
     wl, vox, dist = float(g["wavelength"]), float(g["voxelsize"]), g["dist"]
     rng = np.random.default_rng(4)
     psi = np.exp(1j * rng.uniform(0, 2, (5, nz, n))).astype(np.complex64)
-    lrg = PropagationLarge(n, nz, wl, vox, dist, gpu_budget_bytes=2 * 2 * nz * n * 8 + 1)
+    lrg = PropagationLarge(n, nz, wl, vox, dist, gpu_budget_bytes=2 * 4 * nz * n * 8 + 1)
     assert lrg._batch(5) == 2
     ref = Propagation(n, nz, 5, len(dist), wl, vox, dist)
     for j in range(len(dist)):

@@ -79,7 +79,7 @@ def test_large_streams_frame_batches():
     from model_mosaic_amd.processing.paganin_large import PaganinLarge
     n, nz = 64, 32
     x = (0.5 + 0.5 * np.random.default_rng(2).random((5, nz, n))).astype(np.float32)
-    lrg = PaganinLarge(n, nz, WL, VOX, 0.1, ALPHA, gpu_budget_bytes=2 * 3 * nz * n * 4 + 1)
+    lrg = PaganinLarge(n, nz, WL, VOX, 0.1, ALPHA, gpu_budget_bytes=2 * 5 * nz * n * 4 + 1)
     assert lrg._batch(5) == 2
     ref = Paganin(n, nz, 5, WL, VOX, 0.1, ALPHA).retrieve(torch.from_numpy(x).cuda()).cpu().numpy()
     assert rel(lrg.retrieve(x, [nz, n]), ref) < 1e-6
