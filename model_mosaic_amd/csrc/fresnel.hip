@@ -57,8 +57,13 @@ __device__ __forceinline__ int frame_row(int m, int t) {
   return (m >= Sh::PT / 4 && m < 3 * Sh::PT / 4) ? t + (m - Sh::PT / 4) * Sh::TPC : -1;
 }
 
+// Column-pass shape: long transforms take 8192 points per workgroup (two
+// columns of a 4096-point transform, 70 KB of LDS), so two workgroups share a
+// CU and one's loads and stores overlap the other's two transforms (4 %
+// faster D than four columns per workgroup, which holds a CU alone); short
+// ones keep 16384 points (many columns per workgroup).
 template <int L>
-using ColShape = RegShape<L, kBatchElems>;
+using ColShape = RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>;
 
 // Column pass: frames (b, nz, n); blockIdx.y = frame, C columns per block.
 // ADJ = false: out = crop IFFT K FFT pad (in);  ADJ = true: out = pad^T IFFT conj(K) FFT crop^T (in).
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
-  // four column groups share each 128-byte line of a row: keep them on one XCD
+  // the column groups that share each 128-byte line of a row run on one XCD
   const unsigned lw = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int bx = (int)(lw % gridDim.x), frame = (int)(lw / gridDim.x);
   const int col = bx * Sh::C + c;

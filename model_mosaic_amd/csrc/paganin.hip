@@ -53,11 +53,11 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rows_fwd(const float* __restric
 
 // Column pass: C columns of one pair (blockIdx.y); FFT along y, times H / (n nz), IFFT.
 template <int L>
-__global__ __launch_bounds__((RegShape<L, kBatchElems>::NT)) void cols(float2* __restrict__ ws, int n,
+__global__ __launch_bounds__((RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>::NT)) void cols(float2* __restrict__ ws, int n,
                                                                        const float* __restrict__ fy2,
                                                                        const float* __restrict__ fx2, float ca,
                                                                        float scale, const float2* __restrict__ tw) {
-  using Sh = RegShape<L, kBatchElems>;
+  using Sh = RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>;  // long columns: two workgroups per CU
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
@@ -151,7 +151,7 @@ static void launch_rows(const mm_paganin* p, const float* in, float* out, int nf
 
 template <int L>
 static void launch_cols(const mm_paganin* p, int nframes, hipStream_t st) {
-  using Sh = RegShape<L, kBatchElems>;
+  using Sh = RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>;
   const float scale = (float)(1.0 / ((double)p->n * p->nz));
   hipLaunchKernelGGL(cols<L>, dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)((nframes + 1) / 2)),
                      dim3(Sh::NT), 0, st, p->ws, p->n, p->fy2, p->fx2, p->ca, scale, p->tw + tw_offset(L));
