@@ -115,16 +115,23 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
   return ((long long)(col >> 2) * twon + row) * 4 + (col & 3);
 }
 
+// Workgroup shapes of the row and line passes (8192 points per workgroup: two
+// workgroups per CU at 70 KB of LDS).
+template <int L>
+using R1Shape = RowReg<L>;
+template <int L>
+using LineShape = RowReg<L>;
+
 // Row pass of R: two object rows per complex FFT.  Row y of slice z, times
 // phi_y[y] phi_x[x], zero-padded to 2n (object centred), is transformed and
 // the half spectrum kx in [0, n] is written to half-spectrum row n/2 + y.  The
 // inputs go straight into the register FFT; the spectrum passes once through
 // LDS to pair Z_k with Z_{L-k} and to write the half-spectrum blocks in order.
 template <int L>
-__global__ __launch_bounds__(RowReg<L>::NT) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
+__global__ __launch_bounds__(R1Shape<L>::NT) void r1_rows_rfft(const float* __restrict__ obj, float2* __restrict__ fde,
                                                            const float* __restrict__ phx, const float* __restrict__ phy,
                                                            int n, int nz, long long fzs, const float2* __restrict__ tw) {
-  using Sh = RowReg<L>;
+  using Sh = R1Shape<L>;
   constexpr int R2 = 2 * Sh::C;
   __shared__ float2 lds[Sh::LDS];
   // offset of destination row r of this workgroup in the half spectrum (-1: none); offsets from the
@@ -487,10 +494,10 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
 // sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).  Inputs and
 // outputs go straight between HBM and the register FFT.
 template <int L>
-__global__ __launch_bounds__(RowReg<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
+__global__ __launch_bounds__(LineShape<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
                                                            const int* __restrict__ sig, int H, long long nrows, int nz,
                                                            float scale, const float2* __restrict__ tw) {
-  using Sh = RowReg<L>;
+  using Sh = LineShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
@@ -561,11 +568,11 @@ __device__ __forceinline__ void rt1_emit_real(const float2 (&v)[Sh::PT], int t, 
 }
 
 template <int L, bool CPLX>
-__global__ __launch_bounds__(RowReg<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
+__global__ __launch_bounds__(LineShape<L>::NT) void rt1_line_fft(const void* __restrict__ sino_v, float2* __restrict__ yre,
                                                            float2* __restrict__ yim, const int* __restrict__ sig, int H,
                                                            long long nrows, int nz, float scale,
                                                            const float2* __restrict__ tw) {
-  using Sh = RowReg<L>;
+  using Sh = LineShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
@@ -1632,7 +1639,7 @@ static int dmalloc(void** p, size_t nb, size_t& bytes, const char* what) {
 
 template <int L>
 static void launch_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
-  using Sh = RowReg<L>;
+  using Sh = R1Shape<L>;
   const long long units = (long long)nz * (p->n / 2);
   hipLaunchKernelGGL(r1_rows_rfft<L>, dim3((unsigned)((units + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, obj,
                      p->fde, p->phx, p->phy, p->n, nz, p->zs, p->tw + tw_offset(L));
@@ -1667,7 +1674,7 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
 
 template <int L>
 static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
-  using Sh = RowReg<L>;
+  using Sh = LineShape<L>;
   const long long rows = (long long)p->ntheta * nz;
   hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
                      sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + tw_offset(L));
@@ -1675,7 +1682,7 @@ static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
 
 template <int L>
 static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
-  using Sh = RowReg<L>;
+  using Sh = LineShape<L>;
   const long long rows = (long long)p->ntheta * nz;
   const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
   if (cplx)
