@@ -115,10 +115,13 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
   return ((long long)(col >> 2) * twon + row) * 4 + (col & 3);
 }
 
-// Workgroup shapes of the row and line passes (8192 points per workgroup: two
-// workgroups per CU at 70 KB of LDS).
+// Workgroup shapes of the row and line passes.  The forward row pass takes
+// 16384 points per workgroup (four transforms = 8 rows at L = 4096: 256-byte
+// runs per half-spectrum block, one workgroup per CU; 4 % faster than two
+// transforms); the C2R rows and the line passes take 8192 (two workgroups per
+// CU at 70 KB of LDS; 16384 points made them slower).
 template <int L>
-using R1Shape = RowReg<L>;
+using R1Shape = RegShape<L, kBatchElems, 0, false>;
 template <int L>
 using LineShape = RowReg<L>;
 
