@@ -129,20 +129,24 @@ __host__ __device__ __forceinline__ void dft(float2* v) {
 }
 
 // Lengths: 2^k (16 points per thread, radix-16 passes) and 3 * 2^k (24 points
-// per thread: one radix-12 pass of two butterflies per thread, then radix-8
-// passes and a radix-4 or radix-2 for the leftover; a 6144-point transform
-// makes three LDS exchanges).
+// per thread from 768 up: one radix-12 pass of two butterflies per thread,
+// then radix-8 passes and a radix-4 or radix-2 for the leftover, so a
+// 6144-point transform makes three LDS exchanges; 12 points per thread below
+// 768: radix-12, then radix-4 / radix-2).
 __host__ __device__ constexpr bool pow2c(int L) { return L > 0 && (L & (L - 1)) == 0; }
 __host__ __device__ constexpr bool fft_len_ok(int L) { return pow2c(L) || (L % 3 == 0 && pow2c(L / 3)); }
-__host__ __device__ constexpr int pt_of(int L) { return pow2c(L) ? 16 : 24; }
+// 3 * 2^k: 24 points per thread from 768 points up (radix-8 passes after the
+// radix-12 one), 12 below (short transforms keep 16+ threads per transform).
+__host__ __device__ constexpr int pt_of(int L) { return pow2c(L) ? 16 : (L >= 768 ? 24 : 12); }
 __host__ __device__ constexpr int next_radix(int L, int NS) {
   return pow2c(L) ? (L / NS >= 16 ? 16 : L / NS)
-                  : (NS == 1 ? 12 : ((L / NS) % 8 == 0 ? 8 : ((L / NS) % 4 == 0 ? 4 : 2)));
+                  : (NS == 1 ? 12
+                             : ((L / NS) % 8 == 0 && pt_of(L) % 8 == 0 ? 8 : ((L / NS) % 4 == 0 ? 4 : 2)));
 }
 
 // ---- register-resident Stockham FFT -----------------------------------------
 // Thread t (0 <= t < TPC = L / PT) of a transform holds v[m] = x[t + m TPC],
-// m < PT (PT = pt_of(L): 16, or 24 for 3 * 2^k), before and after the
+// m < PT (PT = pt_of(L): 16, or 12 / 24 for 3 * 2^k), before and after the
 // transform.  A radix-R pass (R divides PT) runs the thread's PT / R
 // butterflies j = t + b TPC on its own registers (butterfly j reads
 // x[j + s L / R] = v[b + s PT / R]); its outputs go to LDS at their Stockham

@@ -116,12 +116,13 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
 }
 
 // Workgroup shapes of the row and line passes.  The forward row pass takes
-// 16384 points per workgroup (four transforms = 8 rows at L = 4096: 256-byte
-// runs per half-spectrum block, one workgroup per CU; 4 % faster than two
-// transforms); the C2R rows and the line passes take 8192 (two workgroups per
-// CU at 70 KB of LDS; 16384 points made them slower).
+// 16384 points per workgroup for power-of-two lengths from 4096 up (four
+// transforms = 8 rows at L = 4096: 256-byte runs per half-spectrum block, one
+// workgroup per CU; 4 % faster than two transforms); 3 * 2^k lengths and the
+// C2R rows and the line passes take 8192 (two workgroups per CU at 70 KB of
+// LDS; 16384 points made them slower).
 template <int L>
-using R1Shape = RegShape<L, kBatchElems, 0, false>;
+using R1Shape = RegShape<L, (pow2c(L) && L >= 4096 ? kBatchElems : kRowBatch), 0, false>;
 template <int L>
 using LineShape = RowReg<L>;
 
@@ -1696,10 +1697,13 @@ static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, h
                        rows, nz, p->C0s, p->tw + tw_offset(L));
 }
 
+// the fused kernel needs one thread layout for the filter (2L) and the line (L) transforms
+__host__ __device__ constexpr bool fbp_fusable(int L) { return 2 * L <= kMaxFFT && pt_of(L) == pt_of(2 * L); }
+
 template <int L>
 static void launch_rt1_fbp(const mm_radon* p, const float* sino, const float* G, const float* A, const float* B,
                            int nz, hipStream_t st) {
-  if constexpr (2 * L <= kMaxFFT) {
+  if constexpr (fbp_fusable(L)) {
     using Sh = typename FbpLineShape<L>::S1;
     const long long rows = (long long)p->ntheta * nz;
     hipLaunchKernelGGL(rt1_fbp_line_fft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, sino,
@@ -1988,8 +1992,24 @@ int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float
   hipStream_t st = (hipStream_t)stream;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
-  MM_DISPATCH_FFT(p->n, (launch_rt1_fbp<LL>(p, sino, G, A, B, nz, st)));
-  if ((rc = check_launch("rt1_fbp_line_fft")) || (rc = mark(p, 4, 1, st))) return rc;
+  if (fbp_fusable(p->n)) {
+    MM_DISPATCH_FFT(p->n, (launch_rt1_fbp<LL>(p, sino, G, A, B, nz, st)));
+    if ((rc = check_launch("rt1_fbp_line_fft"))) return rc;
+  } else {
+    // a length whose filter transform holds a different number of points per thread (n = 384): filter a
+    // copy in the second representative buffer (unused by real sinograms, large enough), then the line pass
+    if (!p->gh2 && (rc = dmalloc((void**)&p->gh2, sizeof(float2) * radon_gh_elems(p->n, p->nz_max, p->ntheta),
+                                 p->bytes, "filtered sinogram")))
+      return rc;
+    float* tmp = reinterpret_cast<float*>(p->gh2);
+    const size_t nb = sizeof(float) * (size_t)p->ntheta * nz * p->n;
+    if (hipMemcpyAsync(tmp, sino, nb, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return fail(MM_ERR_HIP, "mm_radon_adj_fbp: sinogram copy");
+    if ((rc = mm_fbp_filter_2n(tmp, G, A, B, (long long)p->ntheta * nz, p->n, stream))) return rc;
+    MM_DISPATCH_FFT(p->n, (launch_rt1<LL>(p, tmp, false, nz, st)));
+    if ((rc = check_launch("rt1_line_fft"))) return rc;
+  }
+  if ((rc = mark(p, 4, 1, st))) return rc;
   return radon_adj_tail(p, false, obj, nz, st);
 }
 

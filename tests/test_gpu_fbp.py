@@ -73,7 +73,8 @@ def test_weights_changed_in_place_rebuild_the_2n_tables():
     assert rel(d.cpu().numpy(), fbp.filter(x, w.cpu().numpy().astype(np.float64))) < 1e-5
 
 
-@pytest.mark.parametrize("n,nt,nz", [(64, 24, 3), (96, 40, 2), (256, 100, 4), (2048, 64, 2)])
+@pytest.mark.parametrize("n,nt,nz", [(64, 24, 3), (96, 40, 2), (256, 100, 4), (2048, 64, 2),
+                                     (384, 40, 2)])  # 384: filter and line transforms differ in points per thread
 def test_RT_filtered_equals_filter_then_RT(n, nt, nz):
     """TomoReal.RT_filtered (filter fused into RT's line pass) = FBPFilter.filter then TomoReal.RT to FP32
     rounding (same transforms; the compiler may contract the fused kernel's arithmetic differently), and
