@@ -41,6 +41,11 @@ int twiddle_bank(const float2** out) {
     };
     for (int L = 2; L <= kMaxFFT; L *= 2) fill(L);
     for (int L = 12; L <= kMaxFFT; L *= 2) fill(L);
+    for (int e = 0; e < 16; ++e)
+      for (int k = 0; k < 16; ++k) {
+        const double a = -2.0 * M_PI * (double)(e * k) / 256.0;
+        host[kTwGrid256 + 16 * e + k] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
     float2* d = nullptr;
     hipError_t e = hipMalloc(&d, host.size() * sizeof(float2));
     if (e != hipSuccess) return fail(MM_ERR_OOM, std::string("twiddle bank: ") + hipGetErrorString(e));

@@ -49,8 +49,12 @@ __host__ __device__ __forceinline__ float2 mul_mi(float2 a) {
 // For every FFT length L = 2^k in [2, kMaxFFT] and L = 3 * 2^k in [12, kMaxFFT]
 // the bank holds exp(-2 pi i t / L), t in [0, L), at element offset
 // tw_offset(L) (computed in FP64 on the host).
+// After them, at kTwGrid256, the grid exp(-2 pi i e k / 256) at 16 e + k (e, k < 16):
+// the twiddles of a radix-16 pass after one radix-16 pass, with the 16 values of
+// k for one e in one 128-byte line.
 constexpr int kMaxFFT = 16384;
-constexpr long kTwiddleBank = 2L * kMaxFFT + 2L * (3 * kMaxFFT / 4);  // elements
+constexpr long kTwGrid256 = 2L * kMaxFFT + 2L * (3 * kMaxFFT / 4);
+constexpr long kTwiddleBank = kTwGrid256 + 256;  // elements
 __host__ __device__ inline constexpr long tw_offset(int L) {
   return (L & (L - 1)) == 0 ? (long)L : 2L * kMaxFFT + L;  // 3 * 2^k: [2 kMax + L, 2 kMax + 2 L)
 }

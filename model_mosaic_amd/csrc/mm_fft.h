@@ -249,10 +249,23 @@ __host__ __device__ __forceinline__ void dft16(float2* a) {
   }
 }
 
-// u[s] *= W_{NS R}^{s k}, s < R: table reads for the powers 1, 2, 4, 8, the
-// rest as products of at most three of them.
+// u[s] *= W_{NS R}^{s k}, s < R.  The radix-16 pass after a radix-16 pass
+// (NS R = 256) reads all 15 from the W_256 grid (one 128-byte line per e for a
+// wave's 16 values of k); other passes read the powers 1, 2, 4, 8 from the
+// length's table and form the rest as products of at most three of them.
 template <int L, int R, int NS, bool INV>
 __device__ __forceinline__ void twiddle(float2* u, int k, const float2* __restrict__ tw) {
+  if constexpr (R == 16 && NS == 16 && pow2c(L)) {
+    // W_256^(e k) straight from the grid (tw points at tw_offset(L) of the bank)
+    const float2* g = tw + (kTwGrid256 - tw_offset(L)) + k;
+#pragma unroll
+    for (int e = 1; e < 16; ++e) {
+      float2 w = g[16 * e];
+      if (INV) w.y = -w.y;
+      u[e] = cmul(u[e], w);
+    }
+    return;
+  }
   constexpr int STEP = L / (NS * R);
   auto ld = [&](int e) {
     float2 w = tw[e * k * STEP];
