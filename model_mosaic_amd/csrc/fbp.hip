@@ -15,7 +15,7 @@ namespace mm {
 namespace fbp {
 
 template <int NE>
-__global__ __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict__ sino, const float* __restrict__ w,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict__ sino, const float* __restrict__ w,
                                                              long long rows, int n, const float2* __restrict__ tw) {
   using Sh = RowReg<NE>;
   __shared__ float2 lds[Sh::LDS];
@@ -33,13 +33,13 @@ __global__ __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict_
     const int j = min(max(i - pad, 0), n - 1);  // edge replication
     v[m] = make_float2(has_a ? pa[j] : 0.f, has_b ? pb[j] : 0.f);
   }
-  fft_reg<NE, Sh::TPC, false>(v, lds + c * Sh::S, t, tw);
+  fft_reg<NE, Sh::TPC, false, 1, false>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     v[m] = cscale(v[m], w[i <= NE / 2 ? i : NE - i]);
   }
-  fft_reg<NE, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
+  fft_reg<NE, Sh::TPC, true, 1, false>(v, lds + c * Sh::S, t, tw);
   const float s = 1.0f / (float)NE;
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(RowReg<NE>::NT) void filter_rows(float* __restrict_
 // outputs out of the last), two rows ride in one complex transform as above.
 // G (with the 1/L), A and B come from the host in FP64.
 template <int L>
-__global__ __launch_bounds__(RowReg<L>::NT) void filter_rows_2n(float* __restrict__ sino,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void filter_rows_2n(float* __restrict__ sino,
                                                                 const float* __restrict__ G,
                                                                 const float* __restrict__ A,
                                                                 const float* __restrict__ B, long long rows,
@@ -85,13 +85,13 @@ __global__ __launch_bounds__(RowReg<L>::NT) void filter_rows_2n(float* __restric
     const int i = t + m * Sh::TPC;
     v[m] = m < MH ? make_float2(has_a ? pa[i] : 0.f, has_b ? pb[i] : 0.f) : make_float2(0.f, 0.f);
   }
-  fft_reg<L, Sh::TPC, false>(v, lds + c * Sh::S, t, tw);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int i = t + m * Sh::TPC;
     v[m] = cscale(v[m], G[i <= L / 2 ? i : L - i]);
   }
-  fft_reg<L, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, lds + c * Sh::S, t, tw);
 #pragma unroll
   for (int m = 0; m < MH; ++m) {
     const int i = t + m * Sh::TPC;

@@ -70,7 +70,7 @@ using ColShape = RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>;
 // Both transforms and the chirp run on the register FFT; only the fold-add of
 // the adjoint mirror padding reads other threads' points (through LDS).
 template <int L, bool ADJ>
-__global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __restrict__ in, float2* __restrict__ out,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __restrict__ in, float2* __restrict__ out,
                                                             int n, int nz, bool mirror, const float2* __restrict__ K,
                                                             const float2* __restrict__ tw) {
   using Sh = ColShape<L>;
@@ -93,14 +93,14 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
     const int sr = ADJ ? frame_row<Sh>(m, t) : pad_src<Sh>(m, t, mirror);
     v[m] = (live && sr >= 0) ? src[(long long)sr * n] : make_float2(0.f, 0.f);
   }
-  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
     v[m] = cmul(v[m], kk);
   }
-  fft_reg<L, Sh::TPC, true>(v, x, t, tw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   if (ADJ && mirror) {
 #pragma unroll
     for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(ColShape<L>::NT) void col_pass(const float2* __rest
 
 // Row pass: rows of length n, C rows per block, nrows = b * nz.
 template <int L, bool ADJ>
-__global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restrict__ in, float2* __restrict__ out,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restrict__ in, float2* __restrict__ out,
                                                           int n, long long nrows, bool mirror,
                                                           const float2* __restrict__ K, const float2* __restrict__ tw) {
   using Sh = RowReg<L>;
@@ -140,14 +140,14 @@ __global__ __launch_bounds__(RowReg<L>::NT) void row_pass(const float2* __restri
     const int sx = ADJ ? frame_row<Sh>(m, t) : pad_src<Sh>(m, t, mirror);
     v[m] = (valid && sx >= 0) ? src[sx] : make_float2(0.f, 0.f);
   }
-  fft_reg<L, Sh::TPC, false>(v, x, t, tw);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     float2 kk = K[t + m * Sh::TPC];
     if (ADJ) kk.y = -kk.y;
     v[m] = cmul(v[m], kk);
   }
-  fft_reg<L, Sh::TPC, true>(v, x, t, tw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   float2* dst = out + (valid ? row : 0) * n;
   if (ADJ && mirror) {
 #pragma unroll

@@ -193,6 +193,12 @@ struct RegShape {
   }
 };
 
+// Register budget of kernels that run two transforms or two stages (Fresnel,
+// Paganin, FBP): at least 4 waves per SIMD (<= 128 VGPRs), so that two
+// 512-lane workgroups share a CU; left free, the compiler keeps both
+// transforms' twiddle loads in flight and drops to one workgroup per CU.
+#define MM_TWO_FFT_BUDGET __attribute__((amdgpu_waves_per_eu(4)))
+
 // Row-pass shape of the register FFT: 8192 points per workgroup, plain mapping.
 template <int L, int XTRA = 0>
 using RowReg = RegShape<L, kRowBatch, XTRA, false>;
@@ -253,9 +259,9 @@ __host__ __device__ __forceinline__ void dft16(float2* a) {
 // (NS R = 256) reads all 15 from the W_256 grid (one 128-byte line per e for a
 // wave's 16 values of k); other passes read the powers 1, 2, 4, 8 from the
 // length's table and form the rest as products of at most three of them.
-template <int L, int R, int NS, bool INV>
+template <int L, int R, int NS, bool INV, bool GRID>
 __device__ __forceinline__ void twiddle(float2* u, int k, const float2* __restrict__ tw) {
-  if constexpr (R == 16 && NS == 16 && pow2c(L)) {
+  if constexpr (GRID && R == 16 && NS == 16 && pow2c(L)) {
     // W_256^(e k) straight from the grid (tw points at tw_offset(L) of the bank)
     const float2* g = tw + (kTwGrid256 - tw_offset(L)) + k;
 #pragma unroll
@@ -293,7 +299,7 @@ __device__ __forceinline__ void twiddle(float2* u, int k, const float2* __restri
 }
 
 // One radix-R pass of fft_reg (NS = product of the radices already applied).
-template <int L, int R, int NS, int TPC, bool INV, int PT>
+template <int L, int R, int NS, int TPC, bool INV, int PT, bool GRID>
 __device__ __forceinline__ void reg_pass(float2 (&v)[PT], float2* __restrict__ x, int t,
                                          const float2* __restrict__ tw) {
   static_assert(PT % R == 0, "the radix divides the points per thread");
@@ -305,7 +311,7 @@ __device__ __forceinline__ void reg_pass(float2 (&v)[PT], float2* __restrict__ x
     const int j = t + b * TPC;
 #pragma unroll
     for (int s = 0; s < R; ++s) u[b][s] = v[b + s * BPT];
-    if constexpr (NS > 1) twiddle<L, R, NS, INV>(u[b], j % NS, tw);
+    if constexpr (NS > 1) twiddle<L, R, NS, INV, GRID>(u[b], j % NS, tw);
     if constexpr (R == 16)
       dft16<INV>(u[b]);
     else if constexpr (R == 12)
@@ -336,15 +342,17 @@ __device__ __forceinline__ void reg_pass(float2 (&v)[PT], float2* __restrict__ x
 // Transform of the PT points each of the TPC threads of one transform holds
 // in v (natural order in and out), x = that transform's LDS (RegShape stride).
 // Contains barriers when L > PT: every thread of the workgroup must call it.
-// Unnormalised in both directions (INV only conjugates the kernel).
-template <int L, int TPC, bool INV, int NS = 1, int PT>
+// Unnormalised in both directions (INV only conjugates the kernel).  GRID = false
+// keeps the product twiddles in every pass (kernels that run two transforms:
+// the grid's 15 loads per pass raise their registers past two workgroups per CU).
+template <int L, int TPC, bool INV, int NS = 1, bool GRID = true, int PT>
 __device__ __forceinline__ void fft_reg(float2 (&v)[PT], float2* __restrict__ x, int t,
                                         const float2* __restrict__ tw) {
   static_assert(PT == pt_of(L) && TPC * PT == L, "register FFT shape");
   if constexpr (NS < L) {
     constexpr int R = next_radix(L, NS);
-    reg_pass<L, R, NS, TPC, INV, PT>(v, x, t, tw);
-    fft_reg<L, TPC, INV, NS * R>(v, x, t, tw);
+    reg_pass<L, R, NS, TPC, INV, PT, GRID>(v, x, t, tw);
+    fft_reg<L, TPC, INV, NS * R, GRID>(v, x, t, tw);
   }
 }
 

@@ -24,7 +24,7 @@ namespace paganin {
 
 // Row pass, forward: unit u = (pair p, row y); transform of (a[y], b[y]).
 template <int L>
-__global__ __launch_bounds__(RowReg<L>::NT) void rows_fwd(const float* __restrict__ in, float2* __restrict__ ws,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void rows_fwd(const float* __restrict__ in, float2* __restrict__ ws,
                                                          int nz, int nframes, long long units,
                                                          const float2* __restrict__ tw) {
   using Sh = RowReg<L>;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(RowReg<L>::NT) void rows_fwd(const float* __restric
 
 // Column pass: C columns of one pair (blockIdx.y); FFT along y, times H / (n nz), IFFT.
 template <int L>
-__global__ __launch_bounds__((RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>::NT)) void cols(float2* __restrict__ ws, int n,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__((RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>::NT)) void cols(float2* __restrict__ ws, int n,
                                                                        const float* __restrict__ fy2,
                                                                        const float* __restrict__ fx2, float ca,
                                                                        float scale, const float2* __restrict__ tw) {
@@ -70,14 +70,14 @@ __global__ __launch_bounds__((RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) v[m] = live ? b[(long long)(t + m * Sh::TPC) * n] : make_float2(0.f, 0.f);
-  fft_reg<L, Sh::TPC, false>(v, lds + c * Sh::S, t, tw);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, lds + c * Sh::S, t, tw);
   const float kx2 = live ? fx2[col] : 0.f;
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const float h = scale * __builtin_amdgcn_rcpf(fmaf(ca, fy2[t + m * Sh::TPC] + kx2, 1.0f));
     v[m] = cscale(v[m], h);
   }
-  fft_reg<L, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, lds + c * Sh::S, t, tw);
   if (!live) return;
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) b[(long long)(t + m * Sh::TPC) * n] = v[m];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__((RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>
 
 // Row pass, inverse: IFFT along x, clip at 1e-6, -log, unpack the two frames.
 template <int L>
-__global__ __launch_bounds__(RowReg<L>::NT) void rows_inv(const float2* __restrict__ ws, float* __restrict__ out,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void rows_inv(const float2* __restrict__ ws, float* __restrict__ out,
                                                          int nz, int nframes, long long units,
                                                          const float2* __restrict__ tw) {
   using Sh = RowReg<L>;

@@ -649,7 +649,7 @@ struct FbpLineShape {  // C2 filter transforms of length 2L and C1 = 2 C2 lines 
 };
 
 template <int L>
-__global__ __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1_fbp_line_fft(const float* __restrict__ sino,
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1_fbp_line_fft(const float* __restrict__ sino,
                                                                   float2* __restrict__ yre,
                                                                   const int* __restrict__ sig, int H,
                                                                   long long nrows, int nz, float scale,
@@ -681,13 +681,13 @@ __global__ __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1_fbp_line_fft(cons
     const int i = t2 + m * S2::TPC;
     v[m] = m < MH ? make_float2(has_a ? pa[i] : 0.f, has_b ? pb[i] : 0.f) : make_float2(0.f, 0.f);
   }
-  fft_reg<2 * L, S2::TPC, false>(v, lds + c2 * S2::S, t2, tw2);
+  fft_reg<2 * L, S2::TPC, false, 1, false>(v, lds + c2 * S2::S, t2, tw2);
 #pragma unroll
   for (int m = 0; m < S2::PT; ++m) {
     const int i = t2 + m * S2::TPC;
     v[m] = cscale(v[m], G[i <= L ? i : 2 * L - i]);
   }
-  fft_reg<2 * L, S2::TPC, true>(v, lds + c2 * S2::S, t2, tw2);
+  fft_reg<2 * L, S2::TPC, true, 1, false>(v, lds + c2 * S2::S, t2, tw2);
   __syncthreads();  // the filter's last exchange is read out
 #pragma unroll
   for (int m = 0; m < MH; ++m) {
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1_fbp_line_fft(cons
     w[m] = (r & 1) ? u : make_float2(-u.x, -u.y);
   }
   __syncthreads();  // the transform's exchanges reuse the staging area
-  fft_reg<L, S1::TPC, false>(w, lds + c1 * S1::S, t1, tw1);
+  fft_reg<L, S1::TPC, false, 1, false>(w, lds + c1 * S1::S, t1, tw1);
   if (valid) rt1_emit_real<S1>(w, t1, sg, yre + row * H, scale);
 }
 
