@@ -2192,6 +2192,47 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
         }
     }
   }
+  {  // the slots the kernel reads give every tile point in the half plane exactly the taps of the
+     // item's entries that land on it: (entry, i0, i1) from the lists against a direct enumeration
+    const int twon = 2 * n;
+    std::vector<std::vector<int>> want(SNP), got(SNP);
+    for (size_t ii = 0; ii < S.items.size(); ++ii) {
+      const SItem& it = S.items[ii];
+      for (auto& v : want) v.clear();
+      for (auto& v : got) v.clear();
+      for (int e = 0; e < it.ne; ++e) {
+        const int code = S.entries[it.eoff + e];
+        int rx, ry;
+        G.centre(code >> 1, rx, ry);
+        const int cx = (code & 1) ? -rx : rx, cy = (code & 1) ? -ry : ry;
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          const int lr = wrapc(cy - M + i1, n) - it.y0;
+          if (lr < 0 || lr >= STS) continue;
+          for (int i0 = 0; i0 < TAPS; ++i0) {
+            const int raw = pmod(cx - M + i0, twon), lc = raw - it.x0;
+            if (raw <= n && lc >= 0 && lc < STS) want[lr * STS + lc].push_back((e * TAPS + i1) * TAPS + i0);
+          }
+        }
+      }
+      std::vector<int> seen(SNP, 0);
+      for (int l = 0; l < SNP; ++l) {
+        const int pt = S.perm[ii * SNP + l];
+        if (seen[pt]++) return bad("scatter lane permutation");
+        const int np = (int)((it.wnp >> (8 * (l / 64))) & 0xFFu);
+        for (int j = 0; j < 2 * np; ++j) {
+          const unsigned u = S.lists[it.loff + (long long)(j / 2) * 2 * SNP + 2 * l + (j & 1)];
+          const int e = (int)(u & 0x1FFu), t = (int)(u >> 9);
+          if (t != (int)SL_IDLE_TAP) got[pt].push_back((e * TAPS + t / TAPS) * TAPS + t % TAPS);
+        }
+      }
+      for (int pt = 0; pt < SNP; ++pt) {
+        if (it.x0 + pt % STS > n) continue;  // never written
+        std::sort(want[pt].begin(), want[pt].end());
+        std::sort(got[pt].begin(), got[pt].end());
+        if (want[pt] != got[pt]) return bad("scatter taps of a point differ from its entries' taps");
+      }
+    }
+  }
   return MM_OK;
 }
 

@@ -54,8 +54,10 @@ def test_phantom_seeded_and_bounded():
 def test_radon_plan_tables_match_kernel_indexing(n):
     """Host-only: every gather staging offset lies inside a half-spectrum plane (tiles at the right
     edge of small grids reach past 2n and must wrap), every patch element is staged once, every tap
-    read stays inside the LDS patch, every scatter slot names a staged entry, the column passes cover every row the
-    gather reads and read no row the scatter leaves unwritten (dense and sparse angle sets)."""
+    read stays inside the LDS patch, every scatter slot names a staged entry, every written scatter point gets exactly
+    the taps of its item's entries that land on it (lists decoded as the kernel does, against a direct enumeration),
+    the column passes cover every row the gather reads and read no row the scatter leaves unwritten (dense and sparse
+    angle sets).
     import ctypes as C
 
     import numpy as np
