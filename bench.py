@@ -381,8 +381,8 @@ def main():
                     help="run a plain device copy of this many MiB first (PMC byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (R/RT shard over z with no "
-              "communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (R and RT shard over z with no "
+              "communication between GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

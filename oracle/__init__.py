@@ -1,6 +1,6 @@
 """CPU oracle for the USFFT Radon transform and Fresnel propagation.
 
-TEST INFRASTRUCTURE ONLY.  This package is the checker, never the product:
+TEST INFRASTRUCTURE ONLY.  This package checks results; it is never the product:
 only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product path (``model_mosaic_amd``) never
 imports, links or executes anything from here and fails loudly when its HIP

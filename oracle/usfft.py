@@ -16,8 +16,8 @@ Restates, operation by operation:
 Every function takes ``dtype``: ``np.float32`` reproduces the reference's FP32
 arithmetic order (the "reference restatement"; not bit-exact, because cuFFT and
 the CUDA fast intrinsics round differently from NumPy), ``np.float64`` is the
-same algorithm in exact arithmetic (the FP64 oracle of the north-star accuracy
-bars).
+same algorithm in exact arithmetic (the FP64 oracle that the product's
+accuracy bars are measured against).
 
 Parity pins (tests/test_oracle_usfft.py):
   * gather + rfft2 against the exact polar NUDFT computed by brute force in
