@@ -57,7 +57,7 @@ def test_radon_plan_tables_match_kernel_indexing(n):
     read stays inside the LDS patch, every scatter slot names a staged entry, every written scatter point gets exactly
     the taps of its item's entries that land on it (lists decoded as the kernel does, against a direct enumeration),
     the column passes cover every row the gather reads and read no row the scatter leaves unwritten (dense and sparse
-    angle sets).
+    angle sets)."""
     import ctypes as C
 
     import numpy as np
