@@ -911,6 +911,7 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
   const int l0 = st.x0 + (p % STS), l1 = st.y0 + (p / STS);
   if (l0 > n) return;
   float2 acc = make_float2(0.f, 0.f);
+#pragma unroll 8  // eight loads in flight; the summation order stays slot order
   for (int sl = st.slot0; sl < st.slot0 + st.nslots; ++sl) {
     const float2 v = partial[((long long)sl * nz + z) * SNP + p];
     acc.x += v.x;
