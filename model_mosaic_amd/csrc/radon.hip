@@ -731,7 +731,7 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(FbpLineShape<L>::S1::NT) void rt1
 // partial tiles that scatter_reduce sums.
 //
 // A point's list is private to its thread, so it lives in registers: two
-// 16-bit slots per VGPR, in a 32 + 8 element vector indexed by the (uniform)
+// 16-bit slots per VGPR, in a 32 + 16 element vector indexed by the (uniform)
 // loop counter, which the compiler lowers to relative register addressing.
 // Each entry has one 80-byte LDS row: its SZC staged values, its offset F and
 // its (theta, k | mirror) key.  A tap reads the row as five 16-byte pieces
@@ -747,12 +747,12 @@ constexpr int SNP = STS * STS;   // points = threads per work item
 constexpr int SWAVES = SNP / 64; // waves per work item
 constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
 constexpr int SE_MAX = 496;      // entry rows in LDS per work item
-constexpr int SL_MAX = 80;       // list slots per point per work item
+constexpr int SL_MAX = 96;       // list slots per point per work item (48 VGPRs; fewer split tiles than 80)
 constexpr int SL_CHUNK = SL_MAX - 8;  // taps per point per work item (room for idle steps)
 constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
 constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
-static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 8 && SWAVES <= 4 && SL_MAX / 2 < 256,
+static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 16 && SWAVES <= 4 && SL_MAX / 2 < 256,
               "slot encoding");
 
 struct __align__(16) SRow {
@@ -776,7 +776,7 @@ struct SplitTile {
 };
 
 typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
-typedef unsigned u32x8 __attribute__((ext_vector_type(SL_PAIRS_B)));
+typedef unsigned u32x16 __attribute__((ext_vector_type(SL_PAIRS_B)));  // power-of-two sizes stay in VGPRs
 
 template <typename Acc>
 __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
@@ -808,7 +808,7 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const int na = min(npairs, SL_PAIRS_A);
   const unsigned* lp = reinterpret_cast<const unsigned*>(lists + it.loff) + p;
   u32x32 la;
-  u32x8 lb;
+  u32x16 lb;
 #pragma unroll
   for (int jp = 0; jp < SL_PAIRS_A; ++jp) la[jp] = jp < npairs ? lp[jp * SNP] : 0u;
 #pragma unroll
