@@ -748,7 +748,7 @@ constexpr int SWAVES = SNP / 64; // waves per work item
 constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
 constexpr int SE_MAX = 496;      // entry rows in LDS per work item
 constexpr int SL_MAX = 96;       // list slots per point per work item (48 VGPRs; fewer split tiles than 80)
-constexpr int SL_CHUNK = SL_MAX - 8;  // taps per point per work item (room for idle steps)
+constexpr int SL_CHUNK = SL_MAX - 4;  // taps per point per work item (room for idle steps)
 constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
 constexpr int SZC = 8;           // z slices per staging round
 constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
