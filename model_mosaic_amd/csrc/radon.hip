@@ -497,6 +497,13 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
 // Line pass of R: rebuild the full line g[r] from its representatives,
 // sino = c1d . IFFT(c1d . g) . real / (4n)   (c1d[r] = (-1)^(r+1)).  Inputs and
 // outputs go straight between HBM and the register FFT.
+//
+// Two lines (angles theta, theta + 1 at one z) ride in one complex transform:
+// the real part of IFFT(a) is IFFT(Herm a), Herm a[r] = (a[r] + conj a[L - r]) / 2,
+// and a rebuilt line is Hermitian already except at r = 0 and r = L/2 (the
+// k = -n/2 and k = 0 samples), where Herm takes the real part.  So
+// IFFT(Herm a + i Herm b) has line a's result as its real part and line b's as
+// its imaginary part.
 template <int L>
 __global__ __launch_bounds__(LineShape<L>::NT) void r4_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
                                                            const int* __restrict__ sig, int H, long long nrows, int nz,
@@ -505,15 +512,16 @@ __global__ __launch_bounds__(LineShape<L>::NT) void r4_line_ifft(const float2* _
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
-  const long long row = (long long)blockIdx.x * Sh::C + c;
-  const bool valid = row < nrows;
-  const int sg = valid ? sig[row / nz] : 1;
-  const float2* src = gh + (valid ? row : 0) * H;
+  // rows (theta, z) and (theta + 1, z), theta even: the pairing does not depend on how z is batched
+  const long long q = (long long)blockIdx.x * Sh::C + c, th2 = q / nz;
+  const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
+  const bool va = ra < nrows, vb = rb < nrows;
+  const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+  const float2* srca = gh + (va ? ra : 0) * H;
+  const float2* srcb = gh + (vb ? rb : 0) * H;
   constexpr int half = L / 2;
-  float2 v[Sh::PT];
-#pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) {
-    const int r = t + m * Sh::TPC;
+  // line value at r from the representatives (k = r - L/2: |k| and a conjugation by the sign)
+  auto rebuild = [&](const float2* src, int sg, bool valid, int r) {
     const int d = r - half;
     int kk;
     bool cj;
@@ -526,27 +534,38 @@ __global__ __launch_bounds__(LineShape<L>::NT) void r4_line_ifft(const float2* _
     }
     float2 w = valid ? src[kk] : make_float2(0.f, 0.f);
     if (cj) w.y = -w.y;
+    if (r == 0 || r == half) w.y = 0.f;  // Hermitian part
+    return w;
+  };
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int r = t + m * Sh::TPC;
+    const float2 a = rebuild(srca, sga, va, r), b = rebuild(srcb, sgb, vb, r);
+    float2 w = make_float2(a.x - b.y, a.y + b.x);  // a + i b
     if (!(r & 1)) w = make_float2(-w.x, -w.y);
     v[m] = w;
   }
   fft_reg<L, Sh::TPC, true>(v, lds + c * Sh::S, t, tw);
-  if (valid) {
-    float* dst = sino + row * L;
+  float* dsta = sino + (va ? ra : 0) * L;
+  float* dstb = sino + (vb ? rb : 0) * L;
 #pragma unroll
-    for (int m = 0; m < Sh::PT; ++m) {
-      const int s = t + m * Sh::TPC;
-      const float w = v[m].x * scale;
-      dst[s] = (s & 1) ? w : -w;
-    }
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int s = t + m * Sh::TPC;
+    const float sc = (s & 1) ? scale : -scale;
+    if (va) dsta[s] = v[m].x * sc;
+    if (vb) dstb[s] = v[m].y * sc;
   }
 }
 
 // Line pass of RT: yhat = c1d . FFT(c1d . y), written at the representatives
 // with the fold weights (1/2 at k = 0 and k = n/2) times `scale`.  A complex
-// sinogram yields the two channels of its real and imaginary parts (which pair
-// yhat(r) with yhat(L - r), so that output passes through LDS; the real one is
-// written straight from registers).
-// Representatives of one real line from its transform (thread t holds r = t + m TPC):
+// sinogram yields the two channels of its real and imaginary parts; a real one
+// puts two rows (angles theta, theta + 1 at one z) into one transform as its
+// real and imaginary parts.  Either way the split pairs yhat(r) with
+// yhat(L - r), so the output passes through LDS.
+// Representatives of one real line from its own transform (used behind the FBP
+// filter; thread t holds r = t + m TPC):
 // yhat(r) = c1d(r) v, k = r - L/2 (sigma > 0; k = L/2 from r = 0, conjugated) or
 // k = L/2 - r (sigma < 0), fold weights 1/2 at k = 0 and k = L/2.
 template <class Sh>
@@ -580,29 +599,71 @@ __global__ __launch_bounds__(LineShape<L>::NT) void rt1_line_fft(const void* __r
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
+  float2* x = lds + c * Sh::S;
+  constexpr int half = L / 2;
+  if constexpr (!CPLX) {
+    // two real rows per transform, as its real and imaginary parts; each row's spectrum
+    // is (V[r] +- conj V[L - r]) / 2 (times -i for the imaginary one), V[0] splits into
+    // its real and imaginary part
+    // rows (theta, z) and (theta + 1, z), theta even: the pairing does not depend on how z is batched
+    const long long q = (long long)blockIdx.x * Sh::C + c, th2 = q / nz;
+    const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
+    const bool va = ra < nrows, vb = rb < nrows;
+    const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+    const float* pa = (const float*)sino_v + (va ? ra : 0) * L;
+    const float* pb = (const float*)sino_v + (vb ? rb : 0) * L;
+    float2 v[Sh::PT];
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int r = t + m * Sh::TPC;
+      float2 w = make_float2(va ? pa[r] : 0.f, vb ? pb[r] : 0.f);
+      if (!(r & 1)) w = make_float2(-w.x, -w.y);
+      v[m] = w;
+    }
+    fft_reg<L, Sh::TPC, false>(v, x, t, tw);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+    __syncthreads();
+    auto yhat = [&](int r) {
+      float2 w = x[lpad(r)];
+      return (r & 1) ? w : make_float2(-w.x, -w.y);
+    };
+    float2* da = yre + (va ? ra : 0) * H;
+    float2* db = yre + (vb ? rb : 0) * H;
+    for (int k = t; k < H; k += Sh::TPC) {
+      const float f = ((k == 0 || k == half) ? 0.5f : 1.0f) * scale;
+      if (k == half) {  // from r = 0 for either sign
+        const float2 v0 = yhat(0);
+        if (va) da[k] = make_float2(v0.x * f, 0.f);
+        if (vb) db[k] = make_float2(v0.y * f, 0.f);
+        continue;
+      }
+      if (va) {
+        const int r = sga > 0 ? half + k : half - k;
+        const float2 p = yhat(r), q = yhat(L - r);
+        da[k] = make_float2(0.5f * (p.x + q.x) * f, 0.5f * (p.y - q.y) * f);
+      }
+      if (vb) {
+        const int r = sgb > 0 ? half + k : half - k;
+        const float2 p = yhat(r), q = yhat(L - r);
+        db[k] = make_float2(0.5f * (p.y + q.y) * f, -0.5f * (p.x - q.x) * f);
+      }
+    }
+    return;
+  }
   const long long row = (long long)blockIdx.x * Sh::C + c;
   const bool valid = row < nrows;
   const long long rr = valid ? row : 0;
   const int sg = valid ? sig[rr / nz] : 1;
-  float2* x = lds + c * Sh::S;
-  constexpr int half = L / 2;
   float2 v[Sh::PT];
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int r = t + m * Sh::TPC;
-    float2 w;
-    if (CPLX)
-      w = valid ? ((const float2*)sino_v)[rr * L + r] : make_float2(0.f, 0.f);
-    else
-      w = make_float2(valid ? ((const float*)sino_v)[rr * L + r] : 0.f, 0.f);
+    float2 w = valid ? ((const float2*)sino_v)[rr * L + r] : make_float2(0.f, 0.f);
     if (!(r & 1)) w = make_float2(-w.x, -w.y);
     v[m] = w;
   }
   fft_reg<L, Sh::TPC, false>(v, x, t, tw);
-  if (!CPLX) {
-    if (valid) rt1_emit_real<Sh>(v, t, sg, yre + rr * H, scale);
-    return;
-  }
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
   __syncthreads();
@@ -1680,8 +1741,8 @@ static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
 template <int L>
 static void launch_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
   using Sh = LineShape<L>;
-  const long long rows = (long long)p->ntheta * nz;
-  hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
+  const long long rows = (long long)p->ntheta * nz, pairs = (long long)((p->ntheta + 1) / 2) * nz;  // two rows per transform
+  hipLaunchKernelGGL(r4_line_ifft<L>, dim3((unsigned)((pairs + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, p->gh,
                      sino, p->sig, p->H, rows, nz, 1.0f / (4.0f * p->n), p->tw + tw_offset(L));
 }
 
@@ -1689,13 +1750,12 @@ template <int L>
 static void launch_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
   using Sh = LineShape<L>;
   const long long rows = (long long)p->ntheta * nz;
-  const dim3 grid((unsigned)((rows + Sh::C - 1) / Sh::C));
-  if (cplx)
-    hipLaunchKernelGGL((rt1_line_fft<L, true>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H, rows,
-                       nz, p->C0s, p->tw + tw_offset(L));
-  else
-    hipLaunchKernelGGL((rt1_line_fft<L, false>), grid, dim3(Sh::NT), 0, st, sino, p->gh, p->gh2, p->sig, p->H,
-                       rows, nz, p->C0s, p->tw + tw_offset(L));
+  if (cplx)  // one complex row per transform
+    hipLaunchKernelGGL((rt1_line_fft<L, true>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st,
+                       sino, p->gh, p->gh2, p->sig, p->H, rows, nz, p->C0s, p->tw + tw_offset(L));
+  else  // two real rows per transform
+    hipLaunchKernelGGL((rt1_line_fft<L, false>), dim3((unsigned)((((p->ntheta + 1) / 2) * (long long)nz + Sh::C - 1) / Sh::C)), dim3(Sh::NT),
+                       0, st, sino, p->gh, p->gh2, p->sig, p->H, rows, nz, p->C0s, p->tw + tw_offset(L));
 }
 
 // the fused kernel needs one thread layout for the filter (2L) and the line (L) transforms
