@@ -111,7 +111,9 @@ int mm_radon_autotune(mm_radon* plan, const float* obj_dev, float* sino_dev, flo
 
 /* Host-only self-check (no counterpart in the reference; for tests on machines
  * without a GPU): builds the gather / scatter tables of an (n, theta) plan and
- * checks them against the kernels' index arithmetic.  MM_OK or MM_ERR_ARG with
+ * checks them against the kernels' index arithmetic, and that the scatter
+ * lists, decoded as the kernel decodes them, give every written grid point
+ * exactly the taps of the samples that land on it.  MM_OK or MM_ERR_ARG with
  * the first violation in mm_last_error(). */
 int mm_radon_plan_check(int n, int ntheta, const double* theta_host);
 
