@@ -79,28 +79,31 @@ def stage_bytes(n, nt):
 
 
 def cpu_baseline():
-    """Oracle (NumPy FP32 restatement) R+RT of one n=2048 slice, single thread."""
+    """Oracle (NumPy FP32 restatement) R+RT of one n=2048 slice, single thread (any BLAS / OpenMP pool
+    held to one thread; NumPy's FFT and element-wise work are single-threaded)."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from oracle import usfft
 
     n = 2048
     x = np.random.default_rng(0).standard_normal((1, n, n)).astype(np.float32)
     times = {}
-    for nt in (256, 512):
-        th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
-        t0 = time.perf_counter()
-        r = usfft.tomo_R(x, th, np.float32)
-        usfft.tomo_RT(r, th, dtype=np.float32)
-        times[nt] = time.perf_counter() - t0
-    per_slice = times[512] + (times[512] - times[256]) * (2048 - 512) / 256
+    with threadpool_limits(limits=1):
+        for nt in (512, 1024):
+            th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+            t0 = time.perf_counter()
+            r = usfft.tomo_R(x, th, np.float32)
+            usfft.tomo_RT(r, th, dtype=np.float32)
+            times[nt] = time.perf_counter() - t0
+    per_slice = times[1024] + (times[1024] - times[512]) * (2048 - 1024) / 512
     return {
         "value": 1.0 / per_slice,
         "unit": "slices/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle.usfft FP32 R+RT of 1 slice n=2048 at 256 angles ({times[256]:.1f} s) and 512 angles "
-                   f"({times[512]:.1f} s), linearly extrapolated in ntheta to 2048 angles ({per_slice:.1f} s/slice)"),
+        "sample": (f"oracle.usfft FP32 R+RT of 1 slice n=2048 at 512 angles ({times[512]:.1f} s) and 1024 angles "
+                   f"({times[1024]:.1f} s), linearly extrapolated in ntheta to 2048 angles ({per_slice:.1f} s/slice)"),
     }
 
 
