@@ -602,10 +602,10 @@ __global__ __launch_bounds__(LineShape<L>::NT) void rt1_line_fft(const void* __r
   float2* x = lds + c * Sh::S;
   constexpr int half = L / 2;
   if constexpr (!CPLX) {
-    // two real rows per transform, as its real and imaginary parts; each row's spectrum
-    // is (V[r] +- conj V[L - r]) / 2 (times -i for the imaginary one), V[0] splits into
-    // its real and imaginary part
-    // rows (theta, z) and (theta + 1, z), theta even: the pairing does not depend on how z is batched
+    // two real rows per transform, (theta, z) and (theta + 1, z) with theta even (the pairing does
+    // not depend on how z is batched), as its real and imaginary parts; each row's spectrum is
+    // (V[r] +- conj V[L - r]) / 2 (times -i for the imaginary one), V[0] splits into its real and
+    // imaginary part
     const long long q = (long long)blockIdx.x * Sh::C + c, th2 = q / nz;
     const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
     const bool va = ra < nrows, vb = rb < nrows;
@@ -640,13 +640,13 @@ __global__ __launch_bounds__(LineShape<L>::NT) void rt1_line_fft(const void* __r
       }
       if (va) {
         const int r = sga > 0 ? half + k : half - k;
-        const float2 p = yhat(r), q = yhat(L - r);
-        da[k] = make_float2(0.5f * (p.x + q.x) * f, 0.5f * (p.y - q.y) * f);
+        const float2 p = yhat(r), pm = yhat(L - r);
+        da[k] = make_float2(0.5f * (p.x + pm.x) * f, 0.5f * (p.y - pm.y) * f);
       }
       if (vb) {
         const int r = sgb > 0 ? half + k : half - k;
-        const float2 p = yhat(r), q = yhat(L - r);
-        db[k] = make_float2(0.5f * (p.y + q.y) * f, -0.5f * (p.x - q.x) * f);
+        const float2 p = yhat(r), pm = yhat(L - r);
+        db[k] = make_float2(0.5f * (p.y + pm.y) * f, -0.5f * (p.x - pm.x) * f);
       }
     }
     return;
