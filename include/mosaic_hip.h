@@ -46,7 +46,7 @@ int mm_version(void);
  * (R :344, RT :582).  eps = 1e-3, mu = -ln(eps)/(2 n^2), m = 4 taps each side
  * (processing/tomo.py:39-42), 2x oversampled Fourier grid.
  *
- * mm_radon_create: n a power of two in [16, 8192], nz_max >= 1,
+ * mm_radon_create: n = 2^k in [16, 8192] or 3 * 2^k in [96, 6144], nz_max >= 1,
  *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of
@@ -127,7 +127,8 @@ void mm_radon_destroy(mm_radon* plan);
  * and processing/propagation_large.py:35 PropagationLarge.D (:202); DT is the
  * exact adjoint of D (the reference has no DT, propagation_large.py:15).
  *
- * mm_fresnel_create: frame (nz, n), nz and n even powers of two in [8, 8192];
+ * mm_fresnel_create: frame (nz, n), nz and n each 2^k in [8, 8192] or 3 * 2^k in
+ *   [24, 6144];
  *   ndist distances (dist_host, metres), wavelength and voxelsize in metres.
  *   pad_mode 0 = symmetric mirror padding (processing/kernels.py:530-550, the
  *   reference), 1 = zero padding.  The chirp exp(-i pi lambda z f^2) is built in
@@ -198,7 +199,8 @@ int mm_abs2(const void* z_dev, float* out_dev, long long count, void* stream);
  * step3_propagation.py:175,223 (psi, D, |.|^2 per angle chunk, :94-110).
  * Tiles of equal (n, angles) share a forward-only Radon plan, tiles of equal
  * (n, nz) a Fresnel plan; all tiles share the workspaces (sized for the
- * largest).  delta[t] (nz, n, n) float32, out[t] (ntheta, nz, n) float32,
+ * largest).  Every n[t] must be a valid Radon width and every (nz[t], n[t])
+ * a valid Fresnel frame (see mm_radon_create / mm_fresnel_create).  delta[t] (nz, n, n) float32, out[t] (ntheta, nz, n) float32,
  * device pointers.
  * mm_mosaic_run enqueues the whole batch as one HIP graph (captured on first
  * use and again whenever the pointers change) on the batch's own stream,

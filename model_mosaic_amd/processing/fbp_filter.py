@@ -7,8 +7,9 @@ transformed back (1 / ne) and cropped to n, in place
 FFT of length ne (the weights are real and even, so the two real results come
 back as the real and imaginary parts), with the edge padding, the weights and
 the crop fused into one kernel (mm_fbp_filter): one read and one write of the
-sinogram.  The weights are computed on the host exactly as the reference does
-(calc_filter with the 12-point quadrature _wint, :48-126).
+sinogram.  The weights are built on the host (calc_filter_host): the ramp
+|t| is integrated by a 12-point piecewise-Lagrange rule (ramp_quadrature, the
+rule processing/fbp_filter.py:94-126 defines) and shaped by the named window.
 
 By default the filter runs on transforms of length 2 n instead of 4 n
 (mm_fbp_filter_2n): with h = irfft(w, 4 n) the edge-padded convolution is a
@@ -27,29 +28,94 @@ from model_mosaic_amd._carriers import as_device_tensor, require_gpu
 FILTERS = ("none", "ramp", "shepp", "cosine", "cosine2", "hamming", "hann", "parzen")
 
 
-def wint(order: int, t: np.ndarray) -> np.ndarray:
-    """Quadrature weights of the ramp |t| on the grid t (processing/fbp_filter.py:94-126): for each window
-    of `order` consecutive grid points, the integrals of x * p_k(x) and p_k(x) over the short intervals of
-    the window, p_k the Lagrange basis, averaged over the windows that cover a point; the last 40 values
-    are replaced by a linear taper."""
+def _window_moments(order: int):
+    """Integrals of the Lagrange basis on `order` equispaced nodes s_i = i / (order - 1) of [0, 1].
+
+    Returns (M0, M1), each (order, order - 1): M0[i, k] = int L_i(s) ds and M1[i, k] = int s L_i(s) ds over
+    the k-th gap [s_k, s_{k+1}].  L_i is evaluated in product form with barycentric weights and integrated
+    by Gauss-Legendre on each gap with enough points to be exact for its degree (order)."""
+    s_nodes = np.arange(order) / (order - 1)
+    gaps = s_nodes[:, None] - s_nodes[None, :]
+    np.fill_diagonal(gaps, 1.0)
+    bary = 1.0 / gaps.prod(axis=1)
+    gx, gw = np.polynomial.legendre.leggauss(order // 2 + 2)
+    M0 = np.empty((order, order - 1))
+    M1 = np.empty((order, order - 1))
+    for k in range(order - 1):
+        lo, hi = s_nodes[k], s_nodes[k + 1]
+        s = lo + (hi - lo) * 0.5 * (gx + 1.0)
+        ds = (hi - lo) * 0.5 * gw
+        basis = np.stack([bary[i] * np.prod(s[None, :] - np.delete(s_nodes, i)[:, None], axis=0)
+                          for i in range(order)])
+        M0[:, k] = basis @ ds
+        M1[:, k] = basis @ (ds * s)
+    return M0, M1
+
+
+def ramp_quadrature(t: np.ndarray, order: int = 12, taper: int = 40) -> np.ndarray:
+    """Weights q with sum_i q_i f(t_i) ~ int_{t_0}^{t_end} t f(t) dt on the increasing grid t: the integrand
+    of the ramp filter |t| (processing/fbp_filter.py:94-126 defines the rule).
+
+    Every run of `order` consecutive grid points is a window; on each window f is replaced by its
+    Lagrange interpolant, and the integral of t times that interpolant over each gap of the window is
+    exact (window moments M0, M1 scaled to the window: t = t_j + h s, h = t_{j+order-1} - t_j).  A gap
+    covered by c windows takes the mean of their c values.  The last `taper` weights are then replaced by
+    the line through the origin that meets the weight at index N - taper."""
+    t = np.asarray(t, dtype=np.float64)
     N = len(t)
-    s = np.linspace(1e-40, 1, order)
-    k1 = np.arange(order)
-    k2 = np.arange(1, order + 2)
-    iv = np.linalg.inv(np.exp(np.outer(k1, np.log(s))))
-    u = np.diff(np.exp(np.outer(k2, np.log(s))) * np.tile(1.0 / k2[:, None], [1, order]))
-    W1 = iv @ u[1:order + 1, :]
-    W2 = iv @ u[0:order, :]
-    p = 1.0 / np.concatenate((np.arange(1, order), (order - 1) * np.ones(N - 2 * (order - 1) - 1), np.arange(order - 1, 0, -1)))
-    w = np.zeros(N)
-    for j in range(N - order + 1):
-        h = t[j + order - 1] - t[j]
-        W = h * h * W1 + h * t[j] * W2
-        for k in range(order - 1):
-            w[j:j + order] += p[j + k] * W[:, k]
-    if N > 40:
-        w[-40:] = w[-40] / (N - 40) * np.arange(N - 40, N)
-    return w
+    nwin = N - order + 1
+    if nwin < 1:
+        raise ValueError(f"ramp_quadrature needs at least {order} grid points, got {N}")
+    M0, M1 = _window_moments(order)
+    gap = np.arange(N - 1)
+    covering = np.minimum(gap, nwin - 1) - np.maximum(0, gap - (order - 2)) + 1
+    share = np.lib.stride_tricks.sliding_window_view(1.0 / covering, order - 1)   # (nwin, order - 1)
+    j = np.arange(nwin)
+    h = t[j + order - 1] - t[j]
+    per_window = (h * h)[:, None] * (share @ M1.T) + (h * t[j])[:, None] * (share @ M0.T)   # (nwin, order)
+    q = np.zeros(N)
+    for i in range(order):
+        q[i:i + nwin] += per_window[:, i]
+    if N > taper:
+        q[N - taper:] = q[N - taper] * np.arange(N - taper, N) / (N - taper)
+    return q
+
+
+# Apodisation windows of the named filters as functions of x = t / d (d = 0.5, the Nyquist frequency of the
+# unpadded row), each with its support (processing/fbp_filter.py:48-91 names the filters and their shapes).
+_WINDOWS = {
+    "ramp": (lambda x: np.ones_like(x), np.inf),
+    "shepp": (lambda x: np.sinc(x / 2.0), 2.0),
+    "cosine": (lambda x: np.cos(0.5 * np.pi * x), 1.0),
+    "cosine2": (lambda x: np.cos(0.5 * np.pi * x) ** 2, 1.0),
+    "hamming": (lambda x: 0.54 + 0.46 * np.cos(np.pi * x), 1.0),
+    "hann": (lambda x: 0.5 * (1.0 + np.cos(np.pi * x)), 1.0),
+    "parzen": (lambda x: (1.0 - x) ** 3, 1.0),
+}
+
+
+def calc_filter_host(n: int, name: str) -> np.ndarray:
+    """(4n // 2 + 1,) float32 half-spectrum weights of the named filter on the 4n-padded grid.
+
+    Frequencies t_k = k / (4n), k = 0 .. 2n.  The ramp part is (4n / 2) * ramp_quadrature(t), 'none'
+    replaces it by the constant 4n / 2 with a doubled DC value; the named window multiplies it on its
+    support.  Negative values are clipped, everything is doubled, DC doubled again, and the result is
+    scaled by 3 n (processing/fbp_filter.py:48-91 fixes these factors)."""
+    ne = 4 * n
+    t = np.arange(ne // 2 + 1) / ne
+    half = 0.5 * ne
+    if name == "none":
+        resp = np.full(t.shape, half)
+        resp[0] *= 2.0
+    elif name in _WINDOWS:
+        shape, support = _WINDOWS[name]
+        x = t / 0.5
+        resp = half * ramp_quadrature(t) * np.where(x <= support, shape(np.minimum(x, support)), 0.0)
+    else:
+        raise ValueError(f"FBPFilter: no filter named {name!r} (known: {', '.join(FILTERS)})")
+    resp = 2.0 * np.maximum(resp, 0.0)
+    resp[0] *= 2.0
+    return (3 * n * resp).astype(np.float32)
 
 
 def two_n_tables(w: np.ndarray, n: int):
@@ -71,35 +137,6 @@ def two_n_tables(w: np.ndarray, n: int):
     return G.astype(np.float32), A.astype(np.float32), B.astype(np.float32)
 
 
-def calc_filter_host(n: int, name: str) -> np.ndarray:
-    """(4n // 2 + 1,) float32 weights of the named filter (processing/fbp_filter.py:48-91)."""
-    d = 0.5
-    ne = 4 * n
-    t = np.arange(0, ne // 2 + 1) / ne
-    if name == "none":
-        wfa = ne * 0.5 + t * 0
-        wfa[0] *= 2
-    elif name == "ramp":
-        wfa = ne * 0.5 * wint(12, t)
-    elif name == "shepp":
-        wfa = ne * 0.5 * wint(12, t) * np.sinc(t / (2 * d)) * (t / d <= 2)
-    elif name == "cosine":
-        wfa = ne * 0.5 * wint(12, t) * np.cos(np.pi * t / (2 * d)) * (t / d <= 1)
-    elif name == "cosine2":
-        wfa = ne * 0.5 * wint(12, t) * np.cos(np.pi * t / (2 * d)) ** 2 * (t / d <= 1)
-    elif name == "hamming":
-        wfa = ne * 0.5 * wint(12, t) * (0.54 + 0.46 * np.cos(np.pi * t / d)) * (t / d <= 1)
-    elif name == "hann":
-        wfa = ne * 0.5 * wint(12, t) * (1 + np.cos(np.pi * t / d)) / 2.0 * (t / d <= 1)
-    elif name == "parzen":
-        wfa = ne * 0.5 * wint(12, t) * (1 - t / d) ** 3 * (t / d <= 1)
-    else:
-        raise ValueError(f"unknown FBP filter '{name}'; expected one of {', '.join(FILTERS)}")
-    wfa = 2 * wfa * (wfa >= 0)
-    wfa[0] *= 2
-    return (3 * n * wfa).astype(np.float32)
-
-
 class FBPFilter:
     """Same interface as processing/fbp_filter.py:39 (``FBPFilter(n, batch_chunk)``, ``calc_filter``,
     ``filter``, ``filter_host``); carriers are torch ROCm tensors."""
@@ -115,7 +152,7 @@ class FBPFilter:
         self.batch_chunk = int(batch_chunk)
         self.method = method
         self._cache: dict[str, torch.Tensor] = {}
-        self._tables: dict[tuple, tuple] = {}
+        self._tables: list[tuple] = []   # (tensor or None, version, weight bytes, (G, A, B))
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     def calc_filter(self, name: str) -> torch.Tensor:
@@ -132,12 +169,11 @@ class FBPFilter:
             raise ValueError("sino must be C-contiguous (reshape view for chunk loop)")
         if sino.dtype != torch.float32 or not sino.is_cuda:
             raise ValueError("sino must be a float32 device tensor")
-        w = as_device_tensor(w, torch.float32, sino.device).contiguous()
-        if w.numel() != self.ne // 2 + 1:
-            raise ValueError(f"filter weights have {w.numel()} values, expected {self.ne // 2 + 1}")
+        self.check_weights(w)
         rows = sino.numel() // self.n
         if self.method == "4n":
-            _lib.check(self._lib.mm_fbp_filter(sino.data_ptr(), w.data_ptr(), rows, self.n, _lib.stream_handle()),
+            wd = as_device_tensor(w, torch.float32, sino.device).contiguous()
+            _lib.check(self._lib.mm_fbp_filter(sino.data_ptr(), wd.data_ptr(), rows, self.n, _lib.stream_handle()),
                        "FBPFilter.filter")
             return sino
         G, A, B = self._two_n(w)
@@ -145,14 +181,44 @@ class FBPFilter:
                                               self.n, _lib.stream_handle()), "FBPFilter.filter")
         return sino
 
-    def _two_n(self, w: torch.Tensor):
-        """Device (G, A, B) of the length-2n form for weights w, built once per weight tensor (keyed by its
-        storage and version counter, so an in-place change of w rebuilds them)."""
-        key = (w.data_ptr(), w._version, w.numel())
-        if key not in self._tables:
-            G, A, B = two_n_tables(w.detach().cpu().numpy(), self.n)
-            self._tables[key] = tuple(torch.from_numpy(a).to(w.device) for a in (G, A, B))
-        return self._tables[key]
+    def check_weights(self, w):
+        size = w.numel() if isinstance(w, torch.Tensor) else np.size(w)
+        if size != self.ne // 2 + 1:
+            raise ValueError(f"filter weights have {size} values, expected {self.ne // 2 + 1}")
+
+    _TABLES_KEPT = 4
+
+    def _two_n(self, w):
+        """Device (G, A, B) of the length-2n form for the weights w (numpy array or tensor).
+
+        Tables are keyed by the weight values (their float32 bytes), never by a tensor's address, so a new
+        tensor at a reused address, or w changed in place, gets its own tables.  A tensor seen before (the
+        entry keeps a reference to it, so its address cannot be reused while cached) at the same version
+        counter skips the host copy.  The last few weight sets are kept."""
+        if isinstance(w, torch.Tensor):
+            for i, (ref, ver, _, tables) in enumerate(self._tables):
+                if ref is w and ver == w._version:
+                    return self._promote(i)
+            host = w.detach().to("cpu", torch.float32).contiguous().numpy()
+            ref, ver = w, w._version
+        else:
+            host = np.ascontiguousarray(np.asarray(w, dtype=np.float32))
+            ref, ver = None, -1
+        key = host.tobytes()
+        for i, (_, _, k, tables) in enumerate(self._tables):
+            if k == key:
+                self._tables[i] = (ref, ver, k, tables)
+                return self._promote(i)
+        G, A, B = two_n_tables(host, self.n)
+        tables = tuple(torch.from_numpy(a).to(self.device) for a in (G, A, B))
+        self._tables.insert(0, (ref, ver, key, tables))
+        del self._tables[self._TABLES_KEPT:]
+        return tables
+
+    def _promote(self, i):
+        if i:
+            self._tables.insert(0, self._tables.pop(i))
+        return self._tables[0][3]
 
     def filter_host(self, sino_h: np.ndarray, w: torch.Tensor, batch_chunk: int | None = None) -> np.ndarray:
         """Host sinogram filtered in place, in slabs of ``batch_chunk`` rows (processing/fbp_filter.py:212)."""

@@ -19,7 +19,7 @@ import ctypes as C
 import torch
 
 from model_mosaic_amd import _lib
-from model_mosaic_amd._carriers import as_device_tensor, require_gpu
+from model_mosaic_amd._carriers import as_device_tensor, check_out, require_gpu
 
 
 class Paganin:
@@ -69,7 +69,11 @@ class Paganin:
         b = x.shape[0]
         assert b <= self._ntheta, f"batch {b} > cached ntheta {self._ntheta}"
         if out is None:
-            out = torch.empty_like(x)
+            res = out = torch.empty_like(x)
+            if added_dim:
+                res = out[0]
+        else:
+            res = check_out(out, x.shape, torch.float32, self.device, "Paganin.retrieve")
         _lib.check(self._lib.mm_paganin_retrieve(self._plan, x.data_ptr(), out.data_ptr(), b, _lib.stream_handle()),
                    "Paganin.retrieve")
-        return out[0] if added_dim else out
+        return res

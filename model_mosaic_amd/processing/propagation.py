@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from model_mosaic_amd import _lib
-from model_mosaic_amd._carriers import as_device_tensor, require_gpu
+from model_mosaic_amd._carriers import as_device_tensor, check_out, require_gpu
 
 
 class Propagation:
@@ -68,12 +68,16 @@ class Propagation:
         b = x.shape[0]
         if b > self.ntheta:
             raise ValueError(f"Propagation: batch {b} exceeds ntheta={self.ntheta}")
+        what = "Propagation.DT" if adj else "Propagation.D"
         if out is None:
-            out = torch.empty_like(x)
+            res = out = torch.empty_like(x)
+            if added_dim:
+                res = out[0]
+        else:
+            res = check_out(out, x.shape, torch.complex64, self.device, what)
         fn = self._lib.mm_fresnel_adj if adj else self._lib.mm_fresnel_fwd
-        _lib.check(fn(self._plan, x.data_ptr(), out.data_ptr(), b, int(j), _lib.stream_handle()),
-                   "Propagation.DT" if adj else "Propagation.D")
-        return out[0] if added_dim else out
+        _lib.check(fn(self._plan, x.data_ptr(), out.data_ptr(), b, int(j), _lib.stream_handle()), what)
+        return res
 
     def D(self, psi, j, out=None) -> torch.Tensor:
         """Forward Fresnel propagation with the j-th distance (processing/propagation.py:47)."""

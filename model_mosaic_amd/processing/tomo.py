@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from model_mosaic_amd import _lib
-from model_mosaic_amd._carriers import as_device_tensor, require_gpu
+from model_mosaic_amd._carriers import as_device_tensor, check_out, require_gpu
 
 
 class TomoReal:
@@ -80,6 +80,8 @@ class TomoReal:
             raise ValueError(f"TomoReal.R: nz={nz} outside [1, {self._nz}]")
         if out is None:
             out = torch.empty((self.ntheta, nz, self.n), dtype=torch.float32, device=self.device)
+        else:
+            check_out(out, (self.ntheta, nz, self.n), torch.float32, self.device, "TomoReal.R")
         _lib.check(self._lib.mm_radon_fwd(self._plan, x.data_ptr(), out.data_ptr(), nz, _lib.stream_handle()),
                    "TomoReal.R")
         return out
@@ -101,6 +103,8 @@ class TomoReal:
             raise ValueError(f"TomoReal.RT: nz={nz} outside [1, {self._nz}]")
         if out is None:
             out = torch.empty((nz, self.n, self.n), dtype=dt, device=self.device)
+        else:
+            check_out(out, (nz, self.n, self.n), dt, self.device, "TomoReal.RT")
         _lib.check(self._lib.mm_radon_adj(self._plan, y.data_ptr(), 0 if is_real else 1, out.data_ptr(), nz,
                                           _lib.stream_handle()), "TomoReal.RT")
         return out
@@ -118,12 +122,12 @@ class TomoReal:
         nz = y.shape[1]
         if not 1 <= nz <= self._nz:
             raise ValueError(f"TomoReal.RT_filtered: nz={nz} outside [1, {self._nz}]")
-        w = as_device_tensor(w, torch.float32, self.device).contiguous()
-        if w.numel() != fbp.ne // 2 + 1:
-            raise ValueError(f"filter weights have {w.numel()} values, expected {fbp.ne // 2 + 1}")
+        fbp.check_weights(w)
         G, A, B = fbp._two_n(w)
         if out is None:
             out = torch.empty((nz, self.n, self.n), dtype=torch.float32, device=self.device)
+        else:
+            check_out(out, (nz, self.n, self.n), torch.float32, self.device, "TomoReal.RT_filtered")
         _lib.check(self._lib.mm_radon_adj_fbp(self._plan, y.data_ptr(), G.data_ptr(), A.data_ptr(), B.data_ptr(),
                                               out.data_ptr(), nz, _lib.stream_handle()), "TomoReal.RT_filtered")
         return out

@@ -4,7 +4,7 @@ from __future__ import annotations
 import torch
 
 from model_mosaic_amd import _lib
-from model_mosaic_amd._carriers import as_device_tensor, require_gpu
+from model_mosaic_amd._carriers import as_device_tensor, check_out, require_gpu
 
 
 def psi_from_proj(delta_raw, inv_norm, inv_beta_ratio, out=None) -> torch.Tensor:
@@ -15,6 +15,8 @@ def psi_from_proj(delta_raw, inv_norm, inv_beta_ratio, out=None) -> torch.Tensor
     p = as_device_tensor(delta_raw, torch.float32, dev)
     if out is None:
         out = torch.empty(p.shape, dtype=torch.complex64, device=dev)
+    else:
+        check_out(out, p.shape, torch.complex64, dev, "psi_from_proj")
     _lib.check(_lib.load().mm_psi_from_proj(p.data_ptr(), out.data_ptr(), p.numel(), float(inv_norm),
                                             float(inv_beta_ratio), _lib.stream_handle()), "psi_from_proj")
     return out
@@ -27,5 +29,7 @@ def abs2(z, out=None) -> torch.Tensor:
     x = as_device_tensor(z, torch.complex64, dev)
     if out is None:
         out = torch.empty(x.shape, dtype=torch.float32, device=dev)
+    else:
+        check_out(out, x.shape, torch.float32, dev, "abs2")
     _lib.check(_lib.load().mm_abs2(x.data_ptr(), out.data_ptr(), x.numel(), _lib.stream_handle()), "abs2")
     return out

@@ -69,3 +69,31 @@ def test_bad_sizes_rejected_before_device_work(lib):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_lib.MosaicLibraryError):
         _lib.load(tmp_path / "nope.so")
+
+
+def _header_ranges(entry, var):
+    """The '2^k in [a, b] or 3 * 2^k in [c, d]' limits the header states for `var` of `entry`."""
+    txt = open(HEADER).read()
+    seg = txt[txt.index(f" * {entry}:"):]
+    seg = seg[:seg.index("*/")]
+    seg = " ".join(re.sub(r"\n\s*\*", " ", seg).split())
+    m = re.search(re.escape(var) + r"[^\[]*?2\^k in \[(\d+), (\d+)\] or 3 \* 2\^k in \[(\d+), (\d+)\]", seg)
+    assert m, f"no size range for {var} in the {entry} comment"
+    return tuple(int(v) for v in m.groups())
+
+
+def test_header_size_limits_match_the_library(lib):
+    """The Radon and Fresnel size limits written in the header are the ones the library enforces: the ends
+    of each range are accepted (host-only plan check for Radon) and the next sizes outside are rejected."""
+    a, b, c, d = _header_ranges("mm_radon_create", "n")
+    th = (C.c_double * 2)(0.0, 0.3)
+    for n in (a, b, c, d):
+        assert lib.mm_radon_plan_check(n, 2, th) == _lib.MM_OK, (n, _lib.last_error())
+    for n in (a // 2, 2 * b, c // 2, 2 * d):
+        assert lib.mm_radon_plan_check(n, 2, th) == _lib.MM_ERR_ARG, n
+    p = C.c_void_p()
+    fa, fb, fc, fd = _header_ranges("mm_fresnel_create", "nz and n")
+    dist = (C.c_double * 1)(0.1)
+    for n in (fa // 2, 2 * fb, fc // 2, 2 * fd):
+        assert lib.mm_fresnel_create(C.byref(p), n, fa, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, n
+        assert lib.mm_fresnel_create(C.byref(p), fa, n, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, n

@@ -99,3 +99,56 @@ def test_RT_filtered_equals_filter_then_RT(n, nt, nz):
     err = rel(fused, ref)
     print(f"RT_filtered n={n}: rel-L2 vs FP64 {err:.2e}")
     assert err < 1e-5
+
+
+def test_weight_tables_follow_the_values_not_the_address():
+    """Numpy weights are converted afresh on every call, and a freed tensor's address is reused by the
+    next one: each call must still filter with its own weights (ramp, then shepp, then ramp modified in
+    place), both for FBPFilter.filter and TomoReal.RT_filtered."""
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter, calc_filter_host
+    from model_mosaic_amd.processing.tomo import TomoReal
+
+    from oracle import usfft
+    n, nt, nz = 64, 24, 2
+    f = FBPFilter(n)
+    x = np.random.default_rng(21).standard_normal((nt, nz, n)).astype(np.float32)
+    weights = [calc_filter_host(n, "ramp"), calc_filter_host(n, "shepp")]
+    for w in weights + weights[::-1]:
+        d = torch.from_numpy(x).cuda()
+        f.filter(d, w)
+        assert rel(d.cpu().numpy(), fbp.filter(x, w.astype(np.float64))) < 1e-5
+    wt = torch.from_numpy(weights[0].copy()).cuda()
+    d = torch.from_numpy(x).cuda()
+    f.filter(d, wt)
+    wt.mul_(0.5)                                   # same tensor, changed in place
+    d = torch.from_numpy(x).cuda()
+    f.filter(d, wt)
+    assert rel(d.cpu().numpy(), fbp.filter(x, 0.5 * weights[0].astype(np.float64))) < 1e-5
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    t = TomoReal(n, nz, th)
+    for w in weights:
+        got = t.RT_filtered(torch.from_numpy(x).cuda(), f, w).cpu().numpy()
+        ref = usfft.tomo_RT(fbp.filter(x, w.astype(np.float64)), th, dtype=np.float64)
+        assert rel(got, ref) < 1e-5
+    assert len(f._tables) <= f._TABLES_KEPT
+
+
+def test_out_buffers_are_checked():
+    """A caller's out= of the wrong size or dtype is rejected before any kernel writes into it."""
+    from model_mosaic_amd.processing.propagation import Propagation
+    from model_mosaic_amd.processing.tomo import TomoReal
+    n, nt, nz = 32, 8, 2
+    t = TomoReal(n, nz, np.linspace(0, np.pi, nt, endpoint=False))
+    x = torch.zeros((nz, n, n), device="cuda")
+    with pytest.raises(ValueError):
+        t.R(x, out=torch.empty((nt, nz, n - 1), device="cuda"))
+    with pytest.raises(ValueError):
+        t.RT(torch.zeros((nt, nz, n), device="cuda"), out=torch.empty((nz, n, n), dtype=torch.complex64, device="cuda"))
+    p = Propagation(n, 16, 2, 1, 4.13e-11, 1.4e-6, [0.1])
+    psi = torch.ones((16, n), dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        p.D(psi, 0, out=torch.empty((16, n), dtype=torch.float32, device="cuda"))
+    o = torch.empty((16, n), dtype=torch.complex64, device="cuda")
+    res = p.D(psi, 0, out=o)
+    assert res is o and res.shape == (16, n)       # a 2-D out comes back whole, not as one row
+    assert torch.allclose(res, p.D(psi, 0))

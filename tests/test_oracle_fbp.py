@@ -49,3 +49,13 @@ def test_filter_is_circular_convolution_of_padded_row():
     conv = np.array([sum(p[j] * kern[(i - j) % ne] for j in range(ne)) for i in range(ne)])
     pad = (ne - n) // 2
     assert np.allclose(fbp.filter(x, w), conv[pad:pad + n], rtol=1e-10, atol=1e-10)
+
+
+def test_edge_pad_matches_numpy_edge_mode():
+    """edge_pad (the reference's edge replication to ne = 4 n) = np.pad(mode="edge") with (ne - n) // 2 on the
+    left and the rest on the right."""
+    x = np.random.default_rng(3).standard_normal((2, 3, 24))
+    for ne in (96, 97):
+        pad = (ne - 24) // 2
+        ref = np.pad(x, ((0, 0), (0, 0), (pad, ne - 24 - pad)), mode="edge")
+        assert np.array_equal(fbp.edge_pad(x, ne), ref)

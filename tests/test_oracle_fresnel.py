@@ -23,6 +23,15 @@ def test_pad_matches_numpy_symmetric():
     assert np.array_equal(p, q)
 
 
+def test_zero_pad_matches_numpy_constant_and_z0_identity():
+    """pad_zero = np.pad(mode="constant") with the frame centred, and D with zero padding at z = 0 is the
+    identity (the chirp is 1, crop(pad(x)) = x)."""
+    q = np.pad(PSI, ((0, 0), (NZ // 2, NZ // 2), (N // 2, N // 2)), mode="constant")
+    assert np.array_equal(fr.pad_zero(PSI), q)
+    fk = fr.fresnel_kernel_2d(N, NZ, WL, VOX, DIST, np.float64)
+    assert rel(fr.D(PSI, fk[0], np.float64, pad="zero"), PSI) < 1e-14
+
+
 def test_pad_transpose_is_adjoint():
     rng = np.random.default_rng(5)
     g = rng.standard_normal((1, 2 * NZ, 2 * N))
