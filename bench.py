@@ -299,23 +299,6 @@ def run_fbp(torch, reps=5):
     return out
 
 
-def mosaic_spec(seed=7):
-    """Uneven mosaic: 4 x-tiles x 9 z-stacks of 96 rows (8-row overlaps between stacks); the x-tiles' detector
-    widths are a seeded draw without replacement from {1536, 2048, 3072, 4096} (so the batch fits one GPU:
-    ~113 GB of inputs, ~57 GB of outputs), angle counts per tile drawn from {1029, 2058}."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    nx, nzs, rows, overlap = 4, 9, 96, 8
-    widths = rng.permutation([1536, 2048, 3072, 4096])[:nx]
-    tiles = []
-    for i in range(nx):
-        for s in range(nzs):
-            nt = int(rng.choice([1029, 2058]))
-            tiles.append(dict(x=i, s=s, n=int(widths[i]), nz=rows, ntheta=nt, rows=(s * (rows - overlap), s * (rows - overlap) + rows),
-                              ztotal=nzs * (rows - overlap) + overlap))
-    return tiles
-
-
 def run_mosaic(torch, reps=2):
     """R, psi, Fresnel D (30 keV, 0.69 um, z = 0.2 m) and |.|^2 over the uneven mosaic as one HIP graph."""
     import numpy as np
@@ -323,7 +306,7 @@ def run_mosaic(torch, reps=2):
     from model_mosaic_amd import phantom
     from model_mosaic_amd.mosaic import MosaicBatch
 
-    spec = mosaic_spec()
+    spec = phantom.uneven_mosaic()
     dev = torch.device("cuda", 0)
     deltas = [phantom.ellipsoids(t["ztotal"], t["n"], seed=100 + t["x"], rows=t["rows"], device=dev) for t in spec]
     tiles = [(t["n"], t["nz"], phantom.angles(t["ntheta"])) for t in spec]

@@ -76,3 +76,21 @@ def smooth_field(nframes: int, nz: int, n: int, peak: float = 2.0, seed: int = 0
     out -= out.amin()
     out *= peak / max(float(out.amax()), 1e-30)
     return out
+
+
+def uneven_mosaic(seed: int = 7):
+    """Uneven mosaic scan: 4 x-tiles x 9 z-stacks of 96 rows (8-row overlaps between stacks).  The x-tiles'
+    detector widths are a seeded draw without replacement from {1536, 2048, 3072, 4096} (so the batch fits one
+    GPU: ~113 GB of inputs, ~57 GB of outputs); each tile's angle count is drawn from {1029, 2058}.  Returns
+    one dict per tile: x, s (stack), n, nz, ntheta, rows (row range in the full sample) and ztotal."""
+    rng = np.random.default_rng(seed)
+    nx, nzs, rows, overlap = 4, 9, 96, 8
+    widths = rng.permutation([1536, 2048, 3072, 4096])[:nx]
+    tiles = []
+    for i in range(nx):
+        for s in range(nzs):
+            nt = int(rng.choice([1029, 2058]))
+            tiles.append(dict(x=i, s=s, n=int(widths[i]), nz=rows, ntheta=nt,
+                              rows=(s * (rows - overlap), s * (rows - overlap) + rows),
+                              ztotal=nzs * (rows - overlap) + overlap))
+    return tiles

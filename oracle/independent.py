@@ -39,9 +39,10 @@ def nudft_polar(obj, theta):
         eta = -r * np.sin(t)
         ex = np.exp(-2j * np.pi * xi[:, None] * xs[None, :])    # (n_r, n_x)
         ey = np.exp(-2j * np.pi * eta[:, None] * xs[None, :])   # (n_r, n_y)
-        # F[z, r] = sum_{y,x} obj[z,y,x] ey[r,y] ex[r,x]
-        tmp = np.einsum("zyx,rx->zry", obj, ex)
-        out[k] = np.einsum("zry,ry->zr", tmp, ey)
+        # F[z, r] = sum_{y,x} obj[z,y,x] ey[r,y] ex[r,x]: the x sum as one matrix product per slice
+        for z in range(nz):
+            tmp = obj[z] @ ex.T                                  # (n_y, n_r)
+            out[k, z] = np.sum(tmp * ey.T, axis=0)
     return out
 
 

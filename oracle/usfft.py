@@ -178,33 +178,36 @@ def gather_half(fde, theta, n, dtype=np.float32, variant="real"):
     return np.ascontiguousarray(np.transpose(g, (1, 0, 2))).astype(cdt)
 
 
-def scatter_full(g, theta, n, dtype=np.float32):
+def scatter_full(g, theta, n, dtype=np.float32, angle_block=128):
     """Adjoint scatter of ``gather`` dir=1 (processing/kernels.py:16-93).
 
     g: (ntheta, nz, n) complex samples.  Returns the centred full grid
     (nz, 2n, 2n) complex with index (n + ell + 2n) % 2n (kernels.py:63,70).
     Accumulation is exact (float64 bincount) before rounding to ``dtype``; the
-    reference's float atomics have no defined order.
+    reference's float atomics have no defined order.  Angles are processed in
+    blocks of ``angle_block`` (the sum is the same; only host memory changes).
     """
     cdt = _cdtype(dtype)
     g = np.asarray(g, dtype=cdt)
     ntheta, nz, _ = g.shape
     twon = 2 * n
-    x0, y0 = _polar_coords(theta, n, dtype, "real")
-    ell0, ell1, w0, w1, coeff0, coeff1 = _window(x0, y0, n, dtype, "real")
-    ex = np.exp(coeff1 * w0 * w0).astype(dtype)            # kernels.py:52-56
-    ey = (coeff0 * np.exp(coeff1 * w1 * w1)).astype(dtype)  # kernels.py:62
-    w = (ex[..., None, :] * ey[..., :, None]).astype(dtype)  # [..., i1, i0]
-    fy = (n + ell1 + twon) % twon
-    fx = (n + ell0 + twon) % twon
-    fidx = (fy[..., :, None] * twon + fx[..., None, :]).reshape(-1)
+    th_all = np.asarray(theta)
     out = np.zeros((nz, twon * twon), dtype=np.complex128)
-    wf = w.reshape(ntheta, n, -1)
-    for z in range(nz):
-        c = (wf * g[:, z, :, None]).astype(cdt).reshape(-1)
-        re = np.bincount(fidx, weights=c.real.astype(np.float64), minlength=twon * twon)
-        im = np.bincount(fidx, weights=c.imag.astype(np.float64), minlength=twon * twon)
-        out[z] = re + 1j * im
+    for a0 in range(0, ntheta, angle_block):
+        a1 = min(ntheta, a0 + angle_block)
+        x0, y0 = _polar_coords(th_all[a0:a1], n, dtype, "real")
+        ell0, ell1, w0, w1, coeff0, coeff1 = _window(x0, y0, n, dtype, "real")
+        ex = np.exp(coeff1 * w0 * w0).astype(dtype)            # kernels.py:52-56
+        ey = (coeff0 * np.exp(coeff1 * w1 * w1)).astype(dtype)  # kernels.py:62
+        w = (ex[..., None, :] * ey[..., :, None]).astype(dtype)  # [..., i1, i0]
+        fy = (n + ell1 + twon) % twon
+        fx = (n + ell0 + twon) % twon
+        fidx = (fy[..., :, None] * twon + fx[..., None, :]).reshape(-1)
+        wf = w.reshape(a1 - a0, n, -1)
+        for z in range(nz):
+            c = (wf * g[a0:a1, z, :, None]).astype(cdt).reshape(-1)
+            out[z] += np.bincount(fidx, weights=c.real.astype(np.float64), minlength=twon * twon)
+            out[z] += 1j * np.bincount(fidx, weights=c.imag.astype(np.float64), minlength=twon * twon)
     return out.reshape(nz, twon, twon).astype(cdt)
 
 

@@ -1,0 +1,297 @@
+"""Parity at the full workload sizes (needs a GPU): every check here runs the HIP path at the size it is
+benchmarked at, against the FP64 oracle where the oracle can finish in tens of seconds, and through
+size-independent identities over the whole batch.
+
+* USFFT Radon, nz = 64, n = 2048, 2048 angles on [0, pi): R and RT of two slices vs the FP64 oracle
+  (R at a random subset of the angles, RT over all of them), R vs the exact NUDFT at three angles, the FP64
+  adjoint identity and the theta = 0 column sums over all 64 slices.
+* Fresnel D / DT, 128 frames of 2048 x 2048, z in {0.05, 0.2, 0.8} m: two frames vs the FP64 oracle per
+  distance, the adjoint identity over all 128 frames.
+* Forward model, nz = 128, n = ntheta = 1024, 30 keV, z = 0.2 m: gradient vs one assembled from the
+  operators (RT, DT and element-wise torch), central differences on 64 random voxels.
+* Uneven mosaic (4 x-tiles x 9 z-stacks, widths 1536 - 4096, 1029 / 2058 angles): every tile bitwise equal
+  to its per-tile operator chain; R of every (width, angle count) pair and D of every width vs the FP64
+  oracle; the whole intensity of the narrowest tile at three angles vs the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fresnel as fr
+from oracle import independent as ind
+from oracle import usfft
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def dot64(a, b):
+    """Real inner product of two equally shaped device tensors, accumulated in FP64 on the device."""
+    if a.is_complex():
+        return float(torch.sum((a.to(torch.complex128).conj() * b.to(torch.complex128)).real))
+    return float(torch.sum(a.double() * b.double()))
+
+
+# ---- USFFT Radon at n = 2048, 2048 angles, 64 slices -------------------------------------------------------
+
+N2, NT2, NZ2 = 2048, 2048, 64
+
+
+@pytest.fixture(scope="module")
+def radon_2048():
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.processing.tomo import TomoReal
+    print(f"[full sizes] Radon n={N2} ntheta={NT2} nz={NZ2}: phantom, plan, R, RT", flush=True)
+    th = phantom.angles(NT2)
+    vol = phantom.ellipsoids(NZ2, N2, seed=0, noise=0.01, device="cuda")   # the benchmark's volume
+    t = TomoReal(N2, NZ2, th)
+    sino = t.R(vol)
+    back = t.RT(sino)
+    torch.cuda.synchronize()
+    yield th, vol, t, sino, back
+    t.free()
+    del vol, sino, back
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(900)
+def test_radon_2048_two_slices_vs_fp64(radon_2048):
+    """R (256 random angles of the 2048, every detector bin) and RT (all 2048 angles) of slices 31, 32.  R at
+    an angle subset equals R at all angles times sqrt(256 / 2048): the angle count enters R only through
+    the 1 / sqrt(n ntheta) scale of phi."""
+    th, vol, t, sino, back = radon_2048
+    zs = [31, 32]
+    x = vol[zs].cpu().numpy()
+    idx = np.sort(np.random.default_rng(2048).choice(NT2, 256, replace=False))
+    scale = math.sqrt(len(idx) / NT2)
+    r64 = usfft.tomo_R(x, th[idx], np.float64) * scale
+    r32 = usfft.tomo_R(x, th[idx], np.float32).astype(np.float64) * scale
+    got = sino[torch.from_numpy(idx).cuda()][:, zs].cpu().numpy()
+    e64, e32 = rel(got, r64), rel(got, r32)
+    print(f"n=2048 ntheta=2048: R rel-L2 vs FP64 oracle {e64:.2e}, vs FP32 reference restatement {e32:.2e}",
+          flush=True)
+    assert e64 < 1e-5 and e32 < 1e-5
+    y = sino[:, zs].cpu().numpy()
+    b64 = usfft.tomo_RT(y, th, dtype=np.float64)
+    eb = rel(back[zs].cpu().numpy(), b64)
+    print(f"n=2048 ntheta=2048: RT rel-L2 vs FP64 oracle {eb:.2e}", flush=True)
+    assert eb < 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_radon_2048_vs_exact_nudft(radon_2048):
+    """R of slice 32 at three angles against the exact polar NUDFT (FP64 brute force): the USFFT's design error
+    with the m = 4 window (eps = 1e-3)."""
+    th, vol, t, sino, back = radon_2048
+    idx = np.array([0, 611, 1403])
+    x = vol[32:33].cpu().numpy()
+    ref = ind.radon_from_nudft(x, th[idx]) * math.sqrt(len(idx) / NT2)
+    got = sino[torch.from_numpy(idx).cuda()][:, 32:33].cpu().numpy()
+    err = rel(got, ref)
+    print(f"n=2048 ntheta=2048: R rel-L2 vs exact NUDFT {err:.2e}", flush=True)
+    assert err < 2e-3
+
+
+def test_radon_2048_adjoint_and_column_sums_all_slices(radon_2048):
+    """<R x, y> = <x, RT y> in FP64 over the whole 64-slice batch (random y, normalised by ||Rx|| ||y||, and
+    the non-cancelling y = R x), and R at theta = 0 = column sums / sqrt(n ntheta) for every slice."""
+    th, vol, t, sino, back = radon_2048
+    g = torch.Generator(device="cuda").manual_seed(11)
+    y = torch.randn(sino.shape, generator=g, device="cuda")
+    lhs, rhs = dot64(sino, y), dot64(vol, t.RT(y))
+    cs = abs(lhs - rhs) / (float(sino.double().norm()) * float(y.double().norm()))
+    self_lhs, self_rhs = dot64(sino, sino), dot64(vol, back)
+    se = abs(self_lhs - self_rhs) / self_lhs
+    print(f"n=2048 ntheta=2048 nz=64 adjoint: cauchy-schwarz {cs:.2e}, y=Rx {se:.2e}", flush=True)
+    assert cs < 1e-6 and se < 1e-6
+    colsum = vol.double().sum(dim=1) / math.sqrt(N2 * NT2)
+    ec = float((sino[0].double() - colsum).norm() / colsum.norm())
+    print(f"n=2048: theta=0 column sums rel-L2 {ec:.2e}", flush=True)
+    assert ec < 1.5e-3
+
+
+# ---- Fresnel D / DT at 128 frames of 2048 x 2048 ----------------------------------------------------------
+
+WL30 = 1.24e-9 / 30.0
+
+
+@pytest.mark.timeout(600)
+def test_fresnel_2048_frames_vs_fp64_and_adjoint():
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.processing.propagation import Propagation
+    from model_mosaic_amd.step_kernels import psi_from_proj
+    b, nz, n, vox, dist = 128, 2048, 2048, 0.69e-6, [0.05, 0.2, 0.8]
+    print(f"[full sizes] Fresnel {b} frames {nz}x{n}, z={dist}", flush=True)
+    p = phantom.smooth_field(b, nz, n, peak=2.0, seed=3, device="cuda")
+    psi = psi_from_proj(p, 1.0, 1e-3)          # exp(i p - 1e-3 p)
+    del p
+    prop = Propagation(n, nz, b, len(dist), WL30, vox, dist)
+    fk = fr.fresnel_kernel_2d(n, nz, WL30, vox, dist, np.float64)
+    frames = [5, 100]
+    x = psi[frames].cpu().numpy()
+    g = torch.Generator(device="cuda").manual_seed(12)
+    y = torch.complex(torch.randn(psi.shape, generator=g, device="cuda"),
+                      torch.randn(psi.shape, generator=g, device="cuda"))
+    for j, z in enumerate(dist):
+        d = prop.D(psi, j)
+        ed = rel(d[frames].cpu().numpy(), fr.D(x, fk[j], np.float64))
+        dty = prop.DT(y, j)
+        et = rel(dty[frames].cpu().numpy(), fr.DT(y[frames].cpu().numpy(), fk[j], np.float64))
+        lhs, rhs = dot64(d, y), dot64(psi, dty)
+        cs = abs(lhs - rhs) / (float(d.abs().double().norm()) * float(y.abs().double().norm()))
+        print(f"Fresnel 2048x2048 z={z}: D rel-L2 vs FP64 {ed:.2e}, DT {et:.2e}, adjoint over {b} frames {cs:.2e}",
+              flush=True)
+        assert ed < 1e-5 and et < 1e-5 and cs < 1e-6
+        del d, dty
+    prop.free()
+    del psi, y
+    torch.cuda.empty_cache()
+
+
+# ---- forward model at nz = 128, n = ntheta = 1024 ---------------------------------------------------------
+
+@pytest.mark.timeout(600)
+def test_model_1024_gradient_and_finite_differences():
+    """The model's gradient (mm_model_bwd) vs the gradient assembled from the operator classes and torch
+    element-wise algebra:  u = D psi, G = DT(4 (|u|^2 - d) u), dL/dp = -k Im(conj(G) psi),
+    dL/db = -k Re(conj(G) psi), dL/ddelta = RT dL/dp, dL/dbeta = RT dL/db;  then central differences of the
+    GPU loss along directions supported on 64 random voxels (one for delta, one for beta)."""
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.model import ForwardModel, misfit
+    from model_mosaic_amd.step_kernels import abs2, psi_from_proj
+    nz, n, nt = 128, 1024, 1024
+    wl, vox, z = WL30, 1.38e-6, 0.2
+    print(f"[full sizes] model nz={nz} n={n} ntheta={nt}", flush=True)
+    theta = phantom.angles(nt)
+    delta_true = phantom.ellipsoids(nz, n, seed=4, device="cuda")
+    fm = ForwardModel(n, nz, theta, wl, vox, [z])
+    with torch.no_grad():
+        p = fm.tomo.R(delta_true)
+        inten = abs2(fm.prop.D(psi_from_proj(p, fm.k, 1e-3), 0))
+        gen = torch.Generator(device="cuda").manual_seed(5)
+        data = torch.poisson(inten * 1e4, generator=gen) / 1e4
+        del p, inten
+    delta = (0.9 * delta_true).contiguous()
+    beta = (0.9e-3 * delta_true).contiguous()
+    dd = delta.clone().requires_grad_(True)
+    bb = beta.clone().requires_grad_(True)
+    loss = misfit(dd, bb, fm, data)
+    loss.backward()
+    gd, gb = dd.grad, bb.grad
+    # gradient assembled from the operators
+    with torch.no_grad():
+        p, b = fm.tomo.R(delta), fm.tomo.R(beta)
+        psi = torch.polar(torch.exp(-fm.k * b), fm.k * p)
+        u = fm.prop.D(psi, 0)
+        resid = u.real ** 2 + u.imag ** 2 - data
+        L_ref = float(torch.sum(resid.double() ** 2))
+        G = fm.prop.DT((4 * resid) * u, 0)
+        cg = torch.conj(G) * psi
+        rgd = fm.tomo.RT((-fm.k * cg.imag).contiguous())
+        rgb = fm.tomo.RT((-fm.k * cg.real).contiguous())
+        del p, b, psi, u, resid, G, cg
+    el = abs(float(loss) - L_ref) / L_ref
+    ed = float((gd - rgd).norm() / rgd.norm())
+    eb = float((gb - rgb).norm() / rgb.norm())
+    print(f"model 1024: loss vs operator chain {el:.2e}, grad delta {ed:.2e}, grad beta {eb:.2e}", flush=True)
+    assert el < 1e-5 and ed < 1e-4 and eb < 1e-4
+    L0 = float(loss)
+    rng = np.random.default_rng(8)
+    for which, g in ((0, gd), (1, gb)):
+        idx = torch.from_numpy(rng.choice(delta.numel(), 64, replace=False)).cuda()
+        v = torch.zeros(delta.numel(), dtype=torch.float64, device="cuda")
+        v[idx] = torch.sign(g.view(-1)[idx].double()) * torch.from_numpy(rng.uniform(0.5, 1.5, 64)).cuda()
+        dirderiv = float(torch.dot(g.view(-1).double(), v))
+        h = 1e-4 * L0 / abs(dirderiv)
+        vols_p, vols_m = [delta, beta], [delta, beta]
+        base = (delta, beta)[which].double()
+        vols_p[which] = (base + h * v.view(delta.shape)).float()
+        vols_m[which] = (base - h * v.view(delta.shape)).float()
+        fd = (float(fm.loss(*vols_p, data)) - float(fm.loss(*vols_m, data))) / (2 * h)
+        err = abs(fd - dirderiv) / abs(dirderiv)
+        print(f"model 1024 FD ({'delta' if which == 0 else 'beta'}): 64-voxel directional rel err {err:.2e}",
+              flush=True)
+        assert err < 1e-3
+    fm.free()
+    torch.cuda.empty_cache()
+
+
+# ---- uneven mosaic -----------------------------------------------------------------------------------------
+
+MOSAIC_VOX, MOSAIC_Z = 0.69e-6, 0.2
+
+
+@pytest.mark.timeout(900)
+def test_uneven_mosaic_tiles_bitwise_and_vs_fp64():
+    from model_mosaic_amd import phantom
+    from model_mosaic_amd.mosaic import MosaicBatch
+    from model_mosaic_amd.processing.propagation import Propagation
+    from model_mosaic_amd.processing.tomo import TomoReal
+    from model_mosaic_amd.step_kernels import abs2, psi_from_proj
+    torch.cuda.empty_cache()
+    spec = phantom.uneven_mosaic()
+    print(f"[full sizes] uneven mosaic: {len(spec)} tiles", flush=True)
+    deltas = [phantom.ellipsoids(t["ztotal"], t["n"], seed=100 + t["x"], rows=t["rows"], device="cuda")
+              for t in spec]
+    tiles = [(t["n"], t["nz"], phantom.angles(t["ntheta"])) for t in spec]
+    pmax = max(float(d.max()) * 0.5 * math.sqrt(t["n"] / t["ntheta"]) for d, t in zip(deltas, spec))
+    k, beta = 2.0 / pmax, 1e-3
+    mb = MosaicBatch(tiles, WL30, MOSAIC_VOX, MOSAIC_Z, k, beta, autotune=True)
+    outs = mb.run(deltas)
+    torch.cuda.synchronize()
+    mb.free()                      # inputs + outputs hold ~170 GB; one (n, ntheta) group of plans at a time below
+    groups = {}
+    for i, (n, nz, th) in enumerate(tiles):
+        groups.setdefault((n, len(th)), []).append(i)
+    for (n, nt), members in sorted(groups.items()):
+        nz, th = tiles[members[0]][1], tiles[members[0]][2]
+        tomo = TomoReal(n, nz, th)
+        prop = Propagation(n, nz, nt, 1, WL30, MOSAIC_VOX, [MOSAIC_Z])
+        for i in members:
+            ref = abs2(prop.D(psi_from_proj(tomo.R(deltas[i]), np.float32(k), np.float32(beta)), 0))
+            assert torch.equal(outs[i], ref), f"tile {i} (n={n}, ntheta={nt}) differs from its operator chain"
+            del ref
+        # R of this (width, angle count) pair: one slice, 24 of the angles, vs the FP64 oracle
+        i = members[0]
+        idx = np.arange(0, nt, nt // 24)[:24]
+        x = deltas[i][40:41].cpu().numpy()
+        ref = usfft.tomo_R(x, th[idx], np.float64) * math.sqrt(len(idx) / nt)
+        got = tomo.R(deltas[i])[torch.from_numpy(idx).cuda()][:, 40:41].cpu().numpy()
+        e = rel(got, ref)
+        print(f"mosaic n={n} ntheta={nt}: {len(members)} tiles bitwise equal to the operator chain; "
+              f"R rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e < 1e-5
+        tomo.free()
+        prop.free()
+        torch.cuda.empty_cache()
+    # D of every width on two frames of 96 rows vs the FP64 oracle
+    for n in sorted({t[0] for t in tiles}):
+        prop = Propagation(n, 96, 2, 1, WL30, MOSAIC_VOX, [MOSAIC_Z])
+        x = (np.random.default_rng(n).standard_normal((2, 96, n))
+             + 1j * np.random.default_rng(n + 1).standard_normal((2, 96, n))).astype(np.complex64)
+        fk = fr.fresnel_kernel_2d(n, 96, WL30, MOSAIC_VOX, [MOSAIC_Z], np.float64)
+        e = rel(prop.D(torch.from_numpy(x).cuda(), 0).cpu().numpy(), fr.D(x, fk[0], np.float64))
+        print(f"mosaic D 96x{n}: rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e < 1e-5
+        prop.free()
+    # the whole intensity of the narrowest tile at three angles: R of all 96 rows, psi, D, |.|^2 in FP64
+    i = min(range(len(tiles)), key=lambda t: (tiles[t][0], len(tiles[t][2])))
+    n, nz, th = tiles[i]
+    idx = np.array([0, len(th) // 3, (2 * len(th)) // 3])
+    x = deltas[i].cpu().numpy()
+    p = np.concatenate([usfft.tomo_R(x[z0:z0 + 8], th[idx], np.float64) for z0 in range(0, nz, 8)], axis=1)
+    p *= math.sqrt(len(idx) / len(th))
+    fk = fr.fresnel_kernel_2d(n, nz, WL30, MOSAIC_VOX, [MOSAIC_Z], np.float64)
+    ref = fr.abs2(fr.D(fr.psi_from_proj(p, k, beta, np.float64), fk[0], np.float64))
+    e = rel(outs[i][torch.from_numpy(idx).cuda()].cpu().numpy(), ref)
+    print(f"mosaic tile {i} (n={n}, {nz} rows, ntheta={len(th)}): intensity at 3 angles rel-L2 vs FP64 {e:.2e}",
+          flush=True)
+    assert e < 1e-5
+    del deltas, outs
+    torch.cuda.empty_cache()
