@@ -196,19 +196,21 @@ def test_model_1024_gradient_and_finite_differences():
         rgd = fm.tomo.RT((-fm.k * cg.imag).contiguous())
         rgb = fm.tomo.RT((-fm.k * cg.real).contiguous())
         del p, b, psi, u, resid, G, cg
-    el = abs(float(loss) - L_ref) / L_ref
+    el = abs(float(loss.detach()) - L_ref) / L_ref
     ed = float((gd - rgd).norm() / rgd.norm())
     eb = float((gb - rgb).norm() / rgb.norm())
     print(f"model 1024: loss vs operator chain {el:.2e}, grad delta {ed:.2e}, grad beta {eb:.2e}", flush=True)
     assert el < 1e-5 and ed < 1e-4 and eb < 1e-4
-    L0 = float(loss)
+    L0 = float(loss.detach())
     rng = np.random.default_rng(8)
     for which, g in ((0, gd), (1, gb)):
         idx = torch.from_numpy(rng.choice(delta.numel(), 64, replace=False)).cuda()
         v = torch.zeros(delta.numel(), dtype=torch.float64, device="cuda")
         v[idx] = torch.sign(g.view(-1)[idx].double()) * torch.from_numpy(rng.uniform(0.5, 1.5, 64)).cuda()
         dirderiv = float(torch.dot(g.view(-1).double(), v))
-        h = 1e-4 * L0 / abs(dirderiv)
+        # a step that changes L by 2e-5 of itself: the central difference's h^2 term is ~1e-4 of the slope
+        # there (at 1e-4 of L it was 1.5e-3), and the FP32 rounding of the loss stays ~1e-5 of the difference
+        h = 2e-5 * L0 / abs(dirderiv)
         vols_p, vols_m = [delta, beta], [delta, beta]
         base = (delta, beta)[which].double()
         vols_p[which] = (base + h * v.view(delta.shape)).float()
