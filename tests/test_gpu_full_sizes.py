@@ -208,17 +208,20 @@ def test_model_1024_gradient_and_finite_differences():
         v = torch.zeros(delta.numel(), dtype=torch.float64, device="cuda")
         v[idx] = torch.sign(g.view(-1)[idx].double()) * torch.from_numpy(rng.uniform(0.5, 1.5, 64)).cuda()
         dirderiv = float(torch.dot(g.view(-1).double(), v))
-        # a step that changes L by 2e-5 of itself: the central difference's h^2 term is ~1e-4 of the slope
-        # there (at 1e-4 of L it was 1.5e-3), and the FP32 rounding of the loss stays ~1e-5 of the difference
-        h = 2e-5 * L0 / abs(dirderiv)
+        # a step that changes L by 1e-4 of itself; the volumes are float32 (delta ~ 1e-7 .. 1e-6), so the step
+        # as applied is rounded per voxel by a few percent: the prediction uses the applied difference itself
+        h = 1e-4 * L0 / abs(dirderiv)
         vols_p, vols_m = [delta, beta], [delta, beta]
         base = (delta, beta)[which].double()
         vols_p[which] = (base + h * v.view(delta.shape)).float()
         vols_m[which] = (base - h * v.view(delta.shape)).float()
-        fd = (float(fm.loss(*vols_p, data)) - float(fm.loss(*vols_m, data))) / (2 * h)
-        err = abs(fd - dirderiv) / abs(dirderiv)
-        print(f"model 1024 FD ({'delta' if which == 0 else 'beta'}): 64-voxel directional rel err {err:.2e}",
-              flush=True)
+        applied = vols_p[which].double() - vols_m[which].double()
+        predicted = float(torch.sum(g.double() * applied))
+        fd = float(fm.loss(*vols_p, data)) - float(fm.loss(*vols_m, data))
+        err = abs(fd - predicted) / abs(predicted)
+        rounding = abs(predicted / (2 * h * dirderiv) - 1)
+        print(f"model 1024 FD ({'delta' if which == 0 else 'beta'}): 64-voxel directional rel err {err:.2e} "
+              f"(float32 rounding of the step {rounding:.1e})", flush=True)
         assert err < 1e-3
     fm.free()
     torch.cuda.empty_cache()
