@@ -208,20 +208,25 @@ def test_model_1024_gradient_and_finite_differences():
         v = torch.zeros(delta.numel(), dtype=torch.float64, device="cuda")
         v[idx] = torch.sign(g.view(-1)[idx].double()) * torch.from_numpy(rng.uniform(0.5, 1.5, 64)).cuda()
         dirderiv = float(torch.dot(g.view(-1).double(), v))
-        # a step that changes L by 1e-4 of itself; the volumes are float32 (delta ~ 1e-7 .. 1e-6), so the step
-        # as applied is rounded per voxel by a few percent: the prediction uses the applied difference itself
+        # central differences at the steps h and 2 h, h changing L by 1e-4 of itself.  At these sizes the
+        # h^2 term of a central difference is ~1e-3 of the slope (the phase makes L strongly non-quadratic in
+        # delta) and the FP32 rounding of the loss ~2e-4 at h, so the two are combined by Richardson
+        # extrapolation, (4 D(h) - D(2h)) / 3, which removes the h^2 term.  Each difference is divided by its
+        # prediction from the step as applied in float32.
         h = 1e-4 * L0 / abs(dirderiv)
-        vols_p, vols_m = [delta, beta], [delta, beta]
         base = (delta, beta)[which].double()
-        vols_p[which] = (base + h * v.view(delta.shape)).float()
-        vols_m[which] = (base - h * v.view(delta.shape)).float()
-        applied = vols_p[which].double() - vols_m[which].double()
-        predicted = float(torch.sum(g.double() * applied))
-        fd = float(fm.loss(*vols_p, data)) - float(fm.loss(*vols_m, data))
-        err = abs(fd - predicted) / abs(predicted)
-        rounding = abs(predicted / (2 * h * dirderiv) - 1)
+        ratios = []
+        for step in (h, 2 * h):
+            vols_p, vols_m = [delta, beta], [delta, beta]
+            vols_p[which] = (base + step * v.view(delta.shape)).float()
+            vols_m[which] = (base - step * v.view(delta.shape)).float()
+            applied = vols_p[which].double() - vols_m[which].double()
+            predicted = float(torch.sum(g.double() * applied))
+            ratios.append((float(fm.loss(*vols_p, data)) - float(fm.loss(*vols_m, data))) / predicted)
+        central = abs(ratios[0] - 1)
+        err = abs((4 * ratios[0] - ratios[1]) / 3 - 1)
         print(f"model 1024 FD ({'delta' if which == 0 else 'beta'}): 64-voxel directional rel err {err:.2e} "
-              f"(float32 rounding of the step {rounding:.1e})", flush=True)
+              f"(central difference alone {central:.2e}, at 2h {abs(ratios[1] - 1):.2e})", flush=True)
         assert err < 1e-3
     fm.free()
     torch.cuda.empty_cache()
