@@ -63,6 +63,19 @@ int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double
  * Same operator as processing/tomo.py:121-151 (TomoReal.R). */
 int mm_radon_fwd(mm_radon* plan, const float* obj_dev, float* sino_dev, int nz, void* stream);
 
+/* R that allocates nothing (no counterpart in the reference): for callers that
+ * cannot allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops).  Same operator, kernels and result as mm_radon_fwd; the
+ * half spectrum and the representatives live in one static device array of
+ * 128 MiB per GPU (held until the process ends) instead of the plan's
+ * workspace.  Needs ceil256(8 nz (n/4 + 1) 8 n) + 8 ntheta nz (n/2 + 1) bytes
+ * of the 128 MiB (n = 256 with 384 angles: up to 91 slices per call; n = 1024
+ * with 1024 angles: 6; n = 2048 with 2048 angles: 1); larger calls return
+ * MM_ERR_ARG.  The plan's device must be current.  Every call on
+ * a GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be ordered by the caller. */
+int mm_radon_fwd_noalloc(mm_radon* plan, const float* obj_dev, float* sino_dev, int nz, void* stream);
+
 /* RT: sino (ntheta, nz, n) float32 (sino_is_c64 = 0) or complex64 (= 1)
  *  -> obj (nz, n, n) of the same element type.
  * Same operator as processing/tomo.py:153-188 (TomoReal.RT): real part for a

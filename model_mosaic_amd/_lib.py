@@ -38,6 +38,7 @@ _SIGS = {
     "mm_version": (C.c_int, []),
     "mm_radon_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_uint]),
     "mm_radon_fwd": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "mm_radon_fwd_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "mm_radon_adj": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "mm_radon_tune": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mm_radon_plan_check": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double)]),

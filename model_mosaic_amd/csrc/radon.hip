@@ -40,6 +40,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -1974,14 +1975,10 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
 }
 }  // namespace mm
 
-extern "C" {
-
-int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* stream) {
-  MM_ARG_CHECK(p != nullptr, "mm_radon_fwd: plan is NULL");
-  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
-               "mm_radon_fwd: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
-  MM_ARG_CHECK(obj && sino, "mm_radon_fwd: null pointer");
-  hipStream_t st = (hipStream_t)stream;
+// R on the half-spectrum buffer p->fde and the representatives p->gh (the plan's own, a batch's shared ones,
+// or the static scratch below).  Every element a stage reads was written by the stage before it in the same
+// call, so the buffers need no clearing.
+static int radon_fwd_chain(mm_radon* p, const float* obj, float* sino, int nz, hipStream_t st) {
   const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 0, 0, st))) return rc;
@@ -1993,6 +1990,73 @@ int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* strea
   MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
   if ((rc = check_launch("r4_line_ifft"))) return rc;
   return mark(p, 3, 1, st);
+}
+
+// ---- R without allocation ---------------------------------------------------
+// The half spectrum and the representatives of mm_radon_fwd_noalloc live in one file-scope device array.  Its
+// name is only a host-side handle: host code uses the address hipGetSymbolAddress returns for the current
+// device (each device holds its own copy), resolved once per device and kept in scratch_dev.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static unsigned char* scratch_dev[kScratchDevices];
+static std::mutex scratch_mu;
+
+// Bytes of scratch R needs for nz slices: the half spectrum (rounded up to 256 bytes), then the representatives.
+static size_t noalloc_bytes(const mm_radon* p, int nz) {
+  const size_t fde_b = (sizeof(float2) * (size_t)nz * (size_t)p->zs + 255) / 256 * 256;
+  return fde_b + sizeof(float2) * (size_t)p->ntheta * (size_t)nz * (size_t)p->H;
+}
+
+static int scratch_base(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    MM_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MM_ARG_CHECK(ptr != nullptr, "mm_radon_fwd_noalloc: no device address for the scratch array");
+    scratch_dev[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = scratch_dev[dev];
+  return MM_OK;
+}
+
+extern "C" {
+
+int mm_radon_fwd(mm_radon* p, const float* obj, float* sino, int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_fwd: plan is NULL");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
+               "mm_radon_fwd: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
+  MM_ARG_CHECK(obj && sino, "mm_radon_fwd: null pointer");
+  return radon_fwd_chain(p, obj, sino, nz, (hipStream_t)stream);
+}
+
+int mm_radon_fwd_noalloc(mm_radon* p, const float* obj, float* sino, int nz, void* stream) {
+  MM_ARG_CHECK(p != nullptr, "mm_radon_fwd_noalloc: plan is NULL");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max,
+               "mm_radon_fwd_noalloc: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
+  MM_ARG_CHECK(obj && sino, "mm_radon_fwd_noalloc: null pointer");
+  const size_t need = noalloc_bytes(p, nz);
+  MM_ARG_CHECK(need <= kScratchBytes, "mm_radon_fwd_noalloc: n=" + std::to_string(p->n) + ", ntheta=" +
+                                          std::to_string(p->ntheta) + ", nz=" + std::to_string(nz) + " needs " +
+                                          std::to_string(need) + " bytes of scratch, the static buffer holds " +
+                                          std::to_string(kScratchBytes) + " (use mm_radon_fwd or fewer slices)");
+  int dev = -1;
+  MM_HIP_TRY(hipGetDevice(&dev));
+  MM_ARG_CHECK(dev == p->dev, "mm_radon_fwd_noalloc: the plan belongs to device " + std::to_string(p->dev) +
+                                  ", device " + std::to_string(dev) + " is current");
+  MM_ARG_CHECK(dev >= 0 && dev < kScratchDevices, "mm_radon_fwd_noalloc: device index " + std::to_string(dev) +
+                                                      " beyond the " + std::to_string(kScratchDevices) + " supported");
+  unsigned char* base = nullptr;
+  int rc = scratch_base(dev, &base);
+  if (rc) return rc;
+  float2* const own_fde = p->fde;
+  float2* const own_gh = p->gh;
+  p->fde = reinterpret_cast<float2*>(base);
+  p->gh = reinterpret_cast<float2*>(base + (need - sizeof(float2) * (size_t)p->ntheta * nz * p->H));
+  rc = radon_fwd_chain(p, obj, sino, nz, (hipStream_t)stream);
+  p->fde = own_fde;
+  p->gh = own_gh;
+  return rc;
 }
 
 static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t st);
