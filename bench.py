@@ -18,9 +18,10 @@ Also reported in the same JSON line:
   * Fresnel D and DT of 128 complex64 frames 2048 x 2048 (psi = exp(i p - 1e-3 p)
     of a smooth random field p with a 2 rad peak, 30 keV, pixel 0.69 um,
     z in {0.05, 0.2, 0.8} m) in frames/s, against B_D = 4 nz n 8 bytes per frame;
-  * cpu_baseline: the NumPy FP32 restatement (oracle) of R+RT on the host,
-    one n=2048 slice at 256 and 512 angles, extrapolated linearly in ntheta
-    to the 2048-angle slice, reported in slices/s.
+  * cpu_baseline: the NumPy FP32 restatement (oracle) of R+RT on the host
+    cores (one thread and all cores of the affinity set, at most 16), for one
+    2048^2 slice at 2048 angles and for nz=8 x 256^2 at 384 angles (median
+    of 5), with the CPU model.
 """
 from __future__ import annotations
 
@@ -78,32 +79,112 @@ def stage_bytes(n, nt):
     }
 
 
+_CPU_JOB = {}
+
+
+def _cpu_R_block(bounds):
+    """R of the whole volume at the angles [b0, b1) (oracle, FP32), rescaled to the full angle count: the angle
+    count enters R only through the 1 / sqrt(n ntheta) of phi."""
+    import numpy as np
+
+    from oracle import usfft
+    b0, b1 = bounds
+    x, th = _CPU_JOB["x"], _CPU_JOB["theta"]
+    return usfft.tomo_R(x, th[b0:b1], np.float32) * np.float32(math.sqrt((b1 - b0) / len(th)))
+
+
+def _cpu_RT_block(args):
+    """RT of the angles [b0, b1) of a sinogram (oracle, FP32), rescaled like _cpu_R_block; the blocks sum to RT."""
+    import numpy as np
+
+    from oracle import usfft
+    (b0, b1), sino = args
+    th = _CPU_JOB["theta"]
+    return usfft.tomo_RT(sino, th[b0:b1], dtype=np.float32) * np.float32(math.sqrt((b1 - b0) / len(th)))
+
+
+def _cpu_R_RT(pool, workers):
+    import numpy as np
+    nt = len(_CPU_JOB["theta"])
+    edges = np.linspace(0, nt, workers + 1).astype(int)
+    blocks = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    if pool is None:
+        sino = _cpu_R_block((0, nt))
+        return sino, _cpu_RT_block(((0, nt), sino))
+    sino = np.concatenate(list(pool.map(_cpu_R_block, blocks)), axis=0)
+    back = sum(pool.map(_cpu_RT_block, [(b, sino[b[0]:b[1]]) for b in blocks]))
+    return sino, back
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline():
-    """Oracle (NumPy FP32 restatement) R+RT of one n=2048 slice, single thread (any BLAS / OpenMP pool
-    held to one thread; NumPy's FFT and element-wise work are single-threaded)."""
+    """The oracle's FP32 NumPy restatement of R+RT on the host cores, run before the GPU is initialised (the
+    worker processes are forked).  Two workloads: the small volume nz=8 x 256^2 at 384 angles (one warm-up, median
+    of 5 runs, on one thread and on all cores) and one 2048^2 slice at all 2048 angles (one run on all cores; on
+    one thread it takes ~40 s).  "All cores" = the cores of this process's affinity set, at most 16: angle blocks
+    of R and of RT in worker processes, the RT blocks summed.
+    The value is the all-cores rate of the 2048-angle n=2048 slice in slices/s."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import statistics
+
     import numpy as np
     from threadpoolctl import threadpool_limits
 
+    from model_mosaic_amd import phantom
     from oracle import usfft
 
-    n = 2048
-    x = np.random.default_rng(0).standard_normal((1, n, n)).astype(np.float32)
-    times = {}
-    with threadpool_limits(limits=1):
-        for nt in (512, 1024):
-            th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    ctx = mp.get_context("fork")
+    res = {"cpu_model": _cpu_model(), "workers": workers}
+
+    def timed(pool, reps):
+        times = []
+        for _ in range(reps):
             t0 = time.perf_counter()
-            r = usfft.tomo_R(x, th, np.float32)
-            usfft.tomo_RT(r, th, dtype=np.float32)
-            times[nt] = time.perf_counter() - t0
-    per_slice = times[1024] + (times[1024] - times[512]) * (2048 - 1024) / 512
+            out = _cpu_R_RT(pool, workers)
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times), out
+
+    with threadpool_limits(limits=1):
+        # small volume: 8 slices of the 12-ellipsoid phantom, 384 angles
+        _CPU_JOB.update(x=phantom.ellipsoids(8, 256, seed=0).numpy(), theta=phantom.angles(384))
+        timed(None, 1)
+        t1, (sino, _) = timed(None, 5)
+        with cf.ProcessPoolExecutor(workers, mp_context=ctx) as pool:
+            timed(pool, 1)
+            tw, _ = timed(pool, 5)
+        r64 = usfft.tomo_R(_CPU_JOB["x"], _CPU_JOB["theta"], np.float64)
+        err = float(np.linalg.norm(sino - r64) / np.linalg.norm(r64))
+        res["small"] = {"workload": "nz=8 x 256^2, 384 angles, R+RT", "median_of": 5,
+                        "one_thread_s": round(t1, 3), "one_thread_slices_per_s": round(8 / t1, 3),
+                        "all_cores_s": round(tw, 3), "all_cores_slices_per_s": round(8 / tw, 3),
+                        "R_rel_l2_fp32_vs_fp64": err}
+        # one n=2048 slice at all 2048 angles
+        _CPU_JOB.update(x=phantom.ellipsoids(64, 2048, seed=0, noise=0.01, rows=(32, 33)).numpy(), theta=phantom.angles(2048))
+        with cf.ProcessPoolExecutor(workers, mp_context=ctx) as pool:
+            tw2, _ = timed(pool, 1)
+        res["n2048"] = {"workload": "1 slice 2048^2, 2048 angles, R+RT", "all_cores_s": round(tw2, 3)}
+    _CPU_JOB.clear()
     return {
-        "value": 1.0 / per_slice,
+        "value": 1.0 / tw2,
         "unit": "slices/s",
-        "cores": 1,
+        "cores": workers,
         "kind": "port",
-        "sample": (f"oracle.usfft FP32 R+RT of 1 slice n=2048 at 512 angles ({times[512]:.1f} s) and 1024 angles "
-                   f"({times[1024]:.1f} s), linearly extrapolated in ntheta to 2048 angles ({per_slice:.1f} s/slice)"),
+        "sample": (f"oracle.usfft FP32 R+RT, one 2048^2 slice at 2048 angles: {tw2:.2f} s on {workers} worker "
+                   f"processes (angle blocks); nz=8 x 256^2 at 384 angles: median of 5 "
+                   f"{tw:.3f} s on {workers} workers, {t1:.3f} s on one thread; CPU {res['cpu_model']}"),
+        "detail": res,
     }
 
 
@@ -371,6 +452,13 @@ def main():
               "communication between GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    cpu = None
+    if not a.no_cpu and not a.only_mosaic:
+        # before the GPU is initialised: the baseline forks worker processes
+        print("[bench] cpu baseline (oracle, host cores)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+        print(f"[bench] cpu baseline {cpu['sample']}", file=sys.stderr, flush=True)
+
     import numpy as np
     import torch
 
@@ -522,11 +610,6 @@ def main():
         mosaic = run_mosaic(torch)
         print(f"[bench] mosaic {mosaic['s_per_batch'] * 1e3:.1f} ms per batch of {mosaic['tiles']} tiles, "
               f"{mosaic['projections_per_s']:.0f} projections/s", file=sys.stderr, flush=True)
-
-    cpu = None
-    if not a.no_cpu:
-        print("[bench] cpu baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline()
 
     out = {
         "metric": "USFFT R+RT slices/s @n=2048 & fwd+bwd model s/iter; % HBM roofline",
