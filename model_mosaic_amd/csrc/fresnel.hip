@@ -168,6 +168,115 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(cons
   }
 }
 
+// ---- mirror padding as a DCT: transforms of the frame length ---------------
+// The symmetric pad of a line x (length N) to 2N is the half-sample-symmetric
+// extension of x shifted by N/2, and the chirp K is even in the frequency
+// index, so  crop IFFT_2N K FFT_2N pad  =  DCT-III . K . DCT-II  on N points:
+//   C_k = sum_j x_j 2 cos(pi k (2j+1) / 2N),   z_j = sum_k w_k K_k C_k cos(pi k (2j+1) / 2N)
+// (w_0 = 1, else 2; K carries 1/(2N)).  The operator is complex-symmetric, so
+// its adjoint is the same pipeline with conj(K): no fold-add.  Both DCTs run on
+// one complex FFT of length N (Makhoul's reordering v_j = x_{2j}, v_{N-1-j} =
+// x_{2j+1}):  with V = FFT(v) and c_k = exp(-i pi k / 2N),
+//   C_k = c_k V_k + conj(c_k) V_{N-k},   C_{N-k} = i (c_k V_k - conj(c_k) V_{N-k}),
+//   U_k = conj(c_k) (K_k C_k - i K_{N-k} C_{N-k}),   x_{perm(j)} = (IFFT U)_j.
+// The partner V_{N-k} of thread t's points sits in thread (TPC - t) mod TPC:
+// one LDS exchange between the two transforms.  The j < N/2 half of the
+// reordering is the registers m < PT/2, so it is fixed at compile time.
+template <class Sh>
+__device__ __forceinline__ int dct_src(int m, int t) {
+  constexpr int N = Sh::PT * Sh::TPC;
+  const int j = t + m * Sh::TPC;
+  return m < Sh::PT / 2 ? 2 * j : 2 * N - 1 - 2 * j;
+}
+
+// c_k = exp(-i pi k / 2N) from the length-2N table (tw2 = bank + tw_offset(2N)); hw = exp(-i pi / 2N).
+__device__ __forceinline__ float2 dct_twiddle(int k, const float2* __restrict__ tw2, float2 hw) {
+  const float2 w = tw2[k >> 1];
+  return (k & 1) ? cmul(w, hw) : w;
+}
+
+// Between the forward and the inverse transform: v holds V_k (k = t + m TPC) on entry and U_k on exit.
+template <class Sh, bool CONJ_K>
+__device__ __forceinline__ void dct_mid(float2 (&v)[Sh::PT], float2* __restrict__ x, int t,
+                                        const float2* __restrict__ K, const float2* __restrict__ tw2, float2 hw) {
+  constexpr int N = Sh::PT * Sh::TPC;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int k = t + m * Sh::TPC;
+    const int kr = k == 0 ? 0 : N - k;
+    const float2 vr = x[lpad(kr)];
+    const float2 c = dct_twiddle(k, tw2, hw);
+    const float2 a = cmul(c, v[m]), b = cmul(cconj(c), vr);
+    const float2 ck = cadd(a, b);
+    const float2 d = csub(a, b);
+    const float2 cr = make_float2(-d.y, d.x);  // C_{N-k} = i (a - b)
+    float2 kk = K[k], kn = K[N - k];
+    if (CONJ_K) {
+      kk.y = -kk.y;
+      kn.y = -kn.y;
+    }
+    const float2 y = cmul(kk, ck), yr = cmul(kn, cr);
+    v[m] = cmul(cconj(c), make_float2(y.x + yr.y, y.y - yr.x));  // conj(c) (Y_k - i Y_{N-k})
+  }
+  __syncthreads();  // the inverse transform's first pass rewrites x
+}
+
+// Column pass on the DCT form: frames (b, nz, n), transforms of length L = nz down the columns.
+template <int L, bool CONJ_K>
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(ColShape<L>::NT) void dct_col_pass(
+    const float2* __restrict__ in, float2* __restrict__ out, int n, const float2* __restrict__ K,
+    const float2* __restrict__ tw, const float2* __restrict__ tw2, float2 hw) {
+  using Sh = ColShape<L>;
+  __shared__ float2 lds[Sh::LDS];
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const unsigned lw = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int bx = (int)(lw % gridDim.x), frame = (int)(lw / gridDim.x);
+  const int col = bx * Sh::C + c;
+  const bool live = col < n;
+  const long long fo = (long long)frame * L * n + col;
+  const float2* src = in + fo;
+  float2* dst = out + fo;
+  float2* x = lds + c * Sh::S;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) v[m] = live ? src[(long long)dct_src<Sh>(m, t) * n] : make_float2(0.f, 0.f);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
+  dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
+  if (!live) return;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) dst[(long long)dct_src<Sh>(m, t) * n] = v[m];
+}
+
+// Row pass on the DCT form: rows of length L = n, C rows per block, nrows = b * nz.
+template <int L, bool CONJ_K>
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void dct_row_pass(
+    const float2* __restrict__ in, float2* __restrict__ out, long long nrows, const float2* __restrict__ K,
+    const float2* __restrict__ tw, const float2* __restrict__ tw2, float2 hw) {
+  using Sh = RowReg<L>;
+  __shared__ float2 lds[Sh::LDS];
+  int c, t;
+  Sh::map(threadIdx.x, c, t);
+  const long long row = (long long)blockIdx.x * Sh::C + c;
+  const bool valid = row < nrows;
+  const float2* src = in + (valid ? row : 0) * L;
+  float2* dst = out + (valid ? row : 0) * L;
+  float2* x = lds + c * Sh::S;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? src[dct_src<Sh>(m, t)] : make_float2(0.f, 0.f);
+  fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
+  dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
+  fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
+  if (!valid) return;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) dst[dct_src<Sh>(m, t)] = v[m];
+}
+
 }  // namespace fresnel
 }  // namespace mm
 
@@ -177,6 +286,7 @@ using namespace mm::fresnel;
 struct mm_fresnel {
   int dev = 0;
   int n = 0, nz = 0, ndist = 0, pad_mode = 0;
+  bool dct = false;      // mirror padding on the DCT form (both frame sides are DCT lengths)
   float2* Kx = nullptr;  // ndist x 2n
   float2* Ky = nullptr;  // ndist x 2nz
   const float2* tw = nullptr;
@@ -206,6 +316,32 @@ static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, 
 }
 
 
+// exp(-i pi / 2N) in FP64, rounded to complex64
+static float2 half_step(int N) {
+  const double a = -M_PI / (2.0 * N);
+  return make_float2((float)std::cos(a), (float)std::sin(a));
+}
+
+// The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48) and the length-2N twiddles.
+static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && 2 * N <= kMaxFFT; }
+
+template <int L, bool CONJ_K>
+static void launch_dct_col(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
+  using Sh = ColShape<L>;
+  hipLaunchKernelGGL((dct_col_pass<L, CONJ_K>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
+                     dim3(Sh::NT), 0, st, in, out, p->n, p->Ky + (size_t)j * 2 * L, p->tw + tw_offset(L),
+                     p->tw + tw_offset(2 * L), half_step(L));
+}
+
+template <int L, bool CONJ_K>
+static void launch_dct_row(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
+  using Sh = RowReg<L>;
+  const long long rows = (long long)nframes * p->nz;
+  hipLaunchKernelGGL((dct_row_pass<L, CONJ_K>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, in,
+                     out, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), p->tw + tw_offset(2 * L),
+                     half_step(L));
+}
+
 static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, int j, void* stream, bool adj) {
   MM_ARG_CHECK(p != nullptr, "mm_fresnel: plan is NULL");
   MM_ARG_CHECK(nframes >= 0, "mm_fresnel: nframes < 0");
@@ -216,6 +352,21 @@ static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, 
   hipStream_t st = (hipStream_t)stream;
   const float2* src = (const float2*)in;
   float2* dst = (float2*)out;
+  if (p->pad_mode == 0 && p->dct) {  // mirror padding: DCT form, DT = the same with conj(K)
+    if (adj) {
+      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, true>(p, src, dst, nframes, j, st)));
+    } else {
+      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, false>(p, src, dst, nframes, j, st)));
+    }
+    int rc = check_launch("fresnel dct_col_pass");
+    if (rc) return rc;
+    if (adj) {
+      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, true>(p, dst, dst, nframes, j, st)));
+    } else {
+      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, false>(p, dst, dst, nframes, j, st)));
+    }
+    return check_launch("fresnel dct_row_pass");
+  }
   const int Ly = 2 * p->nz, Lx = 2 * p->n;
   if (adj) {
     MM_DISPATCH_FFT(Ly, (launch_col_pass<LL, true>(p, src, dst, nframes, j, st)));
@@ -266,6 +417,7 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
   p->nz = nz;
   p->ndist = ndist;
   p->pad_mode = pad_mode;
+  p->dct = pad_mode == 0 && dct_len_ok(n) && dct_len_ok(nz);
   double* ddist = nullptr;
   if (hipMalloc(&ddist, sizeof(double) * ndist) != hipSuccess) return bail(fail(MM_ERR_OOM, "fresnel distances"));
   if (hipMalloc(&p->Kx, sizeof(float2) * (size_t)ndist * 2 * n) != hipSuccess ||
