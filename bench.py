@@ -314,12 +314,20 @@ def run_model(torch, steps=3, warmup=1):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     bdir = b_dir(n, nt)
-    frame = nz * n
-    # 4 R + 2 RT directions, 2 D + 1 DT separable passes (4 x 8 bytes per pixel), 5 element-wise passes
-    alg = 6 * nz * bdir + 3 * nt * 4 * frame * 8 + nt * frame * (8 + 12 + 16 + 16 + 16)
+    px = nt * nz * n
+    # yardstick: R of delta and beta and RT of the two gradients (2 directions each, B_dir per slice), a fused
+    # psi-build + D + |.|^2 (f32 in, c64 through, f32 out: 24 B/px), D recompute and DT (two separable passes
+    # of complex64 each: 32 B/px each) and the chain rule (16 B/px)
+    yard = 4 * nz * bdir + px * (24 + 32 + 32 + 16)
+    # this implementation: the same four Radon directions; forward = psi-build inside D's column pass (8 B/px
+    # in, 8 out) + row pass keeping u and reading data (8 + 8 + 4); backward from the kept p, b, u = residual
+    # inside DT's column pass (8 + 4 in, 8 out) + chain rule inside its row pass (8 + 8 in, 8 out)
+    impl = 4 * nz * bdir + px * (16 + 20 + 20 + 24)
     model.free()
-    return {"s_per_iter": dt, "loss": float(loss), "config": f"nz={nz} n={n} ntheta={nt} 30keV z=0.2m pixel=1.38um",
-            "algorithmic_GB": alg / 1e9, "hbm_frac": alg / dt / 1e9 / PEAK_HBM_GBS}
+    return {"s_per_iter": dt, "loss": float(loss.detach()),
+            "config": f"nz={nz} n={n} ntheta={nt} 30keV z=0.2m pixel=1.38um",
+            "yardstick_GB": yard / 1e9, "hbm_frac": yard / dt / 1e9 / PEAK_HBM_GBS,
+            "implementation_GB": impl / 1e9, "implementation_hbm_frac": impl / dt / 1e9 / PEAK_HBM_GBS}
 
 
 def run_fbp(torch, reps=5):

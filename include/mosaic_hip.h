@@ -257,16 +257,23 @@ int mm_plain_scatter(const void* g_dev, void* G_dev, const float* theta_dev, int
  *   L = sum (I - data)^2  (accumulated in FP64).
  * Gradient: G = DT_j(4 (I - data) u), dL/dp = -k Im(conj(G) psi),
  *   dL/db = -k Re(conj(G) psi), dL/ddelta = RT(dL/dp), dL/dbeta = RT(dL/db).
- * psi and u are recomputed in the backward pass, never stored.
  * The Fresnel plan's frame (nz, n) must equal (nz_call, radon n).
+ * mm_model_fwd leaves p, b and u in the model's workspace and returns a token
+ * for them (token may be NULL).  mm_model_bwd given that token, with no other
+ * mm_model_fwd / mm_model_bwd on the model since and the same delta, beta,
+ * data and j (the caller guarantees they are unchanged), starts from p, b, u
+ * and recomputes only psi; with token 0 or a stale token it runs the forward
+ * first.  psi is never stored.  With mirror padding (pad_mode 0) and frame
+ * sides of 2^k >= 16 or 3 * 2^k >= 48 the element-wise steps run inside the
+ * Fresnel passes.
  */
 int mm_model_create(mm_model** model, mm_radon* radon, mm_fresnel* fresnel, int nz, float k);
 /* loss_dev: one double on the device. */
 int mm_model_fwd(mm_model* model, const float* delta_dev, const float* beta_dev, const float* data_dev, int j,
-                 double* loss_dev, void* stream);
+                 double* loss_dev, unsigned long long* token, void* stream);
 /* grad_* (nz, n, n) float32, multiplied by gscale (the upstream gradient). */
 int mm_model_bwd(mm_model* model, const float* delta_dev, const float* beta_dev, const float* data_dev, int j,
-                 float gscale, float* grad_delta_dev, float* grad_beta_dev, void* stream);
+                 float gscale, unsigned long long token, float* grad_delta_dev, float* grad_beta_dev, void* stream);
 size_t mm_model_workspace_bytes(const mm_model* model);
 void mm_model_destroy(mm_model* model);
 

@@ -75,8 +75,8 @@ _SIGS = {
     "mm_plain_gather": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "mm_plain_scatter": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "mm_model_create": (C.c_int, [C.POINTER(_vp), _vp, _vp, C.c_int, C.c_float]),
-    "mm_model_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
-    "mm_model_bwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, _vp, _vp, _vp]),
+    "mm_model_fwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, C.POINTER(C.c_ulonglong), _vp]),
+    "mm_model_bwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_ulonglong, _vp, _vp, _vp]),
     "mm_model_workspace_bytes": (C.c_size_t, [_vp]),
     "mm_model_destroy": (None, [_vp]),
 }

@@ -169,6 +169,18 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(cons
 }
 
 // ---- mirror padding as a DCT: transforms of the frame length ---------------
+// Workgroup sizes of the DCT-form passes (complex points per workgroup; build-time shape knobs).
+#ifndef MM_FRESNEL_DCT_ROW_BATCH
+#define MM_FRESNEL_DCT_ROW_BATCH 4096
+#endif
+#ifndef MM_FRESNEL_DCT_COL_BATCH
+#define MM_FRESNEL_DCT_COL_BATCH kRowBatch
+#endif
+template <int L>
+using DctColShape = RegShape<L, (L >= 2048 ? MM_FRESNEL_DCT_COL_BATCH : kBatchElems)>;
+template <int L>
+using DctRowShape = RegShape<L, MM_FRESNEL_DCT_ROW_BATCH, 0, false>;
+
 // The symmetric pad of a line x (length N) to 2N is the half-sample-symmetric
 // extension of x shifted by N/2, and the chirp K is even in the frequency
 // index, so  crop IFFT_2N K FFT_2N pad  =  DCT-III . K . DCT-II  on N points:
@@ -224,12 +236,90 @@ __device__ __forceinline__ void dct_mid(float2 (&v)[Sh::PT], float2* __restrict_
   __syncthreads();  // the inverse transform's first pass rewrites x
 }
 
+// Load / store policies of the DCT-form passes.  An index is the flat element
+// index of the (frames, nz, n) array.  The plain operator moves complex64; the
+// forward model builds psi = exp(-k b) e^{i k p} from the two float32
+// sinograms while loading, and its backward forms 4 (|u|^2 - d) u while
+// loading and the chain rule while storing, so none of these element-wise
+// steps is a pass of its own.
+struct LoadC64 {
+  const float2* __restrict__ in;
+  __device__ __forceinline__ float2 operator()(long long i) const { return in[i]; }
+};
+struct LoadPsi {  // psi of step3_propagation.py:94-103 with separate phase / attenuation sinograms
+  const float* __restrict__ p;
+  const float* __restrict__ b;
+  float k;
+  __device__ __forceinline__ float2 operator()(long long i) const {
+    float sn, cs;
+    sincosf(k * p[i], &sn, &cs);
+    const float at = expf(-k * b[i]);
+    return make_float2(at * cs, at * sn);
+  }
+};
+struct LoadResidual {  // 4 (|u|^2 - d) u
+  const float2* __restrict__ u;
+  const float* __restrict__ d;
+  __device__ __forceinline__ float2 operator()(long long i) const {
+    const float2 v = u[i];
+    const float r = 4.0f * (v.x * v.x + v.y * v.y - d[i]);
+    return make_float2(r * v.x, r * v.y);
+  }
+};
+struct StoreC64 {
+  float2* __restrict__ out;
+  static constexpr bool kLoss = false;
+  __device__ __forceinline__ void operator()(long long i, float2 v, double&) const { out[i] = v; }
+};
+struct StoreLoss {  // u kept for the backward; (|u|^2 - d)^2 summed in FP64
+  float2* __restrict__ out;
+  const float* __restrict__ d;
+  static constexpr bool kLoss = true;
+  __device__ __forceinline__ void operator()(long long i, float2 v, double& acc) const {
+    out[i] = v;
+    const double r = (double)(v.x * v.x + v.y * v.y) - (double)d[i];
+    acc += r * r;
+  }
+};
+struct StoreChain {  // dL/dp = -k Im(conj(G) psi), dL/db = -k Re(conj(G) psi), times gscale, over p and b
+  float* __restrict__ p;
+  float* __restrict__ b;
+  float k, gscale;
+  static constexpr bool kLoss = false;
+  __device__ __forceinline__ void operator()(long long i, float2 g, double&) const {
+    float sn, cs;
+    sincosf(k * p[i], &sn, &cs);
+    const float at = expf(-k * b[i]);
+    const float2 psi = make_float2(at * cs, at * sn);
+    const float re = g.x * psi.x + g.y * psi.y;  // Re(conj(g) psi)
+    const float im = g.x * psi.y - g.y * psi.x;  // Im(conj(g) psi)
+    p[i] = -k * im * gscale;
+    b[i] = -k * re * gscale;
+  }
+};
+
+// Sum of one double per thread into partial[blockIdx.x] (x: the workgroup's LDS, free after the transforms).
+template <int NT>
+__device__ __forceinline__ void block_sum(double acc, float2* __restrict__ x, double* __restrict__ partial,
+                                          unsigned slot) {
+  double* red = reinterpret_cast<double*>(x);
+  __syncthreads();
+  red[threadIdx.x] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[slot] = red[0];
+}
+
 // Column pass on the DCT form: frames (b, nz, n), transforms of length L = nz down the columns.
-template <int L, bool CONJ_K>
-__global__ MM_TWO_FFT_BUDGET __launch_bounds__(ColShape<L>::NT) void dct_col_pass(
-    const float2* __restrict__ in, float2* __restrict__ out, int n, const float2* __restrict__ K,
-    const float2* __restrict__ tw, const float2* __restrict__ tw2, float2 hw) {
-  using Sh = ColShape<L>;
+template <int L, bool CONJ_K, class Load, class Store>
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctColShape<L>::NT) void dct_col_pass(
+    Load ld, Store st, int n, const float2* __restrict__ K, const float2* __restrict__ tw,
+    const float2* __restrict__ tw2, float2 hw, double* __restrict__ partial) {
+  using Sh = DctColShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
@@ -238,43 +328,46 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(ColShape<L>::NT) void dct_col_pas
   const int col = bx * Sh::C + c;
   const bool live = col < n;
   const long long fo = (long long)frame * L * n + col;
-  const float2* src = in + fo;
-  float2* dst = out + fo;
   float2* x = lds + c * Sh::S;
   float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) v[m] = live ? src[(long long)dct_src<Sh>(m, t) * n] : make_float2(0.f, 0.f);
+  for (int m = 0; m < Sh::PT; ++m) v[m] = live ? ld(fo + (long long)dct_src<Sh>(m, t) * n) : make_float2(0.f, 0.f);
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
   dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
-  if (!live) return;
+  double acc = 0.0;
+  if (live) {
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) dst[(long long)dct_src<Sh>(m, t) * n] = v[m];
+    for (int m = 0; m < Sh::PT; ++m) st(fo + (long long)dct_src<Sh>(m, t) * n, v[m], acc);
+  }
+  if constexpr (Store::kLoss) block_sum<Sh::NT>(acc, lds, partial, blockIdx.x + gridDim.x * blockIdx.y);
 }
 
 // Row pass on the DCT form: rows of length L = n, C rows per block, nrows = b * nz.
-template <int L, bool CONJ_K>
-__global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void dct_row_pass(
-    const float2* __restrict__ in, float2* __restrict__ out, long long nrows, const float2* __restrict__ K,
-    const float2* __restrict__ tw, const float2* __restrict__ tw2, float2 hw) {
-  using Sh = RowReg<L>;
+template <int L, bool CONJ_K, class Load, class Store>
+__global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctRowShape<L>::NT) void dct_row_pass(
+    Load ld, Store st, long long nrows, const float2* __restrict__ K, const float2* __restrict__ tw,
+    const float2* __restrict__ tw2, float2 hw, double* __restrict__ partial) {
+  using Sh = DctRowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
   Sh::map(threadIdx.x, c, t);
   const long long row = (long long)blockIdx.x * Sh::C + c;
   const bool valid = row < nrows;
-  const float2* src = in + (valid ? row : 0) * L;
-  float2* dst = out + (valid ? row : 0) * L;
+  const long long ro = (valid ? row : 0) * L;
   float2* x = lds + c * Sh::S;
   float2 v[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? src[dct_src<Sh>(m, t)] : make_float2(0.f, 0.f);
+  for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? ld(ro + dct_src<Sh>(m, t)) : make_float2(0.f, 0.f);
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
   dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
-  if (!valid) return;
+  double acc = 0.0;
+  if (valid) {
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) dst[dct_src<Sh>(m, t)] = v[m];
+    for (int m = 0; m < Sh::PT; ++m) st(ro + dct_src<Sh>(m, t), v[m], acc);
+  }
+  if constexpr (Store::kLoss) block_sum<Sh::NT>(acc, lds, partial, blockIdx.x);
 }
 
 }  // namespace fresnel
@@ -325,21 +418,28 @@ static float2 half_step(int N) {
 // The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48) and the length-2N twiddles.
 static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && 2 * N <= kMaxFFT; }
 
-template <int L, bool CONJ_K>
-static void launch_dct_col(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
-  using Sh = ColShape<L>;
-  hipLaunchKernelGGL((dct_col_pass<L, CONJ_K>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
-                     dim3(Sh::NT), 0, st, in, out, p->n, p->Ky + (size_t)j * 2 * L, p->tw + tw_offset(L),
-                     p->tw + tw_offset(2 * L), half_step(L));
+template <int L, bool CONJ_K, class Load, class Store>
+static void launch_dct_col(const mm_fresnel* p, Load ld, Store sto, int nframes, int j, double* partial,
+                           hipStream_t st) {
+  using Sh = DctColShape<L>;
+  hipLaunchKernelGGL((dct_col_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
+                     dim3(Sh::NT), 0, st, ld, sto, p->n, p->Ky + (size_t)j * 2 * L, p->tw + tw_offset(L),
+                     p->tw + tw_offset(2 * L), half_step(L), partial);
 }
 
-template <int L, bool CONJ_K>
-static void launch_dct_row(const mm_fresnel* p, const float2* in, float2* out, int nframes, int j, hipStream_t st) {
-  using Sh = RowReg<L>;
+template <int L, bool CONJ_K, class Load, class Store>
+static void launch_dct_row(const mm_fresnel* p, Load ld, Store sto, int nframes, int j, double* partial,
+                           hipStream_t st) {
+  using Sh = DctRowShape<L>;
   const long long rows = (long long)nframes * p->nz;
-  hipLaunchKernelGGL((dct_row_pass<L, CONJ_K>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT), 0, st, in,
-                     out, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), p->tw + tw_offset(2 * L),
-                     half_step(L));
+  hipLaunchKernelGGL((dct_row_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT),
+                     0, st, ld, sto, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), p->tw + tw_offset(2 * L),
+                     half_step(L), partial);
+}
+
+template <int L>
+static long long dct_row_parts(const mm_fresnel* p, int nframes) {
+  return ((long long)nframes * p->nz + DctRowShape<L>::C - 1) / DctRowShape<L>::C;
 }
 
 static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, int j, void* stream, bool adj) {
@@ -353,17 +453,19 @@ static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, 
   const float2* src = (const float2*)in;
   float2* dst = (float2*)out;
   if (p->pad_mode == 0 && p->dct) {  // mirror padding: DCT form, DT = the same with conj(K)
+    const LoadC64 ld_in{src}, ld_out{dst};
+    const StoreC64 sto{dst};
     if (adj) {
-      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, true>(p, src, dst, nframes, j, st)));
+      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, true>(p, ld_in, sto, nframes, j, nullptr, st)));
     } else {
-      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, false>(p, src, dst, nframes, j, st)));
+      MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, false>(p, ld_in, sto, nframes, j, nullptr, st)));
     }
     int rc = check_launch("fresnel dct_col_pass");
     if (rc) return rc;
     if (adj) {
-      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, true>(p, dst, dst, nframes, j, st)));
+      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, true>(p, ld_out, sto, nframes, j, nullptr, st)));
     } else {
-      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, false>(p, dst, dst, nframes, j, st)));
+      MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, false>(p, ld_out, sto, nframes, j, nullptr, st)));
     }
     return check_launch("fresnel dct_row_pass");
   }
@@ -384,6 +486,48 @@ static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, 
 }
 
 namespace mm {
+// Forward of the model on the DCT form: psi from (P, B) -> u = D_j psi (kept), loss partials of
+// (|u|^2 - d)^2 into partial[0 .. *nparts).  Column pass first (psi built while loading), row pass
+// second (coalesced loads of d next to the stores of u).
+int fresnel_model_fwd(const mm_fresnel* p, const float* P, const float* B, float k, float2* u, const float* d,
+                      double* partial, int* nparts, int nframes, int j, hipStream_t st) {
+  if (!(p->pad_mode == 0 && p->dct)) return fail(MM_ERR_UNSUPPORTED, "fused model passes need the DCT form");
+  const LoadPsi lpsi{P, B, k};
+  const LoadC64 lu{u};
+  const StoreC64 su{u};
+  const StoreLoss sl{u, d};
+  MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, false>(p, lpsi, su, nframes, j, nullptr, st)));
+  int rc = check_launch("model psi + D columns");
+  if (rc) return rc;
+  MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, false>(p, lu, sl, nframes, j, partial, st)));
+  MM_DISPATCH_FFT(p->n, (*nparts = (int)dct_row_parts<LL>(p, nframes)));
+  return check_launch("model D rows + loss");
+}
+
+// Backward of the model on the DCT form: G = DT_j(4 (|u|^2 - d) u) over u, then P, B <- gscale times
+// dL/dp, dL/db (psi recomputed from P, B while storing).
+int fresnel_model_bwd(const mm_fresnel* p, float2* u, const float* d, float* P, float* B, float k, float gscale,
+                      int nframes, int j, hipStream_t st) {
+  if (!(p->pad_mode == 0 && p->dct)) return fail(MM_ERR_UNSUPPORTED, "fused model passes need the DCT form");
+  const LoadResidual lr{u, d};
+  const LoadC64 lu{u};
+  const StoreC64 su{u};
+  const StoreChain sc{P, B, k, gscale};
+  MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, true>(p, lr, su, nframes, j, nullptr, st)));
+  int rc = check_launch("model residual + DT columns");
+  if (rc) return rc;
+  MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, true>(p, lu, sc, nframes, j, nullptr, st)));
+  return check_launch("model DT rows + chain");
+}
+
+bool fresnel_fused_ok(const mm_fresnel* p) { return p->pad_mode == 0 && p->dct; }
+
+// Loss partials the fused forward writes for nframes frames (one per row-pass workgroup).
+int fresnel_model_parts(const mm_fresnel* p, int nframes, long long* parts) {
+  MM_DISPATCH_FFT(p->n, (*parts = dct_row_parts<LL>(p, nframes)));
+  return MM_OK;
+}
+
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist) {
   *n = p->n;
   *nz = p->nz;

@@ -71,6 +71,14 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
 size_t radon_fde_elems(int n, int nz_max);
 size_t radon_gh_elems(int n, int nz_max, int ntheta);
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);
+// Fused model passes on the DCT form of a mirror-padded Fresnel plan (fresnel.hip); MM_ERR_UNSUPPORTED
+// when the plan has no DCT form (fresnel_fused_ok false).
+bool fresnel_fused_ok(const mm_fresnel* p);
+int fresnel_model_parts(const mm_fresnel* p, int nframes, long long* parts);
+int fresnel_model_fwd(const mm_fresnel* p, const float* P, const float* B, float k, float2* u, const float* d,
+                      double* partial, int* nparts, int nframes, int j, hipStream_t st);
+int fresnel_model_bwd(const mm_fresnel* p, float2* u, const float* d, float* P, float* B, float k, float gscale,
+                      int nframes, int j, hipStream_t st);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 // FFT lengths the kernels are built for: 2^k and 3 * 2^k.

@@ -5,11 +5,18 @@
 //
 // Forward:  p = R delta, b = R beta, psi = exp(i k p - k b), u = D psi,
 //           L = sum (|u|^2 - d)^2   (FP64 block partials + one-block reduce).
-// Backward: psi and u are recomputed (only the two float32 sinograms p, b and
-//           one complex64 sinogram-sized buffer are held),
-//           r = 4 (|u|^2 - d) u,  G = DT r,
+// Backward: r = 4 (|u|^2 - d) u,  G = DT r,
 //           dL/dp = -k Im(conj(G) psi),  dL/db = -k Re(conj(G) psi),
 //           dL/ddelta = RT dL/dp,  dL/dbeta = RT dL/db.
+// The forward leaves p, b and u in the model's workspace under a token; a
+// backward given that token (no other forward since) starts from them and
+// recomputes only psi, otherwise it reruns the forward first.  psi itself is
+// never stored.  With a mirror-padded Fresnel plan on the DCT form (the
+// default sizes) psi is built inside D's column pass, the loss partials come
+// from its row pass, 4 (|u|^2 - d) u is formed inside DT's column pass and the
+// chain rule inside its row pass (fresnel_model_fwd / fresnel_model_bwd);
+// other plans run the element-wise kernels below around D and DT.
+#include <algorithm>
 #include <vector>
 
 #include "mm_common.h"
@@ -99,13 +106,16 @@ using namespace mm::model;
 struct mm_model {
   mm_radon* radon = nullptr;
   mm_fresnel* fresnel = nullptr;
-  int n = 0, nz = 0, ntheta = 0;
+  int n = 0, nz = 0, ntheta = 0, ndist = 0;
   float k = 0.f;
+  bool fused = false;        // Fresnel plan on the DCT form: element-wise steps inside its passes
   float* p = nullptr;        // ntheta x nz x n
   float* b = nullptr;        // ntheta x nz x n
   float2* u = nullptr;       // ntheta x nz x n
-  double* partial = nullptr; // kMaxBlocks
+  double* partial = nullptr; // loss partials
   size_t bytes = 0;
+  unsigned long long issued = 0;  // tokens handed out so far
+  unsigned long long state = 0;   // token of the forward whose p, b, u the workspace holds (0: none)
 };
 
 static void model_free(mm_model* m) {
@@ -135,57 +145,89 @@ int mm_model_create(mm_model** model, mm_radon* radon, mm_fresnel* fresnel, int 
   m->n = rn;
   m->nz = nz;
   m->ntheta = ntheta;
+  m->ndist = ndist;
   m->k = k;
   const size_t cnt = (size_t)ntheta * nz * rn;
+  m->fused = fresnel_fused_ok(fresnel);
+  long long nparts = kMaxBlocks;
+  if (m->fused) {
+    long long fp = 0;
+    int rc = fresnel_model_parts(fresnel, ntheta, &fp);
+    if (rc) {
+      model_free(m);
+      return rc;
+    }
+    nparts = std::max<long long>(nparts, fp);
+  }
   if (hipMalloc(&m->p, cnt * sizeof(float)) != hipSuccess || hipMalloc(&m->b, cnt * sizeof(float)) != hipSuccess ||
       hipMalloc(&m->u, cnt * sizeof(float2)) != hipSuccess ||
-      hipMalloc(&m->partial, kMaxBlocks * sizeof(double)) != hipSuccess) {
+      hipMalloc(&m->partial, nparts * sizeof(double)) != hipSuccess) {
     model_free(m);
     return fail(MM_ERR_OOM, "mm_model_create: workspace allocation failed");
   }
-  m->bytes = cnt * (2 * sizeof(float) + sizeof(float2)) + kMaxBlocks * sizeof(double);
+  m->bytes = cnt * (2 * sizeof(float) + sizeof(float2)) + nparts * sizeof(double);
   *model = m;
   return MM_OK;
 }
 
-static int model_forward_common(mm_model* m, const float* delta, const float* beta, int j, hipStream_t st) {
+// p = R delta, b = R beta, u = D psi and the loss partials (*nparts of them in m->partial).
+static int model_forward_state(mm_model* m, const float* delta, const float* beta, const float* data, int j,
+                               int* nparts, hipStream_t st) {
   int rc;
+  m->state = 0;
   if ((rc = mm_radon_fwd(m->radon, delta, m->p, m->nz, st))) return rc;
   if ((rc = mm_radon_fwd(m->radon, beta, m->b, m->nz, st))) return rc;
+  if (m->fused) return fresnel_model_fwd(m->fresnel, m->p, m->b, m->k, m->u, data, m->partial, nparts, m->ntheta, j, st);
   const long long cnt = (long long)m->ntheta * m->nz * m->n;
   hipLaunchKernelGGL(psi_kernel, dim3(blocks_for(cnt)), dim3(kBlock), 0, st, m->p, m->b, m->u, cnt, m->k);
   if ((rc = check_launch("model psi"))) return rc;
-  return mm_fresnel_fwd(m->fresnel, m->u, m->u, m->ntheta, j, st);
+  if ((rc = mm_fresnel_fwd(m->fresnel, m->u, m->u, m->ntheta, j, st))) return rc;
+  const unsigned nb = blocks_for(cnt);
+  hipLaunchKernelGGL(loss_partial_kernel, dim3(nb), dim3(kBlock), 0, st, m->u, data, cnt, m->partial);
+  *nparts = (int)nb;
+  return check_launch("model loss");
 }
 
 int mm_model_fwd(mm_model* m, const float* delta, const float* beta, const float* data, int j, double* loss_dev,
-                 void* stream) {
+                 unsigned long long* token, void* stream) {
   MM_ARG_CHECK(m != nullptr, "mm_model_fwd: model is NULL");
   MM_ARG_CHECK(delta && beta && data && loss_dev, "mm_model_fwd: null pointer");
+  MM_ARG_CHECK(j >= 0 && j < m->ndist, "mm_model_fwd: distance index " + std::to_string(j) + " outside [0, " +
+                                           std::to_string(m->ndist) + ")");
   hipStream_t st = (hipStream_t)stream;
-  int rc = model_forward_common(m, delta, beta, j, st);
+  int nparts = 0;
+  int rc = model_forward_state(m, delta, beta, data, j, &nparts, st);
   if (rc) return rc;
-  const long long cnt = (long long)m->ntheta * m->nz * m->n;
-  const unsigned nb = blocks_for(cnt);
-  hipLaunchKernelGGL(loss_partial_kernel, dim3(nb), dim3(kBlock), 0, st, m->u, data, cnt, m->partial);
-  if ((rc = check_launch("model loss"))) return rc;
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kBlock), 0, st, m->partial, (int)nb, loss_dev);
-  return check_launch("model reduce");
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kBlock), 0, st, m->partial, nparts, loss_dev);
+  if ((rc = check_launch("model reduce"))) return rc;
+  m->state = ++m->issued;
+  if (token) *token = m->state;
+  return MM_OK;
 }
 
 int mm_model_bwd(mm_model* m, const float* delta, const float* beta, const float* data, int j, float gscale,
-                 float* gdelta, float* gbeta, void* stream) {
+                 unsigned long long token, float* gdelta, float* gbeta, void* stream) {
   MM_ARG_CHECK(m != nullptr, "mm_model_bwd: model is NULL");
   MM_ARG_CHECK(delta && beta && data && gdelta && gbeta, "mm_model_bwd: null pointer");
+  MM_ARG_CHECK(j >= 0 && j < m->ndist, "mm_model_bwd: distance index " + std::to_string(j) + " outside [0, " +
+                                           std::to_string(m->ndist) + ")");
   hipStream_t st = (hipStream_t)stream;
-  int rc = model_forward_common(m, delta, beta, j, st);
-  if (rc) return rc;
-  const long long cnt = (long long)m->ntheta * m->nz * m->n;
-  hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(cnt)), dim3(kBlock), 0, st, m->u, data, cnt);
-  if ((rc = check_launch("model residual"))) return rc;
-  if ((rc = mm_fresnel_adj(m->fresnel, m->u, m->u, m->ntheta, j, st))) return rc;
-  hipLaunchKernelGGL(chain_kernel, dim3(blocks_for(cnt)), dim3(kBlock), 0, st, m->u, m->p, m->b, cnt, m->k, gscale);
-  if ((rc = check_launch("model chain"))) return rc;
+  int rc;
+  if (token == 0 || token != m->state) {  // the workspace does not hold this forward's p, b, u
+    int nparts = 0;
+    if ((rc = model_forward_state(m, delta, beta, data, j, &nparts, st))) return rc;
+  }
+  m->state = 0;  // u, p, b are overwritten below
+  if (m->fused) {
+    if ((rc = fresnel_model_bwd(m->fresnel, m->u, data, m->p, m->b, m->k, gscale, m->ntheta, j, st))) return rc;
+  } else {
+    const long long cnt = (long long)m->ntheta * m->nz * m->n;
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(cnt)), dim3(kBlock), 0, st, m->u, data, cnt);
+    if ((rc = check_launch("model residual"))) return rc;
+    if ((rc = mm_fresnel_adj(m->fresnel, m->u, m->u, m->ntheta, j, st))) return rc;
+    hipLaunchKernelGGL(chain_kernel, dim3(blocks_for(cnt)), dim3(kBlock), 0, st, m->u, m->p, m->b, cnt, m->k, gscale);
+    if ((rc = check_launch("model chain"))) return rc;
+  }
   if ((rc = mm_radon_adj(m->radon, m->p, 0, gdelta, m->nz, st))) return rc;
   return mm_radon_adj(m->radon, m->b, 0, gbeta, m->nz, st);
 }

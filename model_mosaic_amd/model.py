@@ -49,6 +49,7 @@ class ForwardModel:
         self.n, self.nz = int(n), int(nz)
         dist = np.atleast_1d(np.asarray(distance, dtype=np.float64))
         self.prop = Propagation(n, nz, self.ntheta, len(dist), wavelength, voxelsize, dist, pad=pad)
+        self.prop_pad = pad
         self.k = float(k) if k is not None else physical_k(wavelength, voxelsize, n, self.ntheta)
         self.device = self.tomo.device
         h = C.c_void_p()
@@ -84,20 +85,26 @@ class ForwardModel:
 
     def loss(self, delta, beta, data, j: int = 0) -> torch.Tensor:
         """FP64 scalar loss on the device."""
+        return self._forward(delta, beta, data, j)[0]
+
+    def _forward(self, delta, beta, data, j: int = 0):
+        """Loss and the token of the p, b, u it leaves in the workspace (see grad)."""
         d, b, y = self._inputs(delta, beta, data)
         out = torch.empty((), dtype=torch.float64, device=self.device)
+        token = C.c_ulonglong(0)
         _lib.check(self._lib.mm_model_fwd(self._h, d.data_ptr(), b.data_ptr(), y.data_ptr(), int(j), out.data_ptr(),
-                                          _lib.stream_handle()), "ForwardModel.loss")
-        return out
+                                          C.byref(token), _lib.stream_handle()), "ForwardModel.loss")
+        return out, int(token.value)
 
-    def grad(self, delta, beta, data, j: int = 0, gscale: float = 1.0):
-        """(dL/ddelta, dL/dbeta) times gscale, float32 (nz, n, n)."""
+    def grad(self, delta, beta, data, j: int = 0, gscale: float = 1.0, token: int = 0):
+        """(dL/ddelta, dL/dbeta) times gscale, float32 (nz, n, n).  ``token`` from the forward of the same
+        inputs lets the backward start from that forward's sinograms and wavefield (0: run the forward)."""
         d, b, y = self._inputs(delta, beta, data)
         gd = torch.empty_like(d)
         gb = torch.empty_like(b)
         _lib.check(self._lib.mm_model_bwd(self._h, d.data_ptr(), b.data_ptr(), y.data_ptr(), int(j),
-                                          C.c_float(gscale), gd.data_ptr(), gb.data_ptr(), _lib.stream_handle()),
-                   "ForwardModel.grad")
+                                          C.c_float(gscale), C.c_ulonglong(int(token)), gd.data_ptr(), gb.data_ptr(),
+                                          _lib.stream_handle()), "ForwardModel.grad")
         return gd, gb
 
 
@@ -106,13 +113,15 @@ class _Misfit(torch.autograd.Function):
     def forward(ctx, delta, beta, model: ForwardModel, data, j: int):
         ctx.model = model
         ctx.j = j
+        loss, ctx.token = model._forward(delta, beta, data, j)
+        # saved tensors carry their version counters: autograd refuses a backward after an in-place change
         ctx.save_for_backward(delta, beta, data)
-        return model.loss(delta, beta, data, j)
+        return loss
 
     @staticmethod
     def backward(ctx, g):
         delta, beta, data = ctx.saved_tensors
-        gd, gb = ctx.model.grad(delta, beta, data, ctx.j, gscale=float(g))
+        gd, gb = ctx.model.grad(delta, beta, data, ctx.j, gscale=float(g), token=ctx.token)
         return gd, gb, None, None, None
 
 

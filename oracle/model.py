@@ -18,16 +18,17 @@ from oracle import fresnel, usfft
 
 
 class Model:
-    def __init__(self, theta, n, nz, wavelength, voxelsize, z, k):
+    def __init__(self, theta, n, nz, wavelength, voxelsize, z, k, pad="symmetric"):
         self.theta = np.asarray(theta, dtype=np.float32)
         self.k = float(k)
+        self.pad = pad
         self.fk = fresnel.fresnel_kernel_2d(n, nz, wavelength, voxelsize, [z], np.float64)[0]
 
     def psi_u(self, delta, beta):
         p = usfft.tomo_R(delta, self.theta, np.float64)
         b = usfft.tomo_R(beta, self.theta, np.float64)
         psi = np.exp(1j * self.k * p - self.k * b)
-        return p, b, psi, fresnel.D(psi, self.fk, np.float64)
+        return p, b, psi, fresnel.D(psi, self.fk, np.float64, pad=self.pad)
 
     def intensity(self, delta, beta):
         return np.abs(self.psi_u(delta, beta)[3]) ** 2
@@ -39,7 +40,7 @@ class Model:
     def grad(self, delta, beta, data):
         _, _, psi, u = self.psi_u(delta, beta)
         r = 4.0 * (np.abs(u) ** 2 - data) * u
-        G = fresnel.DT(r, self.fk, np.float64)
+        G = fresnel.DT(r, self.fk, np.float64, pad=self.pad)
         q = np.conj(G) * psi
         gp = -self.k * q.imag
         gb = -self.k * q.real

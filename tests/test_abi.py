@@ -50,7 +50,8 @@ def test_null_plan_is_an_argument_error(lib):
     with pytest.raises(ValueError):
         _lib.check(rc, "mm_radon_fwd")
     assert lib.mm_fresnel_fwd(None, None, None, 1, 0, None) == _lib.MM_ERR_ARG
-    assert lib.mm_model_fwd(None, None, None, None, 0, None, None) == _lib.MM_ERR_ARG
+    assert lib.mm_model_fwd(None, None, None, None, 0, None, None, None) == _lib.MM_ERR_ARG
+    assert lib.mm_model_bwd(None, None, None, None, 0, 1.0, 0, None, None, None) == _lib.MM_ERR_ARG
 
 
 def test_bad_sizes_rejected_before_device_work(lib):

@@ -15,17 +15,20 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
-@pytest.fixture(scope="module")
-def setup():
+@pytest.fixture(scope="module", params=["symmetric", "zero"])
+def setup(request):
+    """Mirror padding runs the element-wise steps inside the Fresnel passes (DCT form); zero padding runs
+    them as kernels of their own around D and DT."""
     from model_mosaic_amd.model import ForwardModel
+    pad = request.param
     theta = np.linspace(0, np.pi, NT, endpoint=False).astype(np.float32)
     rng = np.random.default_rng(0)
     d_true = np.abs(rng.standard_normal((NZ, N, N))).astype(np.float32)
-    orc = Model(theta, N, NZ, WL, VOX, Z, K)
+    orc = Model(theta, N, NZ, WL, VOX, Z, K, pad=pad)
     data = orc.intensity(d_true, 0.02 * d_true).astype(np.float32)
     delta = (d_true + 0.1 * rng.standard_normal(d_true.shape)).astype(np.float32)
     beta = (0.02 * d_true + 0.01 * rng.standard_normal(d_true.shape)).astype(np.float32)
-    fm = ForwardModel(N, NZ, theta, WL, VOX, [Z], k=K)
+    fm = ForwardModel(N, NZ, theta, WL, VOX, [Z], k=K, pad=pad)
     return fm, orc, data, delta, beta
 
 
@@ -39,7 +42,10 @@ def test_loss_and_grad_vs_oracle(setup):
     e_l, e_d, e_b = abs(loss - ref) / ref, rel(gd.cpu().numpy(), rgd), rel(gb.cpu().numpy(), rgb)
     print(f"model: loss {e_l:.2e}  grad delta {e_d:.2e}  grad beta {e_b:.2e}")
     assert e_l < 1e-5
-    assert e_d < 1e-4 and e_b < 1e-4
+    # the gradients are sums with heavy cancellation (random residuals); FP32 rounding against the FP64 oracle
+    # measures 2e-5 .. 4e-5 with mirror padding and 2e-5 .. 1.1e-4 with zero padding at these sizes
+    bar = 1e-4 if fm.prop_pad == "symmetric" else 2e-4
+    assert e_d < bar and e_b < bar
 
 
 def test_autograd_function(setup):
@@ -83,3 +89,22 @@ def test_central_finite_differences_64_voxels(setup):
         err = abs(fd - dirderiv) / abs(dirderiv)
         print(f"FD check ({'delta' if which == 0 else 'beta'}): 64-voxel directional rel err {err:.2e}")
         assert err < 1e-3
+
+
+def test_forward_token_reuse_and_staleness(setup):
+    """A backward given its forward's token starts from the sinograms and wavefield that forward left and equals
+    the backward that reruns the forward (token 0) bit for bit; a token made stale by a later forward, or
+    used up by a backward, falls back to rerunning the forward."""
+    fm, orc, data, delta, beta = setup
+    dd, bb, yy = (torch.from_numpy(a).cuda() for a in (delta, beta, data))
+    fresh = fm.grad(dd, bb, yy)
+    _, tok = fm._forward(dd, bb, yy)
+    assert tok != 0
+    reuse = fm.grad(dd, bb, yy, token=tok)
+    assert all(torch.equal(a, b) for a, b in zip(reuse, fresh))
+    again = fm.grad(dd, bb, yy, token=tok)          # used up by the backward above
+    assert all(torch.equal(a, b) for a, b in zip(again, fresh))
+    _, tok1 = fm._forward(dd, bb, yy)
+    fm._forward(2 * dd, bb, yy)                      # a later forward on other inputs
+    stale = fm.grad(dd, bb, yy, token=tok1)
+    assert all(torch.equal(a, b) for a, b in zip(stale, fresh))
