@@ -414,7 +414,8 @@ def run_mosaic(torch, reps=2):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     info = mb.info()
-    alg = sum(t["nz"] * b_dir(t["n"], t["ntheta"]) + 56 * t["ntheta"] * t["nz"] * t["n"] for t in spec)
+    # per tile: R (B_dir per slice) + Fresnel D of the ntheta frames at B_D = 4 passes of complex64 (32 B/px)
+    alg = sum(t["nz"] * b_dir(t["n"], t["ntheta"]) + 32 * t["ntheta"] * t["nz"] * t["n"] for t in spec)
     nproj = sum(t["ntheta"] for t in spec)
     res = {"config": "4 x-tiles x 9 z-stacks, 96 rows each (8-row overlaps), widths per x-tile from {1536, 2048, 3072, "
                      "4096}, ntheta per tile from {1029, 2058}, R then psi, Fresnel D (30 keV, 0.69 um, 0.2 m), |.|^2",
@@ -422,7 +423,9 @@ def run_mosaic(torch, reps=2):
            "s_per_batch": round(dt, 5), "projections_per_s": round(nproj / dt, 1),
            "algorithmic_GB": round(alg / 1e9, 3), "hbm_frac": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
            "plan_s": round(plan_s, 2), "radon_plans": info["radon_plans"], "fresnel_plans": info["fresnel_plans"],
-           "graph_captures": info["graph_captures"], "gather_xcd_group": sorted(set(info["gather_xcd_group"])),
+           "graph_captures": info["graph_captures"],
+           "tuned_per_plan": sorted({(t["n"], t["ntheta"], g, c) for t, g, c in
+                                     zip(spec, info["gather_xcd_group"], info["col_split"])}),
            "workspace_GB": round(info["workspace_bytes"] / 1e9, 2)}
     mb.free()
     del deltas, outs

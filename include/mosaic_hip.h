@@ -117,9 +117,11 @@ int mm_radon_tune(mm_radon* plan, int knob, int value);
 /* Current value of a knob (or MM_ERR_ARG). */
 int mm_radon_get_tune(const mm_radon* plan, int knob);
 /* Tile autotuner: times R (obj -> sino) for every gather XCD group size in
- * {1, 2, 4, 8, 16} and keeps the fastest; with back != NULL also RT
- * (sino -> back) for the scatter knob.  Synchronises the stream (not for use
- * under graph capture).  obj is only read; sino and back are overwritten. */
+ * {1, 2, 4, 8, 16} and keeps the fastest; for n = 2048 (the only width whose
+ * column transforms the split knob reshapes) then R (and RT) with the column
+ * split off and on; with back != NULL last RT (sino -> back) for the scatter
+ * knob.  Synchronises the stream (not for use under graph capture).  obj is
+ * only read; sino and back are overwritten. */
 int mm_radon_autotune(mm_radon* plan, const float* obj_dev, float* sino_dev, float* back_dev, int nz, void* stream);
 
 /* Host-only self-check (no counterpart in the reference; for tests on machines
@@ -233,8 +235,8 @@ int mm_mosaic_create(mm_mosaic** plan, int ntiles, const int* n, const int* nz, 
                      float beta_ratio, int pad_mode, unsigned flags);
 int mm_mosaic_run(mm_mosaic* plan, const float* const* delta_dev, float* const* out_dev, void* stream);
 int mm_mosaic_info(const mm_mosaic* plan, int what);
-/* Gather XCD group knob of tile t's Radon plan (after autotuning: the chosen one). */
-int mm_mosaic_tile_knob(const mm_mosaic* plan, int tile);
+/* Launch knob (MM_RADON_TUNE_*) of tile t's Radon plan (after autotuning: the chosen value). */
+int mm_mosaic_tile_knob(const mm_mosaic* plan, int tile, int knob);
 size_t mm_mosaic_workspace_bytes(const mm_mosaic* plan);
 void mm_mosaic_destroy(mm_mosaic* plan);
 

@@ -69,7 +69,7 @@ _SIGS = {
                                    C.c_int, C.c_uint]),
     "mm_mosaic_run": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp]),
     "mm_mosaic_info": (C.c_int, [_vp, C.c_int]),
-    "mm_mosaic_tile_knob": (C.c_int, [_vp, C.c_int]),
+    "mm_mosaic_tile_knob": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mm_mosaic_workspace_bytes": (C.c_size_t, [_vp]),
     "mm_mosaic_destroy": (None, [_vp]),
     "mm_plain_gather": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),

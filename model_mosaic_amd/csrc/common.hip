@@ -67,11 +67,7 @@ __global__ void psi_from_proj_kernel(const float* __restrict__ p, float2* __rest
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < count; i += stride) {
-    float phase = p[i] * inv_norm;
-    float atten = expf(-phase * inv_beta_ratio);
-    float s, c;
-    sincosf(phase, &s, &c);
-    psi[i] = make_float2(atten * c, atten * s);
+    psi[i] = psi_of(p[i], inv_norm, inv_beta_ratio);
   }
 }
 
@@ -80,8 +76,7 @@ __global__ void abs2_kernel(const float2* __restrict__ z, float* __restrict__ ou
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < count; i += stride) {
-    float2 v = z[i];
-    out[i] = v.x * v.x + v.y * v.y;
+    out[i] = abs2_of(z[i]);
   }
 }
 

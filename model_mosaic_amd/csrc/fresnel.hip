@@ -257,6 +257,11 @@ struct LoadPsi {  // psi of step3_propagation.py:94-103 with separate phase / at
     return make_float2(at * cs, at * sn);
   }
 };
+struct LoadPsiRatio {  // psi of one projection sinogram (step3_propagation.py:94-103)
+  const float* __restrict__ p;
+  float inv_norm, inv_beta_ratio;
+  __device__ __forceinline__ float2 operator()(long long i) const { return psi_of(p[i], inv_norm, inv_beta_ratio); }
+};
 struct LoadResidual {  // 4 (|u|^2 - d) u
   const float2* __restrict__ u;
   const float* __restrict__ d;
@@ -280,6 +285,11 @@ struct StoreLoss {  // u kept for the backward; (|u|^2 - d)^2 summed in FP64
     const double r = (double)(v.x * v.x + v.y * v.y) - (double)d[i];
     acc += r * r;
   }
+};
+struct StoreAbs2 {  // intensity |u|^2 (step3_propagation.py:105-110)
+  float* __restrict__ out;
+  static constexpr bool kLoss = false;
+  __device__ __forceinline__ void operator()(long long i, float2 v, double&) const { out[i] = abs2_of(v); }
 };
 struct StoreChain {  // dL/dp = -k Im(conj(G) psi), dL/db = -k Re(conj(G) psi), times gscale, over p and b
   float* __restrict__ p;
@@ -521,6 +531,22 @@ int fresnel_model_bwd(const mm_fresnel* p, float2* u, const float* d, float* P, 
 }
 
 bool fresnel_fused_ok(const mm_fresnel* p) { return p->pad_mode == 0 && p->dct; }
+
+// out = |D_j psi(p)|^2 with psi built while the column pass loads p and |.|^2 taken while the row pass stores;
+// work (nframes, nz, n) complex64 holds the wavefield between the passes.
+int fresnel_intensity(const mm_fresnel* p, const float* P, float inv_norm, float inv_beta_ratio, float2* work,
+                      float* out, int nframes, int j, hipStream_t st) {
+  if (!(p->pad_mode == 0 && p->dct)) return fail(MM_ERR_UNSUPPORTED, "fused intensity passes need the DCT form");
+  const LoadPsiRatio lpsi{P, inv_norm, inv_beta_ratio};
+  const LoadC64 lw{work};
+  const StoreC64 sw{work};
+  const StoreAbs2 sa{out};
+  MM_DISPATCH_FFT(p->nz, (launch_dct_col<LL, false>(p, lpsi, sw, nframes, j, nullptr, st)));
+  int rc = check_launch("psi + D columns");
+  if (rc) return rc;
+  MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, false>(p, lw, sa, nframes, j, nullptr, st)));
+  return check_launch("D rows + |.|^2");
+}
 
 // Loss partials the fused forward writes for nframes frames (one per row-pass workgroup).
 int fresnel_model_parts(const mm_fresnel* p, int nframes, long long* parts) {

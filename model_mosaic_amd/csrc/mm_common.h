@@ -45,6 +45,19 @@ __host__ __device__ __forceinline__ float2 mul_mi(float2 a) {
   return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// ---- element-wise steps of step3_propagation.py:94-110 ----------------------
+// One definition for the stand-alone kernels and for the Fresnel passes that
+// fuse them, so both give the same bits.
+// psi = exp(-phase * inv_beta_ratio) * (cos phase + i sin phase), phase = p * inv_norm (HIP's precise expf / sincosf)
+__device__ __forceinline__ float2 psi_of(float p, float inv_norm, float inv_beta_ratio) {
+  const float phase = p * inv_norm;
+  const float atten = expf(-phase * inv_beta_ratio);
+  float s, c;
+  sincosf(phase, &s, &c);
+  return make_float2(atten * c, atten * s);
+}
+__device__ __forceinline__ float abs2_of(float2 v) { return v.x * v.x + v.y * v.y; }
+
 // ---- twiddle bank ----------------------------------------------------------
 // For every FFT length L = 2^k in [2, kMaxFFT] and L = 3 * 2^k in [12, kMaxFFT]
 // the bank holds exp(-2 pi i t / L), t in [0, L), at element offset
@@ -79,6 +92,8 @@ int fresnel_model_fwd(const mm_fresnel* p, const float* P, const float* B, float
                       double* partial, int* nparts, int nframes, int j, hipStream_t st);
 int fresnel_model_bwd(const mm_fresnel* p, float2* u, const float* d, float* P, float* B, float k, float gscale,
                       int nframes, int j, hipStream_t st);
+int fresnel_intensity(const mm_fresnel* p, const float* P, float inv_norm, float inv_beta_ratio, float2* work,
+                      float* out, int nframes, int j, hipStream_t st);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 // FFT lengths the kernels are built for: 2^k and 3 * 2^k.

@@ -10,7 +10,9 @@
 // buffers are shared by all tiles (sized for the largest; the tiles run one
 // after another on one stream), Radon plans are forward-only (no scatter
 // tables), and the graph is captured once per set of input / output pointers
-// and replayed as a single hipGraphLaunch.
+// and replayed as a single hipGraphLaunch.  With mirror padding psi is built
+// inside the Fresnel column pass and |.|^2 taken inside its row pass
+// (fresnel_intensity), so a tile is R plus two Fresnel passes.
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -66,11 +68,15 @@ static int mosaic_enqueue(mm_mosaic* m, const float* const* delta, float* const*
   int rc;
   for (size_t t = 0; t < m->tiles.size(); ++t) {
     const mm_mosaic::Tile& T = m->tiles[t];
-    if ((rc = mm_radon_fwd(m->rplans[T.rplan], delta[t], m->p, T.nz, st)) ||
-        (rc = mm_psi_from_proj(m->p, m->psi, T.count, m->k, m->beta_ratio, st)) ||
-        (rc = mm_fresnel_fwd(m->fplans[T.fplan], m->psi, m->psi, T.ntheta, 0, st)) ||
-        (rc = mm_abs2(m->psi, out[t], T.count, st)))
+    if ((rc = mm_radon_fwd(m->rplans[T.rplan], delta[t], m->p, T.nz, st))) return rc;
+    mm_fresnel* fp = m->fplans[T.fplan];
+    if (fresnel_fused_ok(fp)) {  // psi built in D's column pass, |.|^2 taken in its row pass
+      if ((rc = fresnel_intensity(fp, m->p, m->k, m->beta_ratio, m->psi, out[t], T.ntheta, 0, st))) return rc;
+    } else if ((rc = mm_psi_from_proj(m->p, m->psi, T.count, m->k, m->beta_ratio, st)) ||
+               (rc = mm_fresnel_fwd(fp, m->psi, m->psi, T.ntheta, 0, st)) ||
+               (rc = mm_abs2(m->psi, out[t], T.count, st))) {
       return rc;
+    }
   }
   return MM_OK;
 }
@@ -228,9 +234,9 @@ int mm_mosaic_info(const mm_mosaic* m, int what) {
   }
 }
 
-int mm_mosaic_tile_knob(const mm_mosaic* m, int tile) {
+int mm_mosaic_tile_knob(const mm_mosaic* m, int tile, int knob) {
   if (!m || tile < 0 || tile >= (int)m->tiles.size()) return MM_ERR_ARG;
-  return mm_radon_get_tune(m->rplans[m->tiles[tile].rplan], MM_RADON_TUNE_GATHER_XCD_GROUP);
+  return mm_radon_get_tune(m->rplans[m->tiles[tile].rplan], knob);
 }
 
 size_t mm_mosaic_workspace_bytes(const mm_mosaic* m) { return m ? m->bytes : 0; }

@@ -40,6 +40,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <thread>
 #include <type_traits>
@@ -2161,9 +2162,8 @@ int mm_radon_autotune(mm_radon* p, const float* obj, float* sino, float* back, i
     return fail(MM_ERR_HIP, "mm_radon_autotune: hipEventCreate");
   }
   int rc = MM_OK;
-  // best of two timed calls for every XCD group size; the knob keeps the fastest
-  auto sweep = [&](int knob, auto&& call) {
-    const int cand[] = {1, 2, 4, 8, 16};
+  // best of two timed calls for every candidate value; the knob keeps the fastest
+  auto sweep = [&](int knob, std::initializer_list<int> cand, auto&& call) {
     float best_ms = 1e30f;
     int best = p->tune[knob];
     if ((rc = call())) return;  // warm-up
@@ -2185,9 +2185,18 @@ int mm_radon_autotune(mm_radon* p, const float* obj, float* sino, float* back, i
     }
     p->tune[knob] = best;
   };
-  sweep(MM_RADON_TUNE_GATHER_XCD_GROUP, [&]() { return mm_radon_fwd(p, obj, sino, nz, stream); });
+  sweep(MM_RADON_TUNE_GATHER_XCD_GROUP, {1, 2, 4, 8, 16}, [&]() { return mm_radon_fwd(p, obj, sino, nz, stream); });
+  // the column split changes the workgroup shape of 4096-point column transforms only (n = 2048)
+  if (!rc && 2 * p->n == 4096) {
+    auto both = [&]() {
+      int r = mm_radon_fwd(p, obj, sino, nz, stream);
+      if (!r && back && !p->fwd_only) r = mm_radon_adj(p, sino, 0, back, nz, stream);
+      return r;
+    };
+    sweep(MM_RADON_TUNE_COL_SPLIT, {0, 1}, both);
+  }
   if (!rc && back && !p->fwd_only)
-    sweep(MM_RADON_TUNE_SCATTER_XCD_GROUP, [&]() { return mm_radon_adj(p, sino, 0, back, nz, stream); });
+    sweep(MM_RADON_TUNE_SCATTER_XCD_GROUP, {1, 2, 4, 8, 16}, [&]() { return mm_radon_adj(p, sino, 0, back, nz, stream); });
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;

@@ -56,7 +56,10 @@ class MosaicBatch:
                 "fresnel_plans": f(self._plan, _lib.MM_MOSAIC_INFO_FRESNEL_PLANS),
                 "graph_captures": f(self._plan, _lib.MM_MOSAIC_INFO_CAPTURES),
                 "tiles": f(self._plan, _lib.MM_MOSAIC_INFO_TILES),
-                "gather_xcd_group": [self._lib.mm_mosaic_tile_knob(self._plan, t) for t in range(len(self.tiles))],
+                "gather_xcd_group": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_GATHER_XCD_GROUP)
+                                     for t in range(len(self.tiles))],
+                "col_split": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_COL_SPLIT)
+                              for t in range(len(self.tiles))],
                 "workspace_bytes": int(self._lib.mm_mosaic_workspace_bytes(self._plan))}
 
     def out_buffers(self):
