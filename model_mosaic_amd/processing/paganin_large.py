@@ -4,8 +4,9 @@ Mirrors ``PaganinLarge`` of processing/paganin_large.py:42-284: constructor
 ``PaganinLarge(n, nz, wavelength, voxelsize, distance, alpha)``,
 ``retrieve(intensity, chunks)`` with intensity a host (ntheta, nz, n) or
 (nz, n) float32 array and ``chunks = [CHUNK_NZ, CHUNK_N]`` (validated with
-the reference's assertions, :124-134), returning a host float32 array of the
-same shape (an internal pinned buffer, overwritten by the next call), and
+the reference's assertions, :124-134), returning the phase as a host float32
+``BandedPinned`` of the same shape (an internal buffer banded along nz,
+overwritten by the next call; processing/paganin_large.py:106-113), and
 ``free()``.
 
 The reference streams strips of a host spectrum through a 40 GB card; here
@@ -19,11 +20,12 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from model_mosaic_amd._carriers import pinned_empty, require_gpu
+from model_mosaic_amd._carriers import require_gpu
 from model_mosaic_amd.processing.paganin import Paganin
-from model_mosaic_amd.processing.pipeline import StreamPipe
+from model_mosaic_amd.processing.pipeline import BandedPinned, StreamPipe, pick_n_bands
 
 DEFAULT_GPU_BUDGET = 96 << 30
+OUT_N_BANDS = 8  # minimum bands of the output buffer (processing/paganin_large.py:39)
 
 
 class PaganinLarge:
@@ -73,11 +75,11 @@ class PaganinLarge:
         b = self._batch(ntheta)
         pg = self._get(b)
         shape = (ntheta, self.nz, self.n)
-        if self._out is None or tuple(self._out.shape) != shape:
-            self._out = pinned_empty(shape, torch.float32)
+        if self._out is None or self._out.shape != shape:
+            self._out = BandedPinned(shape, np.float32, pick_n_bands(shape, np.float32, 1, OUT_N_BANDS), band_axis=1)
         nb = (ntheta + b - 1) // b
         if nb == 1:
-            self._out.copy_(pg.retrieve(x.to(pg.device, dtype=torch.float32).contiguous()))
+            self._out.copy_from_gpu(pg.retrieve(x.to(pg.device, dtype=torch.float32).contiguous()))
             torch.cuda.synchronize(pg.device)
         else:  # frame ranges streamed: uploads and downloads overlap the retrieval of the range between
             shape = (b, self.nz, self.n)
@@ -93,8 +95,7 @@ class PaganinLarge:
                 return pg.retrieve(d, out=res[: d.shape[0]])
 
             def store(k, r):
-                out[k * b: k * b + r.shape[0]].copy_(r)
+                out.copy_from(r.numpy(), (slice(k * b, k * b + r.shape[0]),))
 
             pipe.run(nb, load, compute, store)
-        res = self._out.numpy()
-        return res[0] if added_dim else res
+        return self._out[0] if added_dim else self._out

@@ -141,3 +141,45 @@ def test_fbp_two_n_tables_reproduce_the_padded_filter(n, name):
     ref = fbp.filter(x, w.astype(np.float64))
     # G, A, B are handed over in float32: 1e-6 relative
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-6
+
+
+def test_banded_pinned_semantics():
+    """BandedPinned (processing/pipeline.py:92-252 interface) in pageable memory: band layout, reads inside one
+    band are views, reads across bands are copies, writes of scalars and arrays, band-wise copy_to / copy_from,
+    the band-count rule, and the errors the reference raises."""
+    from model_mosaic_amd.processing.pipeline import BandedPinned, pick_n_bands
+    ref = (np.arange(2 * 12 * 5) + 1j * np.arange(2 * 12 * 5)[::-1]).reshape(2, 12, 5).astype(np.complex64)
+    bp = BandedPinned(ref.shape, np.complex64, n_bands=4, band_axis=1, pinned=False)
+    assert bp.band_rows == 3 and len(bp.bands) == 4 and bp.bands[0].shape == (2, 3, 5) and bp.nbytes == ref.nbytes
+    for i, band in enumerate(bp.bands):          # the band-wise fill of step3_propagation_large.py:210-226
+        band.real = ref[:, 3 * i:3 * i + 3].real
+        band.imag = ref[:, 3 * i:3 * i + 3].imag
+    assert np.array_equal(np.asarray(bp), ref)
+    v = bp[:, 3:6]
+    assert np.array_equal(v, ref[:, 3:6]) and np.shares_memory(v, bp.bands[1])
+    assert np.array_equal(bp[1, 2:10, 1:4], ref[1, 2:10, 1:4])
+    assert np.array_equal(bp[:, 7], ref[:, 7]) and np.array_equal(bp[0, -1], ref[0, -1])
+    assert np.array_equal(bp[1], ref[1])
+    bp[:, 4:8] = 0
+    want = ref.copy()
+    want[:, 4:8] = 0
+    assert np.array_equal(np.asarray(bp), want)
+    bp[0, 2:11, :2] = ref[0, 2:11, :2] * 2
+    want[0, 2:11, :2] = ref[0, 2:11, :2] * 2
+    assert np.array_equal(np.asarray(bp), want)
+    dst = np.zeros((1, 12, 5), np.complex64)
+    bp.copy_to(dst, (slice(1, 2),))
+    assert np.array_equal(dst[0], want[1])
+    bp.copy_from(ref[:1], (slice(0, 1),))
+    want[0] = ref[0]
+    assert np.array_equal(np.asarray(bp), want)
+    assert np.array_equal(np.abs(bp - want), np.zeros_like(want, dtype=np.float32))   # ndarray ops via __array__
+    bp.fill(3)
+    assert np.all(np.asarray(bp) == 3)
+    with pytest.raises(ValueError):
+        BandedPinned((2, 12, 5), np.float32, n_bands=5, pinned=False)
+    with pytest.raises(NotImplementedError):
+        bp[:, ::2]
+    assert pick_n_bands((4, 96, 64), np.complex64, 1, 8) == 8
+    assert pick_n_bands((4, 96, 64), np.complex64, 1, 8, max_band_bytes=4 * 6 * 64 * 8) == 16
+    assert pick_n_bands((1, 7, 3), np.float32, 1, 2) == 7
