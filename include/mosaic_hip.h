@@ -51,12 +51,15 @@ int mm_version(void);
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of
  *     MM_RADON_FP64_TILES  scatter tiles accumulate in FP64;
+ *     MM_RADON_KAHAN_TILES scatter tiles accumulate in FP32 with Kahan
+ *                          compensation (not together with FP64_TILES);
  *     MM_RADON_GATHER_ZP1  gather tiles of width <= 32 stage one z slice per
  *                          round (default: two slices interleaved per patch
  *                          element; wider tiles always stage one).
  */
 #define MM_RADON_FP64_TILES 1u
 #define MM_RADON_GATHER_ZP1 2u
+#define MM_RADON_KAHAN_TILES 4u
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags);
 
 /* R: obj (nz, n, n) float32 -> sino (ntheta, nz, n) float32, 1 <= nz <= nz_max.

@@ -29,8 +29,8 @@
 //    LDS (applying the conj reflection and the (-1)^(k0+k1) centring sign while
 //    loading) and each thread sums its 81 taps from LDS.  A scatter workgroup
 //    owns a tile of the output half plane, accumulates the taps of every
-//    sample that touches it with LDS atomics (FP32, or FP64 with
-//    MM_RADON_FP64_TILES) and writes the tile once; only tiles at the 8 x 8
+//    sample that touches it with LDS atomics (FP32, FP64 with
+//    MM_RADON_FP64_TILES, or compensated FP32 with MM_RADON_KAHAN_TILES) and writes the tile once; only tiles at the 8 x 8
 //    quadtree floor with more than 1024 samples (around the origin, where
 //    every angle passes) are split into partial tiles that a second kernel
 //    sums.  Per-sample weights stay in registers across the z loop.
@@ -809,11 +809,23 @@ constexpr int STS = 16;          // output tile edge
 constexpr int SNP = STS * STS;   // points = threads per work item
 constexpr int SWAVES = SNP / 64; // waves per work item
 constexpr int SGROUP = 16;       // lanes per LDS cycle of a 16-byte read
-constexpr int SE_MAX = 496;      // entry rows in LDS per work item
+#ifndef MM_SCATTER_SE_MAX
+#define MM_SCATTER_SE_MAX 496
+#endif
+#ifndef MM_SCATTER_SZC
+#define MM_SCATTER_SZC 8
+#endif
+#ifndef MM_SCATTER_WAVES
+#define MM_SCATTER_WAVES 4
+#endif
+#ifndef MM_SCATTER_RELPOS
+#define MM_SCATTER_RELPOS 0
+#endif
+constexpr int SE_MAX = MM_SCATTER_SE_MAX;  // entry rows in LDS per work item
 constexpr int SL_MAX = 96;       // list slots per point per work item (48 VGPRs; fewer split tiles than 80)
 constexpr int SL_CHUNK = SL_MAX - 4;  // taps per point per work item (room for idle steps)
 constexpr int SL_PAIRS_A = 32, SL_PAIRS_B = SL_MAX / 2 - SL_PAIRS_A;
-constexpr int SZC = 8;           // z slices per staging round
+constexpr int SZC = MM_SCATTER_SZC;  // z slices per staging round
 constexpr unsigned SL_IDLE_TAP = 127;  // slot = row | (9 * i1 + i0) << 9; tap code 127: weight 0
 static_assert(SE_MAX <= 512 && SL_MAX % 2 == 0 && SL_PAIRS_B == 16 && SWAVES <= 4 && SL_MAX / 2 < 256,
               "slot encoding");
@@ -823,7 +835,7 @@ struct __align__(16) SRow {
   float2 f;       // fractional offset (mirror: negated)
   int2 tk;        // theta, k | mirror << 30
 };
-static_assert(sizeof(SRow) == 80, "five 16-byte pieces per row");
+static_assert(sizeof(SRow) == 16 * (SZC / 2 + 1), "SZC / 2 + 1 16-byte pieces per row");
 
 struct SItem {
   long long loff;  // pairs of slots: lists[loff + (j / 2) * 2 SNP + 2 p + (j & 1)], j < nl
@@ -841,30 +853,65 @@ struct SplitTile {
 typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
 typedef unsigned u32x16 __attribute__((ext_vector_type(SL_PAIRS_B)));  // power-of-two sizes stay in VGPRs
 
+// Compensated (Kahan) FP32 accumulator for the scatter tiles (MM_RADON_KAHAN_TILES): the running sum s and
+// the negated low-order part c lost by its last addition; the value is s - c.
+struct KahanAcc {
+  float s = 0.f, c = 0.f;
+  __device__ KahanAcc() = default;
+  __device__ explicit KahanAcc(float v) : s(v) {}
+  __device__ __forceinline__ KahanAcc& operator+=(KahanAcc x) {
+    const float y = x.s - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+    return *this;
+  }
+  __device__ explicit operator float() const { return s - c; }
+};
+
 template <typename Acc>
-__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) void scatter_lists(
+__global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(MM_SCATTER_WAVES, MM_SCATTER_WAVES))) void scatter_lists(
     const float2* __restrict__ yh, float2* __restrict__ Hb, float2* __restrict__ partial,
     const SItem* __restrict__ items, const unsigned char* __restrict__ perm, const int* __restrict__ entries,
     const unsigned short* __restrict__ lists, const double2* __restrict__ dirs, int n, int H, int nz, long long hzs,
     int zchunk, float a2, int xg) {
   __shared__ SRow R[SE_MAX];
+#if !MM_SCATTER_RELPOS
   __shared__ float2 TT[128];  // tap code t = 9 i1 + i0 -> (i0 - M, i1 - M); idle code: far away (weight 0)
+#endif
   const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
   const unsigned item = lw % gridDim.x;
   const SItem it = items[item];
+#if !MM_SCATTER_RELPOS
   if (threadIdx.x < 128) {
     const int t = threadIdx.x;
     TT[t] = t < TAPS * TAPS ? make_float2((float)(t % TAPS - M), (float)(t / TAPS - M)) : make_float2(1e3f, 1e3f);
   }
+#endif
   const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
   const int p = threadIdx.x;                                 // lane: list lp[.. p ..]
   const int pt = perm[(long long)item * SNP + p];           // the tile point it owns
   const int l0 = it.x0 + (pt % STS), l1 = it.y0 + (pt / STS);
+#if MM_SCATTER_RELPOS
+  // entry position relative to the tile centre (x0 + 7.5, y0 + 7.5), wrapped onto this tile: the
+  // tap distance is then (point - centre) - position, with no tap code involved
+  const float pcx = (float)(pt % STS) - 7.5f, pcy = (float)(pt / STS) - 7.5f;
+#endif
   for (int e = p; e < it.ne; e += SNP) {
     const int code = entries[it.eoff + e];
     const SampleGeom g = sample_geom(code >> 1, H, dirs);
     const bool mir = (code & 1) != 0;
+#if MM_SCATTER_RELPOS
+    const int cx = mir ? -g.rx : g.rx, cy = mir ? -g.ry : g.ry;
+    const int twon = 2 * n;
+    const int ix = ((cx - it.x0 + M) % twon + twon) % twon - M, iy = ((cy - it.y0 + M) % twon + twon) % twon - M;
+    const double2 d = dirs[g.th];
+    const double X = (2.0 * g.k) * d.x, Y = (2.0 * g.k) * d.y;
+    const double fx = X - (double)g.rx, fy = Y - (double)g.ry;
+    R[e].f = make_float2((float)((double)ix - 7.5 + (mir ? -fx : fx)), (float)((double)iy - 7.5 + (mir ? -fy : fy)));
+#else
     R[e].f = mir ? make_float2(-g.fx, -g.fy) : make_float2(g.fx, g.fy);
+#endif
     R[e].tk = make_int2(g.th, g.k | (mir ? (1 << 30) : 0));
   }
   const int npairs = __builtin_amdgcn_readfirstlane((int)((it.wnp >> (8 * (p >> 6))) & 0xFFu));
@@ -907,10 +954,17 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       // f is read with the value pieces; the tap's integer offsets come from TT (no
       // division by 9, no conversions), and an idle slot's offsets are so far off that
       // its weight underflows to exactly 0 (no select); a2 = -a log2(e)
+#if MM_SCATTER_RELPOS
+      const float2 ft = *reinterpret_cast<const float2*>(&row[SZC / 2]);  // position relative to the tile centre
+      const float dx = pcx - ft.x, dy = pcy - ft.y;
+      float w = __builtin_amdgcn_exp2f(a2 * (dx * dx + dy * dy));
+      if (t == (int)SL_IDLE_TAP) w = 0.f;  // an idle step reads another lane's row
+#else
       const float4 ft = row[SZC / 2];  // f, tk
       const float2 tb = TT[t];
       const float dx = tb.x - ft.x, dy = tb.y - ft.y;
       const float w = __builtin_amdgcn_exp2f(a2 * (dx * dx + dy * dy));
+#endif
 #pragma unroll
       for (int q = 0; q < SZC / 2; ++q) {
         const float4 v = row[q];
@@ -1833,6 +1887,10 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
                        (float)(-(double)p->a * M_LOG2E), xg);
+  else if (p->flags & MM_RADON_KAHAN_TILES)
+    hipLaunchKernelGGL(scatter_lists<KahanAcc>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
+                       p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
+                       (float)(-(double)p->a * M_LOG2E), xg);
   else
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
@@ -1880,7 +1938,10 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
-  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1)) == 0, "mm_radon_create: unknown flags");
+  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES)) == 0,
+               "mm_radon_create: unknown flags");
+  MM_ARG_CHECK(!((flags & MM_RADON_FP64_TILES) && (flags & MM_RADON_KAHAN_TILES)),
+               "mm_radon_create: MM_RADON_FP64_TILES and MM_RADON_KAHAN_TILES exclude each other");
   const double eps = 1e-3;
   const double mu = -std::log(eps) / (2.0 * n * n);
   const int m = (int)std::ceil(2.0 * n / M_PI * std::sqrt(-mu * std::log(eps) + (mu * n) * (mu * n) / 4.0));

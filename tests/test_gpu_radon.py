@@ -105,8 +105,8 @@ def test_random_angles_and_partial_nz(tomo_cls):
     assert rel(t.RT(torch.from_numpy(y).cuda()).cpu().numpy(), usfft.tomo_RT(y, th, dtype=np.float64)) < 1e-5
 
 
-@pytest.mark.parametrize("fp64_tiles", [False, True])
-def test_adjoint_identity(tomo_cls, fp64_tiles):
+@pytest.mark.parametrize("fp64_tiles,kahan", [(False, False), (True, False), (False, True)])
+def test_adjoint_identity(tomo_cls, fp64_tiles, kahan):
     """<R x, y> = <x, RT y> (tests/test_large_variants.py:122-172), inner products in FP64.
 
     Random x, y make <Rx, y> cancel down to ~sqrt(N) of its terms, which
@@ -118,7 +118,7 @@ def test_adjoint_identity(tomo_cls, fp64_tiles):
     rng = np.random.default_rng(0)
     x = torch.from_numpy(rng.standard_normal((nz, n, n)).astype(np.float32)).cuda()
     y = torch.from_numpy(rng.standard_normal((nt, nz, n)).astype(np.float32)).cuda()
-    t = tomo_cls(n, nz, th, fp64_tiles=fp64_tiles)
+    t = tomo_cls(n, nz, th, fp64_tiles=fp64_tiles, kahan_tiles=kahan)
     rx = t.R(x).double()
     lhs = torch.sum(rx * y.double()).item()
     rhs = torch.sum(x.double() * t.RT(y).double()).item()
@@ -253,3 +253,21 @@ def test_large_streams_z_slabs_bitwise(tomo_cls, cplx):
     got = tl.RT(sino, [n, nt, n])
     assert got.dtype == sino.dtype
     assert np.array_equal(got, t.RT(torch.from_numpy(sino).cuda()).cpu().numpy())
+
+
+@pytest.mark.parametrize("n,nt,nz", [(256, 384, 2), (2048, 256, 1)])
+def test_scatter_accumulators_vs_fp64(tomo_cls, n, nt, nz):
+    """RT with the scatter tiles accumulated in FP32, compensated FP32 (Kahan) and FP64, each vs the FP64 oracle:
+    all within the 1e-5 bar, the compensated and FP64 sums no worse than plain FP32 (beyond rounding noise)."""
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    y = np.random.default_rng(n + 1).standard_normal((nt, nz, n)).astype(np.float32)
+    ref = usfft.tomo_RT(y, th, dtype=np.float64)
+    errs = {}
+    for name, kw in (("fp32", {}), ("kahan", {"kahan_tiles": True}), ("fp64", {"fp64_tiles": True})):
+        got = tomo_cls(n, nz, th, **kw).RT(torch.from_numpy(y).cuda()).cpu().numpy()
+        errs[name] = rel(got, ref)
+    print(f"RT n={n} ntheta={nt}: rel-L2 vs FP64 " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert max(errs.values()) < 1e-5
+    assert errs["kahan"] <= 1.05 * errs["fp32"] and errs["fp64"] <= 1.05 * errs["fp32"]
+    with pytest.raises(ValueError):
+        tomo_cls(64, 1, th[:8], fp64_tiles=True, kahan_tiles=True)
