@@ -46,8 +46,33 @@ def _compile(src: str) -> Path:
     return obj
 
 
+H5_SRC = CSRC / "h5io.cpp"
+H5_LIB = HERE / "libmosaic_h5.so"
+H5_PREFIX = Path(os.environ.get("MM_HDF5_PREFIX", "/opt/conda"))
+
+
+def build_h5(verbose: bool = False):
+    """Builds libmosaic_h5.so (the dxchange HDF5 layer, g++ on the HDF5 C library); None if the library's
+    headers are not installed (the HDF5 layer then raises when used)."""
+    if not (H5_PREFIX / "include" / "hdf5.h").exists():
+        if verbose:
+            print(f"HDF5 headers not found under {H5_PREFIX}: libmosaic_h5.so not built")
+        return None
+    tmp = H5_LIB.with_suffix(".so.tmp")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", str(H5_SRC),
+           f"-I{H5_PREFIX / 'include'}", f"-L{H5_PREFIX / 'lib'}", "-lhdf5", f"-Wl,-rpath,{H5_PREFIX / 'lib'}",
+           "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print(f"built {H5_LIB}")
+    return H5_LIB
+
+
 def build(jobs: int | None = None, verbose: bool = False) -> Path:
-    """Compiles every HIP source and links libmosaic_hip.so; returns its path."""
+    """Compiles every HIP source and links libmosaic_hip.so, then the HDF5 layer; returns the HIP library's path."""
     OBJDIR.mkdir(exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -60,6 +85,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_h5(verbose)
     return LIB
 
 
