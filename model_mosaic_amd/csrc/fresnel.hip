@@ -15,6 +15,7 @@
 // pi lambda z f^2 is formed and reduced mod 2 pi in FP64, then the FP32 sincos
 // of the reduced phase is taken and 1/(2n) resp. 1/(2nz) is folded in.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "mm_fft.h"
@@ -176,8 +177,22 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(cons
 #ifndef MM_FRESNEL_DCT_COL_BATCH
 #define MM_FRESNEL_DCT_COL_BATCH kRowBatch
 #endif
-template <int L>
-using DctColShape = RegShape<L, (L >= 2048 ? MM_FRESNEL_DCT_COL_BATCH : kBatchElems)>;
+// Short column transforms (fewer than 16 threads each) put consecutive lanes on consecutive columns of one
+// row: a wave's load or store is one contiguous run of 64 columns instead of 8-row pieces of 8 columns.
+// With lanes of one wave on different transforms at stride S in LDS, an odd S keeps every 32-lane 8-byte
+// LDS access on distinct bank pairs.
+template <int L, int BATCH>
+struct ColFastShape : RegShape<L, BATCH> {
+  using Base = RegShape<L, BATCH>;
+  static constexpr int S = Base::S0 | 1;
+  static constexpr int LDS = Base::C * S;
+  __device__ static __forceinline__ void map(int tid, int& c, int& t) {
+    c = tid % Base::C;
+    t = tid / Base::C;
+  }
+};
+template <int L, int BATCH = (L >= 2048 ? MM_FRESNEL_DCT_COL_BATCH : kBatchElems)>
+using DctColShape = std::conditional_t<(RegShape<L, BATCH>::TPC < 16), ColFastShape<L, BATCH>, RegShape<L, BATCH>>;
 template <int L>
 using DctRowShape = RegShape<L, MM_FRESNEL_DCT_ROW_BATCH, 0, false>;
 
