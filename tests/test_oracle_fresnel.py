@@ -88,3 +88,57 @@ def test_psi_and_abs2_closed_form():
     ph = p.astype(np.float64) * 1.5
     assert np.allclose(psi, np.exp(-ph * 1e-3) * np.exp(1j * ph), rtol=1e-12)
     assert np.allclose(fr.abs2(psi), np.exp(-2 * ph * 1e-3), rtol=1e-12)
+
+
+def _dct_axis_matrices(L):
+    """Unnormalised DCT-II (2 cos) and DCT-III (weights 1, 2, 2, ...) matrices of length L."""
+    k = np.arange(L)
+    c2 = 2 * np.cos(np.pi * np.outer(k, 2 * k + 1) / (2 * L))
+    c3 = np.where(k == 0, 1, 2)[None, :] * np.cos(np.pi * np.outer(2 * k + 1, k) / (2 * L))
+    return c2, c3
+
+
+def _makhoul_line(x, kk):
+    """DCT-III(kk * DCT-II(x)) of one line through one length-L FFT and one inverse FFT with the even /
+    odd reordering v_j = x_2j, v_(L-1-j) = x_(2j+1): the arithmetic of the build's mirror-padded Fresnel
+    passes, restated independently of the oracle."""
+    L = x.shape[-1]
+    k = np.arange(L)
+    v = np.concatenate([x[0::2], x[1::2][::-1]])
+    V = np.fft.fft(v)
+    c = np.exp(-1j * np.pi * k / (2 * L))
+    a, b = c * V, np.conj(c) * V[(-k) % L]
+    C = a + b
+    C[L - k[1:L // 2]] = (1j * (a - b))[1:L // 2]
+    Y = kk * C
+    U = np.conj(c) * (Y - 1j * Y[(-k) % L])
+    U[0] = Y[0]
+    u = np.fft.ifft(U) * L
+    out = np.empty_like(u)
+    out[0::2], out[1::2] = u[:L // 2], u[L // 2:][::-1]
+    return out
+
+
+def test_mirror_padded_propagator_is_dct_form():
+    """Mirror padding makes each axis of D the product DCT-III * K * DCT-II on frame-length transforms
+    (the 2N kernel is even in its index, the pad is the half-sample-symmetric extension shifted by N/2).
+    Checked against the padded 2-D oracle in FP64, with explicit cosine matrices and with the one-FFT
+    reordered form; a sign or conjugation slip in the reordered form fails."""
+    kx, ky = fr.fresnel_kernels_1d(N, NZ, WL, VOX, DIST, np.float64)
+    fker = fr.fresnel_kernel_2d(N, NZ, WL, VOX, DIST, np.float64)
+    c2x, c3x = _dct_axis_matrices(N)
+    c2y, c3y = _dct_axis_matrices(NZ)
+    for j in range(len(DIST)):
+        ref = fr.D(PSI, fker[j], np.float64)
+        kxn, kyn = kx[j, :N] / (2 * N), ky[j, :NZ] / (2 * NZ)
+        mats = np.einsum("yk,k,kl->yl", c3y, kyn, c2y)
+        matx = np.einsum("xk,k,kl->xl", c3x, kxn, c2x)
+        explicit = np.einsum("yl,blm,xm->byx", mats, PSI, matx)
+        assert rel(explicit, ref) < 1e-13
+        rows = np.apply_along_axis(_makhoul_line, 2, PSI, kxn)
+        both = np.apply_along_axis(_makhoul_line, 1, rows, kyn)
+        assert rel(both, ref) < 1e-13
+        # DT of the mirror-padded propagator is the same pipeline with conj(K)
+        conj_rows = np.apply_along_axis(_makhoul_line, 2, PSI, np.conj(kxn))
+        conj_both = np.apply_along_axis(_makhoul_line, 1, conj_rows, np.conj(kyn))
+        assert rel(conj_both, fr.DT(PSI, fker[j], np.float64)) < 1e-13
