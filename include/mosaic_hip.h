@@ -46,7 +46,10 @@ int mm_version(void);
  * (R :344, RT :582).  eps = 1e-3, mu = -ln(eps)/(2 n^2), m = 4 taps each side
  * (processing/tomo.py:39-42), 2x oversampled Fourier grid.
  *
- * mm_radon_create: n = 2^k in [16, 8192] or 3 * 2^k in [96, 6144], nz_max >= 1,
+ * mm_radon_create: n = 2^k in [16, 16384] or 3 * 2^k in [96, 12288], nz_max >= 1,
+ *   (above 8192 the 2n-point row and column passes run as two n-point halves,
+ *   split by frequency parity forward and by output parity inverse; the FBP
+ *   filter, mm_radon_adj_fbp, stays within n <= 8192),
  *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of
@@ -145,8 +148,9 @@ void mm_radon_destroy(mm_radon* plan);
  * and processing/propagation_large.py:35 PropagationLarge.D (:202); DT is the
  * exact adjoint of D (the reference has no DT, propagation_large.py:15).
  *
- * mm_fresnel_create: frame (nz, n), nz and n each 2^k in [8, 8192] or 3 * 2^k in
- *   [24, 6144];
+ * mm_fresnel_create: frame (nz, n), nz and n each 2^k in [8, 16384] or 3 * 2^k in
+ *   [24, 12288]; a side above 8192 needs pad_mode 0 and both sides 2^k >= 16 or
+ *   3 * 2^k >= 48 (it runs only on the frame-length DCT form);
  *   ndist distances (dist_host, metres), wavelength and voxelsize in metres.
  *   pad_mode 0 = symmetric mirror padding (processing/kernels.py:530-550, the
  *   reference), 1 = zero padding.  The chirp exp(-i pi lambda z f^2) is built in

@@ -216,7 +216,8 @@ __device__ __forceinline__ int dct_src(int m, int t) {
   return m < Sh::PT / 2 ? 2 * j : 2 * N - 1 - 2 * j;
 }
 
-// c_k = exp(-i pi k / 2N) from the length-2N table (tw2 = bank + tw_offset(2N)); hw = exp(-i pi / 2N).
+// c_k = exp(-i pi k / 2N) from the first N/2 values of the length-2N table (tw2: the bank's, or the plan's own
+// for N > kMaxFFT / 2); hw = exp(-i pi / 2N).
 __device__ __forceinline__ float2 dct_twiddle(int k, const float2* __restrict__ tw2, float2 hw) {
   const float2 w = tw2[k >> 1];
   return (k & 1) ? cmul(w, hw) : w;
@@ -407,6 +408,9 @@ struct mm_fresnel {
   bool dct = false;      // mirror padding on the DCT form (both frame sides are DCT lengths)
   float2* Kx = nullptr;  // ndist x 2n
   float2* Ky = nullptr;  // ndist x 2nz
+  // exp(-2 pi i j / 2N), j < N/2, for a DCT side N whose 2N-point twiddles are past the bank (N > kMaxFFT / 2)
+  float2* cx2 = nullptr;
+  float2* cy2 = nullptr;
   const float2* tw = nullptr;
   size_t bytes = 0;
 };
@@ -415,6 +419,8 @@ static void fresnel_free(mm_fresnel* p) {
   if (!p) return;
   if (p->Kx) (void)hipFree(p->Kx);
   if (p->Ky) (void)hipFree(p->Ky);
+  if (p->cx2) (void)hipFree(p->cx2);
+  if (p->cy2) (void)hipFree(p->cy2);
   delete p;
 }
 
@@ -440,8 +446,14 @@ static float2 half_step(int N) {
   return make_float2((float)std::cos(a), (float)std::sin(a));
 }
 
-// The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48) and the length-2N twiddles.
-static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && 2 * N <= kMaxFFT; }
+// The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48); c_k comes from the bank's length-2N
+// table, or from the plan's own half table when 2N is past the bank.
+static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && N <= kMaxFFT; }
+template <int L>
+static const float2* dct_tw2(const mm_fresnel* p, bool col) {
+  if constexpr (2 * L <= kMaxFFT) return p->tw + tw_offset(2 * L);
+  else return col ? p->cy2 : p->cx2;
+}
 
 template <int L, bool CONJ_K, class Load, class Store>
 static void launch_dct_col(const mm_fresnel* p, Load ld, Store sto, int nframes, int j, double* partial,
@@ -449,7 +461,7 @@ static void launch_dct_col(const mm_fresnel* p, Load ld, Store sto, int nframes,
   using Sh = DctColShape<L>;
   hipLaunchKernelGGL((dct_col_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
                      dim3(Sh::NT), 0, st, ld, sto, p->n, p->Ky + (size_t)j * 2 * L, p->tw + tw_offset(L),
-                     p->tw + tw_offset(2 * L), half_step(L), partial);
+                     dct_tw2<L>(p, true), half_step(L), partial);
 }
 
 template <int L, bool CONJ_K, class Load, class Store>
@@ -458,7 +470,7 @@ static void launch_dct_row(const mm_fresnel* p, Load ld, Store sto, int nframes,
   using Sh = DctRowShape<L>;
   const long long rows = (long long)nframes * p->nz;
   hipLaunchKernelGGL((dct_row_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT),
-                     0, st, ld, sto, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), p->tw + tw_offset(2 * L),
+                     0, st, ld, sto, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), dct_tw2<L>(p, false),
                      half_step(L), partial);
 }
 
@@ -582,13 +594,16 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
                       const double* dist_host, int pad_mode) {
   MM_ARG_CHECK(plan != nullptr, "mm_fresnel_create: plan is NULL");
   *plan = nullptr;
-  auto size_ok = [](int v) { return ((is_pow2(v) && v >= 8) || (is_fft_len(v) && v >= 24)) && 2 * v <= kMaxFFT; };
-  MM_ARG_CHECK(size_ok(n), "mm_fresnel_create: n must be 2^k in [8, 8192] or 3 * 2^k in [24, 6144], got " +
+  auto size_ok = [](int v) { return ((is_pow2(v) && v >= 8) || (is_fft_len(v) && v >= 24)) && v <= kMaxFFT; };
+  MM_ARG_CHECK(size_ok(n), "mm_fresnel_create: n must be 2^k in [8, 16384] or 3 * 2^k in [24, 12288], got " +
                                std::to_string(n));
-  MM_ARG_CHECK(size_ok(nz), "mm_fresnel_create: nz must be 2^k in [8, 8192] or 3 * 2^k in [24, 6144], got " +
+  MM_ARG_CHECK(size_ok(nz), "mm_fresnel_create: nz must be 2^k in [8, 16384] or 3 * 2^k in [24, 12288], got " +
                                 std::to_string(nz));
   MM_ARG_CHECK(ndist >= 1 && dist_host != nullptr, "mm_fresnel_create: need ndist >= 1 distances");
   MM_ARG_CHECK(pad_mode == 0 || pad_mode == 1, "mm_fresnel_create: pad_mode must be 0 (symmetric) or 1 (zero)");
+  // a side past 8192 runs only on the frame-length DCT form (the padded form's transforms are 2N long)
+  MM_ARG_CHECK((2 * n <= kMaxFFT && 2 * nz <= kMaxFFT) || (pad_mode == 0 && dct_len_ok(n) && dct_len_ok(nz)),
+               "mm_fresnel_create: sides above 8192 need symmetric padding and both sides 2^k >= 16 or 3 * 2^k >= 48");
   MM_ARG_CHECK(voxelsize > 0 && wavelength >= 0, "mm_fresnel_create: bad wavelength / voxelsize");
   mm_fresnel* p = new mm_fresnel();
   auto bail = [&](int code) {
@@ -611,6 +626,22 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
     return bail(fail(MM_ERR_OOM, "fresnel chirp tables"));
   }
   p->bytes = sizeof(float2) * (size_t)ndist * 2 * (n + nz);
+  for (int axis = 0; axis < 2 && p->dct; ++axis) {
+    const int N = axis ? nz : n;
+    if (2 * N <= kMaxFFT) continue;
+    std::vector<float2> h(N / 2);
+    for (int k = 0; k < N / 2; ++k) {
+      const double a = -M_PI * (double)k / N;  // exp(-2 pi i k / 2N)
+      h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    float2** dst = axis ? &p->cy2 : &p->cx2;
+    if (hipMalloc((void**)dst, sizeof(float2) * h.size()) != hipSuccess ||
+        hipMemcpy(*dst, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(ddist);
+      return bail(fail(MM_ERR_OOM, "fresnel DCT twiddles"));
+    }
+    p->bytes += sizeof(float2) * h.size();
+  }
   hipError_t e = hipMemcpy(ddist, dist_host, sizeof(double) * ndist, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(chirp_table, dim3((ndist * 2 * n + 255) / 256), dim3(256), 0, 0, p->Kx, ddist, ndist, 2 * n,

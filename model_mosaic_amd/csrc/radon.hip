@@ -1170,6 +1170,243 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
 }
 
 // ---------------------------------------------------------------------------
+// 2n-point transforms longer than one workgroup holds (n > kMaxFFT / 2)
+// ---------------------------------------------------------------------------
+// The 2n-point rows and columns of R and RT have n nonzero inputs (forward) or
+// n kept outputs (inverse), centred at n/2; each becomes two n-point transforms
+// of one workgroup, with h_j = exp(-i pi j / n) from a per-plan FP64-built
+// table (hsplit):
+//   forward, split by the parity of the frequency (x centred at n/2):
+//     Z[2m] = (-1)^m V[m],  Z[2m+1] = -i (-1)^m W[m],  V = FFT_n(x),  W = FFT_n(x_j h_j);
+//   inverse, split by the parity of the output point:
+//     y[2r] = IFFT_n(Z[k] + Z[k+n])[r],  y[2r+1] = IFFT_n((Z[k] - Z[k+n]) conj(h_k))[r],
+//     and the kept points p in [n/2, 3n/2) are r in [n/4, 3n/4) of both halves.
+// Both halves are formed by one workgroup, one after the other: every input is
+// in registers before the first transform's first barrier, so the in-place
+// column passes may then write, and the first half's results leave before the
+// second transform (one register copy of the inputs is live, not two).
+template <int L>  // L = n: one transform per workgroup
+using SplitShape = RegShape<L, L>;
+
+// The twiddle table pointer and thread index for a kernel's second transform, opaque to the compiler:
+// otherwise it reuses the first transform's twiddle loads and LDS addresses and keeps them live across
+// both (past the register budget).
+__device__ __forceinline__ const float2* opaque_twiddles(const float2* tw) {
+  asm volatile("" : "+s"(tw));
+  return tw;
+}
+__device__ __forceinline__ int opaque_index(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// R's row pass: workgroup 2 u + par forms the even (par = 0) or odd (par = 1)
+// half-spectrum columns of row pair u (rows y0, y0 + 1 of slice z), unpacking
+// the two real rows from the complex transform as r1_rows_rfft does:
+// A_k = (Z_k + conj Z_{2n-k}) / 2, B_k = (Z_k - conj Z_{2n-k}) / (2i), where
+// Z_{2n-k} is V[n-m] (k = 2m) or W[n-1-m] (k = 2m+1) with the same sign.
+template <int L>
+__global__ __launch_bounds__(SplitShape<L>::NT) void r1_rows_split(const float* __restrict__ obj,
+                                                                 float2* __restrict__ fde,
+                                                                 const float* __restrict__ phx,
+                                                                 const float* __restrict__ phy,
+                                                                 const float2* __restrict__ hsplit, int nz,
+                                                                 long long fzs, const float2* __restrict__ tw) {
+  using Sh = SplitShape<L>;
+  static_assert(Sh::C == 1 && Sh::NT == Sh::TPC, "one transform per workgroup");
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int half = L / 2;
+  // the four workgroups of two row pairs (one 128-byte line of every block) share an L2
+  const unsigned wg = xcd_group(blockIdx.x, gridDim.x, 4);
+  const long long pair = wg >> 1;
+  const int par = (int)(wg & 1u);
+  if (pair >= (long long)nz * half) return;  // the whole workgroup, before any barrier
+  const int t = threadIdx.x;
+  const int z = (int)(pair / half), y0 = 2 * (int)(pair - (long long)z * half);
+  const float* ra = obj + ((long long)z * L + y0) * L;
+  const float* rb = ra + L;
+  const float pya = phy[y0], pyb = phy[y0 + 1];
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int j = t + m * Sh::TPC;
+    const float p = phx[j];
+    const float2 w = make_float2(ra[j] * pya * p, rb[j] * pyb * p);
+    v[m] = par ? cmul(w, hsplit[j]) : w;
+  }
+  fft_reg<L, Sh::TPC, false>(v, lds, t, tw);
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) lds[lpad(t + m * Sh::TPC)] = v[m];
+  __syncthreads();
+  const long long rowa = (long long)z * fzs + 4LL * (half + y0);  // hs_idx(half + y0, col) minus the column part
+  const int cnt = par ? half : half + 1;
+  for (int mm = t; mm < cnt; mm += Sh::NT) {
+    const int col = 2 * mm + par;
+    const float2 zk = lds[lpad(mm)];
+    const float2 zm = lds[lpad(par ? L - 1 - mm : (mm == 0 ? 0 : L - mm))];
+    float2 a = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));  // (zk + conj zm) / 2
+    float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y + zm.y));  // (zk - conj zm) / 2
+    float2 b;
+    if (par) {
+      a = make_float2(a.y, -a.x);  // -i a
+      b = make_float2(-d.x, -d.y);
+    } else {
+      b = make_float2(d.y, -d.x);  // d / i
+    }
+    if (mm & 1) {
+      a = make_float2(-a.x, -a.y);
+      b = make_float2(-b.x, -b.y);
+    }
+    const long long o = rowa + (long long)(col >> 2) * 4 * (2 * L) + (col & 3);
+    fde[o] = a;      // row half + y0
+    fde[o + 4] = b;  // row half + y0 + 1
+  }
+}
+
+// Column pass (in place) of one half-spectrum column of a z plane: forward, the
+// object rows [n/2, 3n/2) to the rows of the disc (supp); inverse, the disc's
+// rows to rows [n/2, 3n/2), unnormalised (the C2R row pass applies both 1/(2n)).
+template <int L, bool INV>
+__global__ __launch_bounds__(SplitShape<L>::NT) void col_fft_split(float2* __restrict__ buf, int ncols, long long zs,
+                                                                 const int2* __restrict__ supp,
+                                                                 const float2* __restrict__ hsplit,
+                                                                 const float2* __restrict__ tw) {
+  using Sh = SplitShape<L>;
+  static_assert(Sh::C == 1 && Sh::NT == Sh::TPC, "one transform per workgroup");
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int L2 = 2 * L;
+  // the four columns of a block (one 32-byte row piece) share an L2
+  const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, 4);
+  const int col = (int)(lw % gridDim.x);
+  if (col >= ncols) return;  // the whole workgroup, before any barrier
+  // column col of the z plane: row i at cb[4 i] (hs_idx with 32-bit offsets from one base)
+  float2* cb = buf + (long long)(lw / gridDim.x) * zs + (long long)(col >> 2) * (4 * L2) + (col & 3);
+  const int2 sp = supp[col >> 2];
+  auto carries = [&](int i) {
+    const int l = i < L ? i : i - L2;
+    return l >= sp.x && l < sp.y;
+  };
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+  if constexpr (!INV) {
+    float2 x[Sh::PT];
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      x[m] = cb[4 * (L / 2 + t + m * Sh::TPC)];
+      v[m] = x[m];
+    }
+    fft_reg<L, Sh::TPC, false>(v, lds, t, tw);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int mm = t + m * Sh::TPC;
+      if (carries(2 * mm)) cb[4 * (2 * mm)] = (mm & 1) ? make_float2(-v[m].x, -v[m].y) : v[m];
+      v[m] = cmul(x[m], hsplit[mm]);
+    }
+    fft_reg<L, Sh::TPC, false>(v, lds, opaque_index(t), opaque_twiddles(tw));
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int mm = t + m * Sh::TPC;
+      const float2 w = make_float2(v[m].y, -v[m].x);  // -i W
+      if (carries(2 * mm + 1)) cb[4 * (2 * mm + 1)] = (mm & 1) ? make_float2(-w.x, -w.y) : w;
+    }
+  } else {
+    float2 d[Sh::PT];  // (Z[k] - Z[k+n]) conj(h_k): the odd output points' half
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int k = t + m * Sh::TPC;
+      const float2 lo = carries(k) ? cb[4 * k] : make_float2(0.f, 0.f);
+      const float2 hi = carries(k + L) ? cb[4 * (k + L)] : make_float2(0.f, 0.f);
+      v[m] = make_float2(lo.x + hi.x, lo.y + hi.y);
+      d[m] = cmul(make_float2(lo.x - hi.x, lo.y - hi.y), cconj(hsplit[k]));
+    }
+    constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4;  // r in [n/4, 3n/4)
+    static_assert(Sh::PT % 4 == 0, "n/4 is a multiple of TPC");
+    fft_reg<L, Sh::TPC, true>(v, lds, t, tw);
+#pragma unroll
+    for (int m = M0; m < M1; ++m) cb[8 * (t + m * Sh::TPC)] = v[m];
+    fft_reg<L, Sh::TPC, true>(d, lds, opaque_index(t), opaque_twiddles(tw));
+#pragma unroll
+    for (int m = M0; m < M1; ++m) cb[8 * (t + m * Sh::TPC) + 4] = d[m];
+  }
+}
+
+// RT's row pass (C2R, crop, phi) for 2n-point rows: Z = A + i B over the full
+// 2n spectrum from the half-spectrum rows (Hermitian extension, imaginary parts
+// dropped at k = 0 and k = n as rt4_rows_c2r does), the output-parity halves
+// above, and output point p = 2r + parity to column p - n/2.  One unit (a row
+// pair, or one complex row) per workgroup; Z[k] and Z[k+n] are read straight
+// from HBM (columns k and n - k: consecutive lanes, consecutive columns).
+template <int L, bool CPLX>
+__global__ __launch_bounds__(SplitShape<L>::NT) void rt4_rows_c2r_split(const float2* __restrict__ Hre,
+                                                                      const float2* __restrict__ Him,
+                                                                      void* __restrict__ out_v,
+                                                                      const float* __restrict__ phx,
+                                                                      const float* __restrict__ phy,
+                                                                      const float2* __restrict__ hsplit, int nz,
+                                                                      long long hzs, float scale,
+                                                                      const float2* __restrict__ tw) {
+  using Sh = SplitShape<L>;
+  static_assert(Sh::C == 1 && Sh::NT == Sh::TPC, "one transform per workgroup");
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int half = L / 2, L2 = 2 * L;
+  const long long nunits = CPLX ? (long long)nz * L : (long long)nz * half;
+  // the units of one 128-byte line of every block (4 rows) share an L2
+  const unsigned wg = xcd_group(blockIdx.x, gridDim.x, CPLX ? 4 : 2);
+  const long long u = wg;
+  if (u >= nunits) return;  // the whole workgroup, before any barrier
+  int z, y;
+  if (CPLX) {
+    z = (int)(u / L);
+    y = (int)(u - (long long)z * L);
+  } else {
+    z = (int)(u / half);
+    y = 2 * (int)(u - (long long)z * half);
+  }
+  const float2* pa = Hre + (long long)z * hzs + 4LL * (half + y);
+  const float2* pb = CPLX ? Him + (long long)z * hzs + 4LL * (half + y) : pa + 4;
+  auto at = [&](const float2* row, int col) { return row[(col >> 2) * (4 * L2) + (col & 3)]; };  // < 2^31
+  auto comb = [](float2 a, float2 bb) { return make_float2(a.x - bb.y, a.y + bb.x); };  // a + i b
+  const int t = threadIdx.x;
+  float2 v[Sh::PT], d[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int k = t + m * Sh::TPC;
+    float2 a0 = at(pa, k), b0 = at(pb, k);        // Z[k], k < n
+    float2 a1 = at(pa, L - k), b1 = at(pb, L - k);  // Z[k + n] = conj of column n - k (k > 0), column n (k = 0)
+    if (k == 0) {
+      a0.y = b0.y = 0.f;
+      a1.y = b1.y = 0.f;
+    } else {
+      a1.y = -a1.y;
+      b1.y = -b1.y;
+    }
+    const float2 lo = comb(a0, b0), hi = comb(a1, b1);
+    v[m] = make_float2(lo.x + hi.x, lo.y + hi.y);
+    d[m] = cmul(make_float2(lo.x - hi.x, lo.y - hi.y), cconj(hsplit[k]));
+  }
+  constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4;  // r in [n/4, 3n/4): output columns 2r + parity - n/2
+  static_assert(Sh::PT % 4 == 0, "n/4 is a multiple of TPC");
+  const float pya = phy[y] * scale, pyb = CPLX ? 0.f : phy[y + 1] * scale;
+  auto emit = [&](float2 yv, int xx) {
+    const float px = phx[xx];
+    if (CPLX) {
+      const float pw = pya * px;
+      ((float2*)out_v)[((long long)z * L + y) * L + xx] = make_float2(yv.x * pw, yv.y * pw);
+    } else {
+      float* o2 = (float*)out_v + ((long long)z * L + y) * L;
+      o2[xx] = yv.x * (pya * px);
+      o2[L + xx] = yv.y * (pyb * px);
+    }
+  };
+  fft_reg<L, Sh::TPC, true>(v, lds, t, tw);
+#pragma unroll
+  for (int m = M0; m < M1; ++m) emit(v[m], 2 * (t + m * Sh::TPC) - half);
+  fft_reg<L, Sh::TPC, true>(d, lds, opaque_index(t), opaque_twiddles(tw));
+#pragma unroll
+  for (int m = M0; m < M1; ++m) emit(d[m], 2 * (t + m * Sh::TPC) + 1 - half);
+}
+
+// ---------------------------------------------------------------------------
 // Host side: geometry, binning, launches
 // ---------------------------------------------------------------------------
 
@@ -1707,6 +1944,8 @@ struct mm_radon {
   int n_groot = 0;  // gather items of the root class (the first n_groot of gitems)
   int2* gsupp = nullptr;  // per 4-column block: centred rows the gather reads
   int2* ssupp = nullptr;  // per 4-column block: centred rows the scatter writes
+  float2* hsplit = nullptr;  // exp(-i pi j / n), j < n: the 2n-point passes run as two n-point halves (split)
+  bool split = false;        // 2n > kMaxFFT
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
   int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   SItem* sitems = nullptr;
@@ -1741,7 +1980,7 @@ static void radon_free(mm_radon* p) {
   if (!p->own_ws) p->fde = p->gh = nullptr;
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
                   p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
-                  p->splits, p->gsupp, p->ssupp};
+                  p->splits, p->gsupp, p->ssupp, p->hsplit};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1842,6 +2081,41 @@ static void launch_rt4(const mm_radon* p, void* out, bool cplx, int nz, hipStrea
                        p->n, nz, p->zs, scale, p->tw + tw_offset(L));
 }
 
+// 2n-point passes as two n-point halves (n > kMaxFFT / 2): one workgroup per row pair and parity, per
+// column, per C2R unit
+template <int L>
+static void launch_r1_split(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
+  const long long wgs = 2LL * nz * (L / 2);
+  hipLaunchKernelGGL(r1_rows_split<L>, dim3((unsigned)wgs), dim3(SplitShape<L>::NT), 0, st, obj, p->fde, p->phx,
+                     p->phy, p->hsplit, nz, p->zs, p->tw + tw_offset(L));
+}
+
+template <int L, bool INV>
+static void launch_col_split(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
+  const int ncols = L + 1;
+  hipLaunchKernelGGL((col_fft_split<L, INV>), dim3((unsigned)ncols, (unsigned)nz), dim3(SplitShape<L>::NT), 0, st,
+                     buf, ncols, p->zs, INV ? p->ssupp : p->gsupp, p->hsplit, p->tw + tw_offset(L));
+}
+
+template <int L>
+static void launch_rt4_split(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
+  const long long units = cplx ? (long long)nz * L : (long long)nz * (L / 2);
+  const float scale = (float)(1.0 / (4.0 * (double)L * L));  // column and row inverse passes, 1/(2n) each
+  if (cplx)
+    hipLaunchKernelGGL((rt4_rows_c2r_split<L, true>), dim3((unsigned)units), dim3(SplitShape<L>::NT), 0, st, p->fde,
+                       p->fde2, out, p->phx, p->phy, p->hsplit, nz, p->zs, scale, p->tw + tw_offset(L));
+  else
+    hipLaunchKernelGGL((rt4_rows_c2r_split<L, false>), dim3((unsigned)units), dim3(SplitShape<L>::NT), 0, st, p->fde,
+                       p->fde2, out, p->phx, p->phy, p->hsplit, nz, p->zs, scale, p->tw + tw_offset(L));
+}
+
+// Runs CALL with constexpr int LL = n for the detector widths whose 2n-point passes are split.
+#define MM_DISPATCH_SPLIT(N_RUNTIME, CALL)                                                            \
+  switch (N_RUNTIME) {                                                                                \
+    case 12288: { constexpr int LL = 12288; CALL; } break;                                            \
+    case 16384: { constexpr int LL = 16384; CALL; } break;                                            \
+    default: return ::mm::fail(MM_ERR_UNSUPPORTED, "split 2n transforms not built for n=" + std::to_string(N_RUNTIME)); \
+  }
 
 static int zsplit_for(int items, int nz) {
   if (items <= 0) return 1;
@@ -1933,8 +2207,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
                       float2* fde_ext, float2* gh_ext, bool fwd_only) {
   MM_ARG_CHECK(plan != nullptr, "mm_radon_create: plan is NULL");
   *plan = nullptr;
-  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && 2 * n <= kMaxFFT,
-               "mm_radon_create: n must be 2^k in [16, 8192] or 3 * 2^k in [96, 6144], got " + std::to_string(n));
+  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && n <= kMaxFFT,
+               "mm_radon_create: n must be 2^k in [16, 16384] or 3 * 2^k in [96, 12288], got " + std::to_string(n));
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
@@ -1966,6 +2240,7 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   p->a = (float)(M_PI * M_PI / (mu * 4.0 * n * n));
   p->C0g = (float)(M_PI / mu / n);
   p->C0s = (float)(M_PI / mu);
+  p->split = 2 * n > kMaxFFT;
 
   // separable phi (processing/tomo.py:46-49,59): phi(y,x) = phy[y] * phx[x]
   std::vector<float> phx(n), phy(n);
@@ -2016,6 +2291,14 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
       (rc = upload(&p->splits, BS.splits, bytes)) || (rc = upload(&p->gsupp, gsupp, bytes)) ||
       (rc = upload(&p->ssupp, ssupp, bytes)))
     return bail(rc);
+  if (p->split) {
+    std::vector<float2> h(n);
+    for (int j = 0; j < n; ++j) {
+      const double a = -M_PI * (double)j / n;
+      h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if ((rc = upload(&p->hsplit, h, bytes))) return bail(rc);
+  }
   p->n_gitems = (int)BG.items.size();
   p->n_groot = BG.n_root;
   p->n_sitems = (int)BS.items.size();
@@ -2044,9 +2327,17 @@ static int radon_fwd_chain(mm_radon* p, const float* obj, float* sino, int nz, h
   const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 0, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
+  if (p->split) {
+    MM_DISPATCH_SPLIT(L1, (launch_r1_split<LL>(p, obj, nz, st)));
+  } else {
+    MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
+  }
   if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, st)));
+  if (p->split) {
+    MM_DISPATCH_SPLIT(L1, (launch_col_split<LL, false>(p, p->fde, nz, st)));
+  } else {
+    MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, st)));
+  }
   if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
   if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
   MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
@@ -2151,20 +2442,27 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
 
 // RT after the line pass: scatter, column IFFT, C2R rows.
 static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t st) {
-  const int L2 = 2 * p->n;
+  const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 5, 0, st))) return rc;
   if ((rc = launch_scatter(p, p->gh, p->fde, nz, st))) return rc;
   if (cplx && (rc = launch_scatter(p, p->gh2, p->fde2, nz, st))) return rc;
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde, nz, st)));
-  if ((rc = check_launch("col_fft (RT)"))) return rc;
-  if (cplx) {
-    MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, p->fde2, nz, st)));
-    if ((rc = check_launch("col_fft (RT im)"))) return rc;
+  for (int ch = 0; ch < (cplx ? 2 : 1); ++ch) {
+    float2* buf = ch ? p->fde2 : p->fde;
+    if (p->split) {
+      MM_DISPATCH_SPLIT(L1, (launch_col_split<LL, true>(p, buf, nz, st)));
+    } else {
+      MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, buf, nz, st)));
+    }
+    if ((rc = check_launch(ch ? "col_fft (RT im)" : "col_fft (RT)"))) return rc;
   }
   if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
-  MM_DISPATCH_FFT(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
+  if (p->split) {
+    MM_DISPATCH_SPLIT(L1, (launch_rt4_split<LL>(p, obj, cplx, nz, st)));
+  } else {
+    MM_DISPATCH_FFT(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
+  }
   if ((rc = check_launch("rt4_rows_c2r"))) return rc;
   return mark(p, 7, 1, st);
 }
@@ -2176,6 +2474,8 @@ int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float
                "mm_radon_adj_fbp: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
   MM_ARG_CHECK(sino && G && A && B && obj, "mm_radon_adj_fbp: null pointer");
   if (p->fwd_only) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: forward-only plan (no scatter tables)");
+  if (2 * p->n > kMaxFFT)
+    return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: the FBP filter's 2n-point transforms are built for n <= 8192");
   hipStream_t st = (hipStream_t)stream;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
@@ -2296,7 +2596,7 @@ int mm_radon_stage_ms(mm_radon* p, float* ms, int max_stages) {
 // every list slot naming a staged entry row).  Builds the tables like
 // mm_radon_create, touches no device.
 int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
-  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && 2 * n <= kMaxFFT,
+  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && n <= kMaxFFT,
                "mm_radon_plan_check: unsupported n " + std::to_string(n));
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_plan_check: need ntheta >= 1 angles");
   HostGeom G;

@@ -38,6 +38,10 @@ import math
 import numpy as np
 import scipy.fft as sfft
 
+# threads of the 2-D transforms in tomo_R / tomo_RT (results do not depend on it: each 1-D transform is
+# computed the same way on whichever thread runs it)
+WORKERS = 1
+
 
 def usfft_params(n):
     """epsilon, mu, m of processing/tomo.py:39-42 (and tomo_large.py:73-76)."""
@@ -220,7 +224,7 @@ def tomo_R(obj, theta, dtype=np.float32):
     phi = phi_2d(n, ntheta, dtype)
     padded = np.zeros((nz, 2 * n, 2 * n), dtype=dtype)
     padded[:, n // 2:3 * n // 2, n // 2:3 * n // 2] = obj.astype(dtype) * phi
-    fde = sfft.rfft2(padded, axes=(-2, -1)).astype(cdt)
+    fde = sfft.rfft2(padded, axes=(-2, -1), workers=WORKERS).astype(cdt)
     sino = gather_half(fde, theta, n, dtype, "real")
     c1d = c1dfftshift(n)
     sino = (sino * c1d).astype(cdt)
@@ -245,7 +249,7 @@ def tomo_RT(data, theta, n=None, dtype=np.float32):
     fde = scatter_full(s, theta, n, dtype)
     c2d = c2dfftshift(n)
     fde = (fde * c2d).astype(cdt)
-    fde = sfft.ifft2(fde, axes=(-2, -1)).astype(cdt)
+    fde = sfft.ifft2(fde, axes=(-2, -1), workers=WORKERS).astype(cdt)
     fde = (fde * c2d).astype(cdt)
     phi = phi_2d(n, ntheta, dtype)
     res = (fde[:, n // 2:3 * n // 2, n // 2:3 * n // 2] * phi).astype(cdt)

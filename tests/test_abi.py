@@ -57,12 +57,16 @@ def test_null_plan_is_an_argument_error(lib):
 def test_bad_sizes_rejected_before_device_work(lib):
     p = C.c_void_p()
     th = (C.c_double * 4)(0, 0.1, 0.2, 0.3)
-    for n in (100, 48, 160, 12288):   # not 2^k / 3 * 2^k, 3 * 2^k below 96, 5 * 2^k, 2n above the largest FFT
+    for n in (100, 48, 160, 24576):   # not 2^k / 3 * 2^k, 3 * 2^k below 96, 5 * 2^k, n above the largest FFT
         rc = lib.mm_radon_create(C.byref(p), n, 1, 4, th, 0)
         assert rc == _lib.MM_ERR_ARG and not p, n
     for n, nz in ((64, 40), (20, 64), (64, 12)):   # 5 * 2^3, 5 * 2^2, 3 * 2^2 below 24
         rc = lib.mm_fresnel_create(C.byref(p), n, nz, 1, 1e-10, 1e-6, (C.c_double * 1)(0.1), 0)
         assert rc == _lib.MM_ERR_ARG and not p, (n, nz)
+    # a side above 8192 runs only on the frame-length DCT form: symmetric padding, both sides DCT lengths
+    for n, nz, pad in ((16384, 64, 1), (16384, 8, 0), (24, 12288, 0)):
+        rc = lib.mm_fresnel_create(C.byref(p), n, nz, 1, 1e-10, 1e-6, (C.c_double * 1)(0.1), pad)
+        assert rc == _lib.MM_ERR_ARG and not p, (n, nz, pad)
     rc = lib.mm_psi_from_proj(None, None, -1, 1.0, 1.0, None)
     assert rc == _lib.MM_ERR_ARG
 
