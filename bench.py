@@ -423,7 +423,7 @@ def run_mosaic(torch, reps=2):
            "s_per_batch": round(dt, 5), "projections_per_s": round(nproj / dt, 1),
            "algorithmic_GB": round(alg / 1e9, 3), "hbm_frac": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
            "plan_s": round(plan_s, 2), "radon_plans": info["radon_plans"], "fresnel_plans": info["fresnel_plans"],
-           "graph_captures": info["graph_captures"],
+           "graph_captures": info["graph_captures"], "branches": info["branches"],
            "tuned_per_plan": sorted({(t["n"], t["ntheta"], g, c) for t, g, c in
                                      zip(spec, info["gather_xcd_group"], info["col_split"])}),
            "workspace_GB": round(info["workspace_bytes"] / 1e9, 2)}

@@ -220,8 +220,10 @@ int mm_abs2(const void* z_dev, float* out_dev, long long count, void* stream);
  * step2_radon.py:147,191 (R per tile chunk) followed by
  * step3_propagation.py:175,223 (psi, D, |.|^2 per angle chunk, :94-110).
  * Tiles of equal (n, angles) share a forward-only Radon plan, tiles of equal
- * (n, nz) a Fresnel plan; all tiles share the workspaces (sized for the
- * largest).  Every n[t] must be a valid Radon width and every (nz[t], n[t])
+ * (n, nz) a Fresnel plan.  The tiles are split over B concurrent branches
+ * of the graph (flags MM_MOSAIC_BRANCHES(B), B in [1, 8], default 2; longest
+ * first, balancing the tiles' HBM bytes), each with its own workspaces (sized
+ * for its largest tile) and stream; a branch runs its tiles in order.  Every n[t] must be a valid Radon width and every (nz[t], n[t])
  * a valid Fresnel frame (see mm_radon_create / mm_fresnel_create).  delta[t] (nz, n, n) float32, out[t] (ntheta, nz, n) float32,
  * device pointers.
  * mm_mosaic_run enqueues the whole batch as one HIP graph (captured on first
@@ -229,20 +231,25 @@ int mm_abs2(const void* z_dev, float* out_dev, long long count, void* stream);
  * ordered after prior work on `stream` and before later work on it.
  * flags: MM_MOSAIC_NO_GRAPH enqueues the launches directly (same results);
  * MM_MOSAIC_AUTOTUNE runs mm_radon_autotune for every Radon plan on its
- * first tile's input at the first run (synchronous). */
+ * first tile's input at the first run (synchronous).  Results do not depend
+ * on the branch count. */
 typedef struct mm_mosaic mm_mosaic;
 #define MM_MOSAIC_NO_GRAPH 1u
 #define MM_MOSAIC_AUTOTUNE 2u
+#define MM_MOSAIC_BRANCHES(b) ((unsigned)(b) << 8)
 #define MM_MOSAIC_INFO_RADON_PLANS 0
 #define MM_MOSAIC_INFO_FRESNEL_PLANS 1
 #define MM_MOSAIC_INFO_CAPTURES 2
 #define MM_MOSAIC_INFO_TILES 3
+#define MM_MOSAIC_INFO_BRANCHES 4
 int mm_mosaic_create(mm_mosaic** plan, int ntiles, const int* n, const int* nz, const int* ntheta,
                      const double* theta_all, double wavelength, double voxelsize, double distance, float k,
                      float beta_ratio, int pad_mode, unsigned flags);
 int mm_mosaic_run(mm_mosaic* plan, const float* const* delta_dev, float* const* out_dev, void* stream);
 int mm_mosaic_info(const mm_mosaic* plan, int what);
-/* Launch knob (MM_RADON_TUNE_*) of tile t's Radon plan (after autotuning: the chosen value). */
+/* Launch knob (MM_RADON_TUNE_*) of tile t's Radon plan (after autotuning: the chosen value), or with
+ * MM_MOSAIC_TILE_BRANCH the graph branch tile t runs on. */
+#define MM_MOSAIC_TILE_BRANCH 100
 int mm_mosaic_tile_knob(const mm_mosaic* plan, int tile, int knob);
 size_t mm_mosaic_workspace_bytes(const mm_mosaic* plan);
 void mm_mosaic_destroy(mm_mosaic* plan);

@@ -28,6 +28,12 @@ MM_MOSAIC_INFO_RADON_PLANS = 0
 MM_MOSAIC_INFO_FRESNEL_PLANS = 1
 MM_MOSAIC_INFO_CAPTURES = 2
 MM_MOSAIC_INFO_TILES = 3
+MM_MOSAIC_INFO_BRANCHES = 4
+MM_MOSAIC_TILE_BRANCH = 100
+
+
+def MM_MOSAIC_BRANCHES(b: int) -> int:
+    return int(b) << 8
 MM_RADON_NSTAGES = 8
 RADON_STAGES = ("r_rows_rfft", "r_col_fft", "r_gather", "r_line_ifft",
                 "rt_line_fft", "rt_scatter", "rt_col_ifft", "rt_rows_c2r")

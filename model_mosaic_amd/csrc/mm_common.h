@@ -81,6 +81,9 @@ void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max);
 // and optionally without the adjoint tables (radon.hip).
 int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags,
                       float2* fde_ext, float2* gh_ext, bool fwd_only);
+// R on caller-given half-spectrum and representative buffers (at least radon_fde_elems / radon_gh_elems of
+// the call's nz) instead of the plan's: plans shared by concurrent graph branches, one workspace each.
+int radon_fwd_ws(mm_radon* p, const float* obj, float* sino, int nz, hipStream_t st, float2* fde, float2* gh);
 size_t radon_fde_elems(int n, int nz_max);
 size_t radon_gh_elems(int n, int nz_max, int ntheta);
 void fresnel_dims(const mm_fresnel* p, int* n, int* nz, int* ndist);

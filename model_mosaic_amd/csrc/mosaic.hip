@@ -13,6 +13,15 @@
 // and replayed as a single hipGraphLaunch.  With mirror padding psi is built
 // inside the Fresnel column pass and |.|^2 taken inside its row pass
 // (fresnel_intensity), so a tile is R plus two Fresnel passes.
+//
+// The tiles are split over B concurrent branches of the graph (default 2):
+// each branch has its own half-spectrum, representative, sinogram and psi
+// workspaces and its own stream, forked from and joined back to the batch
+// stream, and runs its tiles one after another; the plans are shared (R runs
+// on the branch's workspace through radon_fwd_ws).  Tiles go to branches by
+// longest-first greedy balancing of their HBM bytes, so one tile's launch
+// tails overlap another's work.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -24,8 +33,17 @@ using namespace mm;
 
 struct mm_mosaic {
   struct Tile {
-    int n = 0, nz = 0, ntheta = 0, rplan = -1, fplan = -1;
+    int n = 0, nz = 0, ntheta = 0, rplan = -1, fplan = -1, branch = 0;
     long long count = 0;  // ntheta * nz * n
+  };
+  struct Branch {
+    float2* fde = nullptr;
+    float2* gh = nullptr;
+    float* p = nullptr;
+    float2* psi = nullptr;
+    hipStream_t st = nullptr;  // branch 0 runs on the batch stream itself
+    hipEvent_t ev_join = nullptr;
+    std::vector<int> tiles;
   };
   int dev = 0;
   unsigned flags = 0;
@@ -34,12 +52,9 @@ struct mm_mosaic {
   std::vector<mm_radon*> rplans;
   std::vector<int> rplan_first;  // first tile of each Radon plan (autotune input)
   std::vector<mm_fresnel*> fplans;
-  float2* fde = nullptr;
-  float2* gh = nullptr;
-  float* p = nullptr;
-  float2* psi = nullptr;
+  std::vector<Branch> branches;
   hipStream_t st = nullptr;  // capturable stream of the batch (the caller's may be the legacy default stream)
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> cap_ptrs;  // inputs and outputs the graph was captured with
@@ -54,29 +69,45 @@ static void mosaic_free(mm_mosaic* m) {
   if (m->graph) (void)hipGraphDestroy(m->graph);
   for (mm_radon* r : m->rplans) mm_radon_destroy(r);
   for (mm_fresnel* f : m->fplans) mm_fresnel_destroy(f);
-  void* ptrs[] = {m->fde, m->gh, m->p, m->psi};
-  for (void* q : ptrs)
-    if (q) (void)hipFree(q);
+  for (mm_mosaic::Branch& b : m->branches) {
+    void* ptrs[] = {b.fde, b.gh, b.p, b.psi};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    if (b.st) (void)hipStreamDestroy(b.st);
+    if (b.ev_join) (void)hipEventDestroy(b.ev_join);
+  }
   if (m->ev_in) (void)hipEventDestroy(m->ev_in);
   if (m->ev_out) (void)hipEventDestroy(m->ev_out);
+  if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
 }
 
-// The chain of every tile, in tile order, on stream st.
+// The chain of every tile: forked from st onto the branches, each branch's tiles in tile order on its own
+// stream and workspaces, joined back to st.
 static int mosaic_enqueue(mm_mosaic* m, const float* const* delta, float* const* out, hipStream_t st) {
   int rc;
-  for (size_t t = 0; t < m->tiles.size(); ++t) {
-    const mm_mosaic::Tile& T = m->tiles[t];
-    if ((rc = mm_radon_fwd(m->rplans[T.rplan], delta[t], m->p, T.nz, st))) return rc;
-    mm_fresnel* fp = m->fplans[T.fplan];
-    if (fresnel_fused_ok(fp)) {  // psi built in D's column pass, |.|^2 taken in its row pass
-      if ((rc = fresnel_intensity(fp, m->p, m->k, m->beta_ratio, m->psi, out[t], T.ntheta, 0, st))) return rc;
-    } else if ((rc = mm_psi_from_proj(m->p, m->psi, T.count, m->k, m->beta_ratio, st)) ||
-               (rc = mm_fresnel_fwd(fp, m->psi, m->psi, T.ntheta, 0, st)) ||
-               (rc = mm_abs2(m->psi, out[t], T.count, st))) {
-      return rc;
+  MM_HIP_TRY(hipEventRecord(m->ev_fork, st));
+  for (size_t b = 1; b < m->branches.size(); ++b) MM_HIP_TRY(hipStreamWaitEvent(m->branches[b].st, m->ev_fork, 0));
+  for (size_t b = 0; b < m->branches.size(); ++b) {
+    const mm_mosaic::Branch& B = m->branches[b];
+    hipStream_t bs = b == 0 ? st : B.st;
+    for (int t : B.tiles) {
+      const mm_mosaic::Tile& T = m->tiles[t];
+      if ((rc = radon_fwd_ws(m->rplans[T.rplan], delta[t], B.p, T.nz, bs, B.fde, B.gh))) return rc;
+      mm_fresnel* fp = m->fplans[T.fplan];
+      if (fresnel_fused_ok(fp)) {  // psi built in D's column pass, |.|^2 taken in its row pass
+        if ((rc = fresnel_intensity(fp, B.p, m->k, m->beta_ratio, B.psi, out[t], T.ntheta, 0, bs))) return rc;
+      } else if ((rc = mm_psi_from_proj(B.p, B.psi, T.count, m->k, m->beta_ratio, bs)) ||
+                 (rc = mm_fresnel_fwd(fp, B.psi, B.psi, T.ntheta, 0, bs)) ||
+                 (rc = mm_abs2(B.psi, out[t], T.count, bs))) {
+        return rc;
+      }
     }
+  }
+  for (size_t b = 1; b < m->branches.size(); ++b) {
+    MM_HIP_TRY(hipEventRecord(m->branches[b].ev_join, m->branches[b].st));
+    MM_HIP_TRY(hipStreamWaitEvent(st, m->branches[b].ev_join, 0));
   }
   return MM_OK;
 }
@@ -89,7 +120,10 @@ int mm_mosaic_create(mm_mosaic** out, int ntiles, const int* n, const int* nz, c
   MM_ARG_CHECK(out != nullptr, "mm_mosaic_create: plan is NULL");
   *out = nullptr;
   MM_ARG_CHECK(ntiles >= 1 && n && nz && ntheta && theta_all, "mm_mosaic_create: need ntiles >= 1 and all arrays");
-  MM_ARG_CHECK((flags & ~(MM_MOSAIC_NO_GRAPH | MM_MOSAIC_AUTOTUNE)) == 0, "mm_mosaic_create: unknown flags");
+  MM_ARG_CHECK((flags & ~(MM_MOSAIC_NO_GRAPH | MM_MOSAIC_AUTOTUNE | MM_MOSAIC_BRANCHES(15))) == 0,
+               "mm_mosaic_create: unknown flags");
+  const int nbranch_req = (int)((flags >> 8) & 15u);
+  MM_ARG_CHECK(nbranch_req <= 8, "mm_mosaic_create: at most 8 branches");
   MM_ARG_CHECK(pad_mode == 0 || pad_mode == 1, "mm_mosaic_create: pad_mode must be 0 or 1");
   mm_mosaic* m = new mm_mosaic();
   auto bail = [&](int code) {
@@ -106,8 +140,7 @@ int mm_mosaic_create(mm_mosaic** out, int ntiles, const int* n, const int* nz, c
   std::vector<int> rnz;                     // nz_max of each Radon plan
   std::vector<std::pair<int, const double*>> rgeo;  // (n, theta) of each Radon plan
   std::vector<int> rnt;
-  long long th_off = 0, max_count = 0;
-  size_t max_fde = 0, max_gh = 0;
+  long long th_off = 0;
   for (int t = 0; t < ntiles; ++t) {
     MM_ARG_CHECK(n[t] >= 16 && nz[t] >= 1 && ntheta[t] >= 1, "mm_mosaic_create: bad tile " + std::to_string(t));
     mm_mosaic::Tile T;
@@ -132,12 +165,26 @@ int mm_mosaic_create(mm_mosaic** out, int ntiles, const int* n, const int* nz, c
     auto jt = fkey.find(fk);
     if (jt == fkey.end()) jt = fkey.emplace(fk, (int)fkey.size()).first;
     T.fplan = jt->second;
-    max_count = std::max(max_count, T.count);
     m->tiles.push_back(T);
   }
-  for (size_t r = 0; r < rgeo.size(); ++r) {
-    max_fde = std::max(max_fde, radon_fde_elems(rgeo[r].first, rnz[r]));
-    max_gh = std::max(max_gh, radon_gh_elems(rgeo[r].first, rnz[r], rnt[r]));
+  // branches: longest-first greedy balance of the tiles' HBM bytes (object + 32 B per projection pixel)
+  const int nbr = std::min(nbranch_req ? nbranch_req : 2, ntiles);
+  m->branches.resize(nbr);
+  {
+    std::vector<int> order(ntiles);
+    for (int t = 0; t < ntiles; ++t) order[t] = t;
+    auto cost = [&](int t) {
+      const mm_mosaic::Tile& T = m->tiles[t];
+      return 4.0 * T.nz * (double)T.n * T.n + 32.0 * (double)T.count;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<double> load(nbr, 0.0);
+    for (int t : order) {
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[b] += cost(t);
+      m->tiles[t].branch = b;
+    }
+    for (int t = 0; t < ntiles; ++t) m->branches[m->tiles[t].branch].tiles.push_back(t);
   }
   auto alloc = [&](void** q, size_t nb, const char* what) {
     if (hipMalloc(q, nb) != hipSuccess) {
@@ -148,14 +195,27 @@ int mm_mosaic_create(mm_mosaic** out, int ntiles, const int* n, const int* nz, c
     return MM_OK;
   };
   int rc;
-  if ((rc = alloc((void**)&m->fde, sizeof(float2) * max_fde, "half-spectrum workspace")) ||
-      (rc = alloc((void**)&m->gh, sizeof(float2) * max_gh, "representative workspace")) ||
-      (rc = alloc((void**)&m->p, sizeof(float) * (size_t)max_count, "sinogram workspace")) ||
-      (rc = alloc((void**)&m->psi, sizeof(float2) * (size_t)max_count, "psi workspace")))
-    return bail(rc);
+  for (mm_mosaic::Branch& B : m->branches) {  // each branch's workspaces fit the largest of its tiles
+    size_t bf = 0, bg = 0;
+    long long bc = 0;
+    for (int t : B.tiles) {
+      const mm_mosaic::Tile& T = m->tiles[t];
+      bf = std::max(bf, radon_fde_elems(T.n, T.nz));
+      bg = std::max(bg, radon_gh_elems(T.n, T.nz, T.ntheta));
+      bc = std::max(bc, T.count);
+    }
+    if ((rc = alloc((void**)&B.fde, sizeof(float2) * bf, "half-spectrum workspace")) ||
+        (rc = alloc((void**)&B.gh, sizeof(float2) * bg, "representative workspace")) ||
+        (rc = alloc((void**)&B.p, sizeof(float) * (size_t)bc, "sinogram workspace")) ||
+        (rc = alloc((void**)&B.psi, sizeof(float2) * (size_t)bc, "psi workspace")))
+      return bail(rc);
+  }
+  // the plans' own half-spectrum / representative pointers are branch 0's (R always runs through
+  // radon_fwd_ws with the branch's workspaces; the autotuner runs on branch 0)
   for (size_t r = 0; r < rgeo.size(); ++r) {
     mm_radon* rp = nullptr;
-    if ((rc = radon_create_impl(&rp, rgeo[r].first, rnz[r], rnt[r], rgeo[r].second, 0, m->fde, m->gh, true)))
+    if ((rc = radon_create_impl(&rp, rgeo[r].first, rnz[r], rnt[r], rgeo[r].second, 0, m->branches[0].fde,
+                                m->branches[0].gh, true)))
       return bail(rc);
     m->rplans.push_back(rp);
     m->bytes += mm_radon_workspace_bytes(rp);
@@ -170,8 +230,13 @@ int mm_mosaic_create(mm_mosaic** out, int ntiles, const int* n, const int* nz, c
   }
   if (hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess)
     return bail(fail(MM_ERR_HIP, "mm_mosaic_create: stream / events"));
+  for (size_t b = 1; b < m->branches.size(); ++b)  // default-priority streams, like the batch stream
+    if (hipStreamCreateWithFlags(&m->branches[b].st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->branches[b].ev_join, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(MM_ERR_HIP, "mm_mosaic_create: branch streams / events"));
   *out = m;
   return MM_OK;
 }
@@ -188,7 +253,8 @@ int mm_mosaic_run(mm_mosaic* m, const float* const* delta, float* const* out, vo
   if ((m->flags & MM_MOSAIC_AUTOTUNE) && !m->tuned) {
     for (size_t r = 0; r < m->rplans.size(); ++r) {
       const int t = m->rplan_first[r];
-      if ((rc = mm_radon_autotune(m->rplans[r], delta[t], m->p, nullptr, m->tiles[t].nz, m->st))) return rc;
+      if ((rc = mm_radon_autotune(m->rplans[r], delta[t], m->branches[0].p, nullptr, m->tiles[t].nz, m->st)))
+        return rc;
     }
     m->tuned = true;
   }
@@ -230,12 +296,14 @@ int mm_mosaic_info(const mm_mosaic* m, int what) {
     case MM_MOSAIC_INFO_FRESNEL_PLANS: return (int)m->fplans.size();
     case MM_MOSAIC_INFO_CAPTURES: return m->captures;
     case MM_MOSAIC_INFO_TILES: return (int)m->tiles.size();
+    case MM_MOSAIC_INFO_BRANCHES: return (int)m->branches.size();
     default: return MM_ERR_ARG;
   }
 }
 
 int mm_mosaic_tile_knob(const mm_mosaic* m, int tile, int knob) {
   if (!m || tile < 0 || tile >= (int)m->tiles.size()) return MM_ERR_ARG;
+  if (knob == MM_MOSAIC_TILE_BRANCH) return m->tiles[tile].branch;
   return mm_radon_get_tune(m->rplans[m->tiles[tile].rplan], knob);
 }
 

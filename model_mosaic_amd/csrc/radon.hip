@@ -23,19 +23,22 @@
 //    returns Re(IFFT2(full grid)), which is what the reference keeps.
 //    A complex sinogram is split into the two Hermitian channels of its real
 //    and imaginary parts (one complex row FFT gives both).
-//  * Gather and scatter are tiled over the Fourier plane: samples are binned
-//    (host, FP64, at plan time) into quadtree tiles of at most 1024 samples.
-//    A gather workgroup stages its tile + 4-point halo of the half spectrum in
-//    LDS (applying the conj reflection and the (-1)^(k0+k1) centring sign while
-//    loading) and each thread sums its 81 taps from LDS.  A scatter workgroup
-//    owns a tile of the output half plane, accumulates the taps of every
-//    sample that touches it with LDS atomics (FP32, FP64 with
-//    MM_RADON_FP64_TILES, or compensated FP32 with MM_RADON_KAHAN_TILES) and writes the tile once; only tiles at the 8 x 8
-//    quadtree floor with more than 1024 samples (around the origin, where
-//    every angle passes) are split into partial tiles that a second kernel
-//    sums.  Per-sample weights stay in registers across the z loop.
-//  * FFTs are the LDS Stockham transforms of mm_fft.h: row transforms over
-//    contiguous rows, column transforms over C = 16384/L adjacent columns.
+//  * The gather is tiled over the Fourier plane: samples are binned (host,
+//    FP64, at plan time) into quadtree tiles of at most 1024 samples.  A
+//    gather workgroup stages its tile + 4-point halo of the half spectrum in
+//    LDS (applying the conj reflection while loading) and each lane sums its
+//    81 taps from LDS; per-sample weights stay in registers across the z loop.
+//  * The scatter is owner-computes, without atomics: a workgroup owns a 16 x 16
+//    tile of the output half plane, each lane one point of it, and walks the
+//    point's plan-time list of (entry, tap) slots in registers, reading the
+//    entries' values from LDS; accumulation is FP32 (FP64 with
+//    MM_RADON_FP64_TILES, compensated FP32 with MM_RADON_KAHAN_TILES) in a
+//    fixed order, so RT is deterministic.  Tiles whose lists exceed one work
+//    item (near the origin, where every angle passes) are split into partial
+//    tiles that scatter_reduce sums.
+//  * FFTs are the register-resident Stockham transforms of mm_fft.h (up to
+//    16384 points per workgroup); the 2n-point passes of n > 8192 run as two
+//    n-point halves (r1_rows_split, col_fft_split, rt4_rows_c2r_split).
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -2372,6 +2375,21 @@ static int scratch_base(int dev, unsigned char** out) {
   *out = scratch_dev[dev];
   return MM_OK;
 }
+
+namespace mm {
+int radon_fwd_ws(mm_radon* p, const float* obj, float* sino, int nz, hipStream_t st, float2* fde, float2* gh) {
+  MM_ARG_CHECK(p != nullptr && obj && sino && fde && gh, "radon_fwd_ws: null pointer");
+  MM_ARG_CHECK(nz >= 1 && nz <= p->nz_max, "radon_fwd_ws: nz outside [1, nz_max]");
+  float2* const own_fde = p->fde;
+  float2* const own_gh = p->gh;
+  p->fde = fde;
+  p->gh = gh;
+  const int rc = radon_fwd_chain(p, obj, sino, nz, st);
+  p->fde = own_fde;
+  p->gh = own_gh;
+  return rc;
+}
+}  // namespace mm
 
 extern "C" {
 

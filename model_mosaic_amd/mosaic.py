@@ -6,7 +6,8 @@ D and |.|^2 per angle chunk (step3_propagation.py:175,223, :94-110).  A mosaic
 scan has tiles of different detector widths, angle counts and z stacks
 (README.md:28-34: 4 x-tiles x 9 z-stacks).  MosaicBatch enqueues every tile's
 chain as one graph (mm_mosaic_* in libmosaic_hip.so): tiles of equal geometry
-share plans, all tiles share the workspaces.
+share plans; the tiles are split over ``branches`` concurrent graph branches,
+each with its own workspaces and stream.
 """
 from __future__ import annotations
 
@@ -25,11 +26,14 @@ class MosaicBatch:
     ``k`` and ``beta_ratio`` follow step3_propagation.py:94-103 (``inv_norm``
     and ``1 / beta_r``): psi = exp(-k beta_ratio p) exp(i k p) for p = R delta.
     ``graph=False`` enqueues the launches directly; ``autotune=True`` picks
-    each Radon plan's gather tile order on its first tile at the first run.
+    each Radon plan's gather tile order (and, for 4096-point columns, the
+    column split) on its first tile at the first run; ``branches`` (1 - 8,
+    default 2) is the number of concurrent graph branches the tiles are
+    balanced over (results do not depend on it).
     """
 
     def __init__(self, tiles, wavelength, voxelsize, distance, k, beta_ratio, pad: str = "symmetric",
-                 graph: bool = True, autotune: bool = False):
+                 graph: bool = True, autotune: bool = False, branches: int = 2):
         require_gpu()
         self._lib = _lib.load()
         if pad not in ("symmetric", "zero"):
@@ -42,7 +46,10 @@ class MosaicBatch:
         nzs = (C.c_int * nt)(*[t[1] for t in self.tiles])
         nths = (C.c_int * nt)(*[len(t[2]) for t in self.tiles])
         th = np.ascontiguousarray(np.concatenate([t[2] for t in self.tiles]).astype(np.float64))
-        flags = (0 if graph else _lib.MM_MOSAIC_NO_GRAPH) | (_lib.MM_MOSAIC_AUTOTUNE if autotune else 0)
+        if not 1 <= int(branches) <= 8:
+            raise ValueError("MosaicBatch: branches must be in [1, 8]")
+        flags = ((0 if graph else _lib.MM_MOSAIC_NO_GRAPH) | (_lib.MM_MOSAIC_AUTOTUNE if autotune else 0)
+                 | _lib.MM_MOSAIC_BRANCHES(int(branches)))
         plan = C.c_void_p()
         _lib.check(self._lib.mm_mosaic_create(C.byref(plan), nt, ns, nzs, nths, th.ctypes.data_as(C.POINTER(C.c_double)),
                                               float(wavelength), float(voxelsize), float(distance), float(k),
@@ -56,6 +63,9 @@ class MosaicBatch:
                 "fresnel_plans": f(self._plan, _lib.MM_MOSAIC_INFO_FRESNEL_PLANS),
                 "graph_captures": f(self._plan, _lib.MM_MOSAIC_INFO_CAPTURES),
                 "tiles": f(self._plan, _lib.MM_MOSAIC_INFO_TILES),
+                "branches": f(self._plan, _lib.MM_MOSAIC_INFO_BRANCHES),
+                "tile_branch": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_MOSAIC_TILE_BRANCH)
+                                for t in range(len(self.tiles))],
                 "gather_xcd_group": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_GATHER_XCD_GROUP)
                                      for t in range(len(self.tiles))],
                 "col_split": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_COL_SPLIT)
@@ -75,8 +85,10 @@ class MosaicBatch:
         for t, ((n, nz, th), d, o) in enumerate(zip(self.tiles, deltas, outs)):
             if d.dtype != torch.float32 or tuple(d.shape) != (nz, n, n) or not d.is_contiguous() or d.device != self.device:
                 raise ValueError(f"MosaicBatch.run: tile {t} input must be contiguous float32 ({nz}, {n}, {n}) on {self.device}")
-            if o.dtype != torch.float32 or tuple(o.shape) != (len(th), nz, n) or not o.is_contiguous():
-                raise ValueError(f"MosaicBatch.run: tile {t} output must be contiguous float32 ({len(th)}, {nz}, {n})")
+            if (o.dtype != torch.float32 or tuple(o.shape) != (len(th), nz, n) or not o.is_contiguous()
+                    or o.device != self.device):
+                raise ValueError(f"MosaicBatch.run: tile {t} output must be contiguous float32 ({len(th)}, {nz}, {n}) "
+                                 f"on {self.device}")
         nt = len(self.tiles)
         din = (C.c_void_p * nt)(*[d.data_ptr() for d in deltas])
         dout = (C.c_void_p * nt)(*[o.data_ptr() for o in outs])

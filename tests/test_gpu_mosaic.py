@@ -84,6 +84,27 @@ def test_no_graph_and_autotune_same_results():
     assert all(g in (1, 2, 4, 8, 16) for g in mb.info()["gather_xcd_group"])
 
 
+def test_branches_same_results():
+    """The tiles split over 1, 2 or 3 concurrent graph branches (each with its own workspaces): same bits, and
+    every branch gets tiles."""
+    from model_mosaic_amd.mosaic import MosaicBatch
+    tiles, deltas = tiles_and_inputs()
+    k, beta = k_for(deltas, tiles), 1e-3
+    ref = MosaicBatch(tiles, WL, VOX, Z, k, beta, branches=1).run(deltas)
+    for nb in (2, 3):
+        mb = MosaicBatch(tiles, WL, VOX, Z, k, beta, branches=nb)
+        for rep in range(2):
+            out = mb.run(deltas)
+            torch.cuda.synchronize()
+            for t, (o, r) in enumerate(zip(out, ref)):
+                assert torch.equal(o, r), f"branches {nb} replay {rep} tile {t}"
+        info = mb.info()
+        assert info["branches"] == nb and sorted(set(info["tile_branch"])) == list(range(nb))
+        mb.free()
+    with pytest.raises(ValueError):
+        MosaicBatch(tiles, WL, VOX, Z, k, beta, branches=9)
+
+
 def test_tile_vs_fp64_oracle():
     """Tile 2 (n=192, 8 rows, 33 angles): intensity against R, psi, D, |.|^2 of the FP64 oracle."""
     from model_mosaic_amd.mosaic import MosaicBatch
