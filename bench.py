@@ -164,12 +164,37 @@ def cpu_baseline():
         with cf.ProcessPoolExecutor(workers, mp_context=ctx) as pool:
             timed(pool, 1)
             tw, _ = timed(pool, 5)
+        nt_small = len(_CPU_JOB["theta"])
+        tr, tt = [], []
+        for _ in range(5):  # R and RT apart, one thread
+            t0 = time.perf_counter()
+            s1 = _cpu_R_block((0, nt_small))
+            t_mid = time.perf_counter()
+            _cpu_RT_block(((0, nt_small), s1))
+            tr.append(t_mid - t0)
+            tt.append(time.perf_counter() - t_mid)
         r64 = usfft.tomo_R(_CPU_JOB["x"], _CPU_JOB["theta"], np.float64)
         err = float(np.linalg.norm(sino - r64) / np.linalg.norm(r64))
         res["small"] = {"workload": "nz=8 x 256^2, 384 angles, R+RT", "median_of": 5,
                         "one_thread_s": round(t1, 3), "one_thread_slices_per_s": round(8 / t1, 3),
+                        "one_thread_R_slices_per_s": round(8 / statistics.median(tr), 3),
+                        "one_thread_RT_slices_per_s": round(8 / statistics.median(tt), 3),
                         "all_cores_s": round(tw, 3), "all_cores_slices_per_s": round(8 / tw, 3),
                         "R_rel_l2_fp32_vs_fp64": err}
+        # Fresnel D of one 2048 x 2048 frame (FP32 restatement: pad, FFT2, chirp, IFFT2, crop), one thread
+        from oracle import fresnel as fr
+        frame = (np.random.default_rng(3).standard_normal((1, 2048, 2048)) * 1j).astype(np.complex64)
+        fk = fr.fresnel_kernel_2d(2048, 2048, 1.24e-9 / 30.0, 0.69e-6, [0.2], np.float32)[0]
+        fr.D(frame, fk, np.float32)
+        tf = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fr.D(frame, fk, np.float32)
+            tf.append(time.perf_counter() - t0)
+        res["fresnel_frame"] = {"workload": "D of one 2048 x 2048 complex64 frame, z = 0.2 m", "median_of": 3,
+                                "one_thread_s": round(statistics.median(tf), 3),
+                                "one_thread_frames_per_s": round(1.0 / statistics.median(tf), 3)}
+        del frame, fk
         # one n=2048 slice at all 2048 angles
         _CPU_JOB.update(x=phantom.ellipsoids(64, 2048, seed=0, noise=0.01, rows=(32, 33)).numpy(), theta=phantom.angles(2048))
         with cf.ProcessPoolExecutor(workers, mp_context=ctx) as pool:
