@@ -15,6 +15,7 @@
 // pi lambda z f^2 is formed and reduced mod 2 pi in FP64, then the FP32 sincos
 // of the reduced phase is taken and 1/(2n) resp. 1/(2nz) is folded in.
 #include <cmath>
+#include <complex>
 #include <type_traits>
 #include <vector>
 
@@ -206,9 +207,15 @@ using DctRowShape = RegShape<L, MM_FRESNEL_DCT_ROW_BATCH, 0, false>;
 // x_{2j+1}):  with V = FFT(v) and c_k = exp(-i pi k / 2N),
 //   C_k = c_k V_k + conj(c_k) V_{N-k},   C_{N-k} = i (c_k V_k - conj(c_k) V_{N-k}),
 //   U_k = conj(c_k) (K_k C_k - i K_{N-k} C_{N-k}),   x_{perm(j)} = (IFFT U)_j.
-// The partner V_{N-k} of thread t's points sits in thread (TPC - t) mod TPC:
-// one LDS exchange between the two transforms.  The j < N/2 half of the
-// reordering is the registers m < PT/2, so it is fixed at compile time.
+// Multiplied out (|c_k| = 1):  U_k = A_k V_k + B_k V_{N-k}  with
+//   A_k = K_k + K_{N-k},   B_k = conj(c_k)^2 (K_k - K_{N-k})
+// (k = 0: V_{N-0} = V_0, K_{N-0} = K_N, U_0 = 2 K_0 V_0); with conj(K) for DT:
+// A -> conj(A), B -> conj(c_k)^2 conj(K_k - K_{N-k}).  The plan holds A, B
+// and the DT B per distance, built in FP64: two complex products per point
+// between the transforms.  The partner V_{N-k} of thread t's points sits in
+// thread (TPC - t) mod TPC: one LDS exchange between the two transforms.  The
+// j < N/2 half of the reordering is the registers m < PT/2, so it is fixed at
+// compile time.
 template <class Sh>
 __device__ __forceinline__ int dct_src(int m, int t) {
   constexpr int N = Sh::PT * Sh::TPC;
@@ -216,17 +223,11 @@ __device__ __forceinline__ int dct_src(int m, int t) {
   return m < Sh::PT / 2 ? 2 * j : 2 * N - 1 - 2 * j;
 }
 
-// c_k = exp(-i pi k / 2N) from the first N/2 values of the length-2N table (tw2: the bank's, or the plan's own
-// for N > kMaxFFT / 2); hw = exp(-i pi / 2N).
-__device__ __forceinline__ float2 dct_twiddle(int k, const float2* __restrict__ tw2, float2 hw) {
-  const float2 w = tw2[k >> 1];
-  return (k & 1) ? cmul(w, hw) : w;
-}
-
 // Between the forward and the inverse transform: v holds V_k (k = t + m TPC) on entry and U_k on exit.
+// M = [A | B | B of DT] of one distance (3 N values).
 template <class Sh, bool CONJ_K>
 __device__ __forceinline__ void dct_mid(float2 (&v)[Sh::PT], float2* __restrict__ x, int t,
-                                        const float2* __restrict__ K, const float2* __restrict__ tw2, float2 hw) {
+                                        const float2* __restrict__ M) {
   constexpr int N = Sh::PT * Sh::TPC;
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) x[lpad(t + m * Sh::TPC)] = v[m];
@@ -234,20 +235,12 @@ __device__ __forceinline__ void dct_mid(float2 (&v)[Sh::PT], float2* __restrict_
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) {
     const int k = t + m * Sh::TPC;
-    const int kr = k == 0 ? 0 : N - k;
-    const float2 vr = x[lpad(kr)];
-    const float2 c = dct_twiddle(k, tw2, hw);
-    const float2 a = cmul(c, v[m]), b = cmul(cconj(c), vr);
-    const float2 ck = cadd(a, b);
-    const float2 d = csub(a, b);
-    const float2 cr = make_float2(-d.y, d.x);  // C_{N-k} = i (a - b)
-    float2 kk = K[k], kn = K[N - k];
-    if (CONJ_K) {
-      kk.y = -kk.y;
-      kn.y = -kn.y;
-    }
-    const float2 y = cmul(kk, ck), yr = cmul(kn, cr);
-    v[m] = cmul(cconj(c), make_float2(y.x + yr.y, y.y - yr.x));  // conj(c) (Y_k - i Y_{N-k})
+    const float2 vr = x[lpad(k == 0 ? 0 : N - k)];
+    float2 a = M[k];
+    if (CONJ_K) a.y = -a.y;
+    const float2 b = M[(CONJ_K ? 2 * N : N) + k];
+    const float2 p = cmul(a, v[m]), q = cmul(b, vr);
+    v[m] = make_float2(p.x + q.x, p.y + q.y);
   }
   __syncthreads();  // the inverse transform's first pass rewrites x
 }
@@ -343,8 +336,8 @@ __device__ __forceinline__ void block_sum(double acc, float2* __restrict__ x, do
 // Column pass on the DCT form: frames (b, nz, n), transforms of length L = nz down the columns.
 template <int L, bool CONJ_K, class Load, class Store>
 __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctColShape<L>::NT) void dct_col_pass(
-    Load ld, Store st, int n, const float2* __restrict__ K, const float2* __restrict__ tw,
-    const float2* __restrict__ tw2, float2 hw, double* __restrict__ partial) {
+    Load ld, Store st, int n, const float2* __restrict__ M, const float2* __restrict__ tw,
+    double* __restrict__ partial) {
   using Sh = DctColShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
@@ -359,7 +352,7 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctColShape<L>::NT) void dct_col_
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) v[m] = live ? ld(fo + (long long)dct_src<Sh>(m, t) * n) : make_float2(0.f, 0.f);
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
-  dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
+  dct_mid<Sh, CONJ_K>(v, x, t, M);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   double acc = 0.0;
   if (live) {
@@ -372,8 +365,8 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctColShape<L>::NT) void dct_col_
 // Row pass on the DCT form: rows of length L = n, C rows per block, nrows = b * nz.
 template <int L, bool CONJ_K, class Load, class Store>
 __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctRowShape<L>::NT) void dct_row_pass(
-    Load ld, Store st, long long nrows, const float2* __restrict__ K, const float2* __restrict__ tw,
-    const float2* __restrict__ tw2, float2 hw, double* __restrict__ partial) {
+    Load ld, Store st, long long nrows, const float2* __restrict__ M, const float2* __restrict__ tw,
+    double* __restrict__ partial) {
   using Sh = DctRowShape<L>;
   __shared__ float2 lds[Sh::LDS];
   int c, t;
@@ -386,7 +379,7 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctRowShape<L>::NT) void dct_row_
 #pragma unroll
   for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? ld(ro + dct_src<Sh>(m, t)) : make_float2(0.f, 0.f);
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
-  dct_mid<Sh, CONJ_K>(v, x, t, K, tw2, hw);
+  dct_mid<Sh, CONJ_K>(v, x, t, M);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   double acc = 0.0;
   if (valid) {
@@ -408,9 +401,8 @@ struct mm_fresnel {
   bool dct = false;      // mirror padding on the DCT form (both frame sides are DCT lengths)
   float2* Kx = nullptr;  // ndist x 2n
   float2* Ky = nullptr;  // ndist x 2nz
-  // exp(-2 pi i j / 2N), j < N/2, for a DCT side N whose 2N-point twiddles are past the bank (N > kMaxFFT / 2)
-  float2* cx2 = nullptr;
-  float2* cy2 = nullptr;
+  float2* Mx = nullptr;  // DCT form: ndist x [A | B | B of DT] x n (see dct_mid)
+  float2* My = nullptr;  // ndist x 3 x nz
   const float2* tw = nullptr;
   size_t bytes = 0;
 };
@@ -419,8 +411,8 @@ static void fresnel_free(mm_fresnel* p) {
   if (!p) return;
   if (p->Kx) (void)hipFree(p->Kx);
   if (p->Ky) (void)hipFree(p->Ky);
-  if (p->cx2) (void)hipFree(p->cx2);
-  if (p->cy2) (void)hipFree(p->cy2);
+  if (p->Mx) (void)hipFree(p->Mx);
+  if (p->My) (void)hipFree(p->My);
   delete p;
 }
 
@@ -440,19 +432,32 @@ static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, 
 }
 
 
-// exp(-i pi / 2N) in FP64, rounded to complex64
-static float2 half_step(int N) {
-  const double a = -M_PI / (2.0 * N);
-  return make_float2((float)std::cos(a), (float)std::sin(a));
-}
-
-// The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48); c_k comes from the bank's length-2N
-// table, or from the plan's own half table when 2N is past the bank.
+// The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48).
 static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && N <= kMaxFFT; }
-template <int L>
-static const float2* dct_tw2(const mm_fresnel* p, bool col) {
-  if constexpr (2 * L <= kMaxFFT) return p->tw + tw_offset(2 * L);
-  else return col ? p->cy2 : p->cx2;
+
+// [A | B | B of DT] (dct_mid) of a DCT side of N points for every distance, in FP64 from the chirp of the
+// 2N-point padded transform (the same phase as chirp_table, K carrying 1/(2N)).
+static std::vector<float2> dct_mid_tables(int N, const double* dist, int ndist, double wavelength, double voxelsize) {
+  const int L = 2 * N;
+  std::vector<float2> out((size_t)ndist * 3 * N);
+  std::vector<std::complex<double>> K(N + 1);
+  for (int j = 0; j < ndist; ++j) {
+    for (int k = 0; k <= N; ++k) {
+      const double f = (double)(k < L / 2 ? k : k - L) / ((double)L * voxelsize);
+      const double ph = std::fmod(M_PI * wavelength * dist[j] * f * f, 2.0 * M_PI);
+      K[k] = std::polar(1.0 / L, -ph);
+    }
+    float2* o = out.data() + (size_t)j * 3 * N;
+    for (int k = 0; k < N; ++k) {
+      const std::complex<double> kk = K[k], kr = K[N - k];  // k = 0: K_N
+      const std::complex<double> c2 = std::polar(1.0, M_PI * k / N);  // conj(c_k)^2
+      const std::complex<double> a = kk + kr, b = c2 * (kk - kr), bt = c2 * std::conj(kk - kr);
+      o[k] = make_float2((float)a.real(), (float)a.imag());
+      o[N + k] = make_float2((float)b.real(), (float)b.imag());
+      o[2 * N + k] = make_float2((float)bt.real(), (float)bt.imag());
+    }
+  }
+  return out;
 }
 
 template <int L, bool CONJ_K, class Load, class Store>
@@ -460,8 +465,7 @@ static void launch_dct_col(const mm_fresnel* p, Load ld, Store sto, int nframes,
                            hipStream_t st) {
   using Sh = DctColShape<L>;
   hipLaunchKernelGGL((dct_col_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((p->n + Sh::C - 1) / Sh::C), (unsigned)nframes),
-                     dim3(Sh::NT), 0, st, ld, sto, p->n, p->Ky + (size_t)j * 2 * L, p->tw + tw_offset(L),
-                     dct_tw2<L>(p, true), half_step(L), partial);
+                     dim3(Sh::NT), 0, st, ld, sto, p->n, p->My + (size_t)j * 3 * L, p->tw + tw_offset(L), partial);
 }
 
 template <int L, bool CONJ_K, class Load, class Store>
@@ -470,8 +474,7 @@ static void launch_dct_row(const mm_fresnel* p, Load ld, Store sto, int nframes,
   using Sh = DctRowShape<L>;
   const long long rows = (long long)nframes * p->nz;
   hipLaunchKernelGGL((dct_row_pass<L, CONJ_K, Load, Store>), dim3((unsigned)((rows + Sh::C - 1) / Sh::C)), dim3(Sh::NT),
-                     0, st, ld, sto, rows, p->Kx + (size_t)j * 2 * L, p->tw + tw_offset(L), dct_tw2<L>(p, false),
-                     half_step(L), partial);
+                     0, st, ld, sto, rows, p->Mx + (size_t)j * 3 * L, p->tw + tw_offset(L), partial);
 }
 
 template <int L>
@@ -628,17 +631,12 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
   p->bytes = sizeof(float2) * (size_t)ndist * 2 * (n + nz);
   for (int axis = 0; axis < 2 && p->dct; ++axis) {
     const int N = axis ? nz : n;
-    if (2 * N <= kMaxFFT) continue;
-    std::vector<float2> h(N / 2);
-    for (int k = 0; k < N / 2; ++k) {
-      const double a = -M_PI * (double)k / N;  // exp(-2 pi i k / 2N)
-      h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    float2** dst = axis ? &p->cy2 : &p->cx2;
+    const std::vector<float2> h = dct_mid_tables(N, dist_host, ndist, wavelength, voxelsize);
+    float2** dst = axis ? &p->My : &p->Mx;
     if (hipMalloc((void**)dst, sizeof(float2) * h.size()) != hipSuccess ||
         hipMemcpy(*dst, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
       (void)hipFree(ddist);
-      return bail(fail(MM_ERR_OOM, "fresnel DCT twiddles"));
+      return bail(fail(MM_ERR_OOM, "fresnel DCT tables"));
     }
     p->bytes += sizeof(float2) * h.size();
   }

@@ -98,21 +98,24 @@ def _dct_axis_matrices(L):
     return c2, c3
 
 
-def _makhoul_line(x, kk):
-    """DCT-III(kk * DCT-II(x)) of one line through one length-L FFT and one inverse FFT with the even /
-    odd reordering v_j = x_2j, v_(L-1-j) = x_(2j+1): the arithmetic of the build's mirror-padded Fresnel
-    passes, restated independently of the oracle."""
+def _makhoul_line(x, k2n, conj=False):
+    """DCT-III(K * DCT-II(x)) of one line through one length-L FFT and one inverse FFT with the even / odd
+    reordering v_j = x_2j, v_(L-1-j) = x_(2j+1) and, between the transforms, U_k = A_k V_k + B_k V_(L-k) with
+    A_k = K_k + K_(L-k), B_k = exp(i pi k / L) (K_k - K_(L-k)) (conj(K) for DT: conj(A) and
+    exp(i pi k / L) conj(K_k - K_(L-k))): the arithmetic of the build's mirror-padded Fresnel passes, restated
+    independently of the oracle.  k2n is the 2L-point chirp carrying 1 / (2L)."""
     L = x.shape[-1]
     k = np.arange(L)
     v = np.concatenate([x[0::2], x[1::2][::-1]])
     V = np.fft.fft(v)
-    c = np.exp(-1j * np.pi * k / (2 * L))
-    a, b = c * V, np.conj(c) * V[(-k) % L]
-    C = a + b
-    C[L - k[1:L // 2]] = (1j * (a - b))[1:L // 2]
-    Y = kk * C
-    U = np.conj(c) * (Y - 1j * Y[(-k) % L])
-    U[0] = Y[0]
+    kk, kr = k2n[:L], k2n[L - k]               # k = 0: K_L
+    a, d = kk + kr, kk - kr
+    c2 = np.exp(1j * np.pi * k / L)            # conj(c_k)^2, c_k = exp(-i pi k / 2L)
+    if conj:
+        a, b = np.conj(a), c2 * np.conj(d)
+    else:
+        b = c2 * d
+    U = a * V + b * V[(-k) % L]
     u = np.fft.ifft(U) * L
     out = np.empty_like(u)
     out[0::2], out[1::2] = u[:L // 2], u[L // 2:][::-1]
@@ -135,10 +138,11 @@ def test_mirror_padded_propagator_is_dct_form():
         matx = np.einsum("xk,k,kl->xl", c3x, kxn, c2x)
         explicit = np.einsum("yl,blm,xm->byx", mats, PSI, matx)
         assert rel(explicit, ref) < 1e-13
-        rows = np.apply_along_axis(_makhoul_line, 2, PSI, kxn)
-        both = np.apply_along_axis(_makhoul_line, 1, rows, kyn)
+        kx2, ky2 = kx[j] / (2 * N), ky[j] / (2 * NZ)
+        rows = np.apply_along_axis(_makhoul_line, 2, PSI, kx2)
+        both = np.apply_along_axis(_makhoul_line, 1, rows, ky2)
         assert rel(both, ref) < 1e-13
         # DT of the mirror-padded propagator is the same pipeline with conj(K)
-        conj_rows = np.apply_along_axis(_makhoul_line, 2, PSI, np.conj(kxn))
-        conj_both = np.apply_along_axis(_makhoul_line, 1, conj_rows, np.conj(kyn))
+        conj_rows = np.apply_along_axis(_makhoul_line, 2, PSI, kx2, True)
+        conj_both = np.apply_along_axis(_makhoul_line, 1, conj_rows, ky2, True)
         assert rel(conj_both, fr.DT(PSI, fker[j], np.float64)) < 1e-13
