@@ -199,6 +199,18 @@ struct RegShape {
 // transforms' twiddle loads in flight and drops to one workgroup per CU.
 #define MM_TWO_FFT_BUDGET __attribute__((amdgpu_waves_per_eu(4)))
 
+// The twiddle table pointer and thread index for a kernel's second transform, opaque to the compiler:
+// otherwise it reuses the first transform's twiddle loads and LDS addresses and keeps them live across
+// both (past the register budget).
+__device__ __forceinline__ const float2* opaque_twiddles(const float2* tw) {
+  asm volatile("" : "+s"(tw));
+  return tw;
+}
+__device__ __forceinline__ int opaque_index(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // Row-pass shape of the register FFT: 8192 points per workgroup, plain mapping.
 template <int L, int XTRA = 0>
 using RowReg = RegShape<L, kRowBatch, XTRA, false>;

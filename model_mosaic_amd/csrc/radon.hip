@@ -1191,17 +1191,6 @@ __global__ __launch_bounds__(C2rShape<L>::NT) void rt4_rows_c2r(const float2* __
 template <int L>  // L = n: one transform per workgroup
 using SplitShape = RegShape<L, L>;
 
-// The twiddle table pointer and thread index for a kernel's second transform, opaque to the compiler:
-// otherwise it reuses the first transform's twiddle loads and LDS addresses and keeps them live across
-// both (past the register budget).
-__device__ __forceinline__ const float2* opaque_twiddles(const float2* tw) {
-  asm volatile("" : "+s"(tw));
-  return tw;
-}
-__device__ __forceinline__ int opaque_index(int t) {
-  asm volatile("" : "+v"(t));
-  return t;
-}
 
 // R's row pass: workgroup 2 u + par forms the even (par = 0) or odd (par = 1)
 // half-spectrum columns of row pair u (rows y0, y0 + 1 of slice z), unpacking
