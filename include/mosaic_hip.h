@@ -48,8 +48,7 @@ int mm_version(void);
  *
  * mm_radon_create: n = 2^k in [16, 16384] or 3 * 2^k in [96, 12288], nz_max >= 1,
  *   (above 8192 the 2n-point row and column passes run as two n-point halves,
- *   split by frequency parity forward and by output parity inverse; the FBP
- *   filter, mm_radon_adj_fbp, stays within n <= 8192),
+ *   split by frequency parity forward and by output parity inverse),
  *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of
@@ -183,7 +182,8 @@ int mm_fbp_filter(float* sino_dev, const float* w_dev, long long rows, int n, vo
  * (i, j in [0, n)); the host passes G = rfft(g) / (2 n) (n + 1 float32, real
  * because g is even), g = h on the lags |l| < n laid out mod 2 n, and A[i] /
  * B[i] (n float32 each) = the sums of h over the lags of the left / right edge
- * padding.  2 n must be 2^k or 3 * 2^k in [16, 16384]. */
+ * padding.  2 n must be 2^k or 3 * 2^k in [16, 32768] (above 16384 the
+ * transforms run as two n-point halves). */
 int mm_fbp_filter_2n(float* sino_dev, const float* G_dev, const float* A_dev, const float* B_dev, long long rows,
                      int n, void* stream);
 

@@ -2481,8 +2481,6 @@ int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float
                "mm_radon_adj_fbp: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
   MM_ARG_CHECK(sino && G && A && B && obj, "mm_radon_adj_fbp: null pointer");
   if (p->fwd_only) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: forward-only plan (no scatter tables)");
-  if (2 * p->n > kMaxFFT)
-    return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: the FBP filter's 2n-point transforms are built for n <= 8192");
   hipStream_t st = (hipStream_t)stream;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;

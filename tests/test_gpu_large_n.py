@@ -146,3 +146,42 @@ def test_fresnel_long_sides_vs_fp64_and_adjoint(nz, n):
         print(f"Fresnel {nz}x{n} z={z}: D rel-L2 vs FP64 {ed:.2e}, DT {et:.2e}, adjoint {cs:.2e}", flush=True)
         assert ed < 1e-5 and et < 1e-5 and cs < 1e-6
     prop.free()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("n,name", [(12288, "shepp"), (16384, "ramp")])
+def test_fbp_split_filter_vs_fp64(n, name):
+    """The length-2n FBP filter above n = 8192 (two n-point halves per row pair; an odd row count leaves the
+    last pair half empty) vs the FP64 oracle's edge-padded 4n filter."""
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter
+
+    from oracle import fbp
+    f = FBPFilter(n)
+    w = f.calc_filter(name)
+    x = np.random.default_rng(n).standard_normal((5, n)).astype(np.float32)
+    d = torch.from_numpy(x).cuda()
+    f.filter(d, w)
+    err = rel(d.cpu().numpy(), fbp.filter(x, w.cpu().numpy().astype(np.float64)))
+    print(f"fbp 2n {name} n={n}: rel-L2 vs FP64 {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+@pytest.mark.timeout(300)
+def test_rt_filtered_above_8192():
+    """RT_filtered at n = 12288 (filter on a copy, then RT's line pass) = filter then RT."""
+    from model_mosaic_amd.processing.fbp_filter import FBPFilter
+    from model_mosaic_amd.processing.tomo import TomoReal
+    n, nt = 12288, 4
+    th = _angles(nt)
+    t = TomoReal(n, 1, th)
+    f = FBPFilter(n)
+    w = f.calc_filter("shepp")
+    y = torch.from_numpy(np.random.default_rng(1).standard_normal((nt, 1, n)).astype(np.float32)).cuda()
+    y0 = y.clone()
+    fused = t.RT_filtered(y, f, w)
+    assert torch.equal(y, y0)
+    two = t.RT(f.filter(y.clone(), w))
+    e = float((fused - two).double().norm() / two.double().norm())
+    t.free()
+    print(f"RT_filtered n={n}: rel-L2 vs filter + RT {e:.2e}", flush=True)
+    assert e < 1e-6
