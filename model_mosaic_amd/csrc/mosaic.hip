@@ -86,30 +86,30 @@ static void mosaic_free(mm_mosaic* m) {
 // The chain of every tile: forked from st onto the branches, each branch's tiles in tile order on its own
 // stream and workspaces, joined back to st.
 static int mosaic_enqueue(mm_mosaic* m, const float* const* delta, float* const* out, hipStream_t st) {
-  int rc;
   MM_HIP_TRY(hipEventRecord(m->ev_fork, st));
   for (size_t b = 1; b < m->branches.size(); ++b) MM_HIP_TRY(hipStreamWaitEvent(m->branches[b].st, m->ev_fork, 0));
-  for (size_t b = 0; b < m->branches.size(); ++b) {
+  int rc = MM_OK;
+  for (size_t b = 0; b < m->branches.size() && rc == MM_OK; ++b) {
     const mm_mosaic::Branch& B = m->branches[b];
     hipStream_t bs = b == 0 ? st : B.st;
-    for (int t : B.tiles) {
+    for (size_t i = 0; i < B.tiles.size() && rc == MM_OK; ++i) {
+      const int t = B.tiles[i];
       const mm_mosaic::Tile& T = m->tiles[t];
-      if ((rc = radon_fwd_ws(m->rplans[T.rplan], delta[t], B.p, T.nz, bs, B.fde, B.gh))) return rc;
+      if ((rc = radon_fwd_ws(m->rplans[T.rplan], delta[t], B.p, T.nz, bs, B.fde, B.gh))) break;
       mm_fresnel* fp = m->fplans[T.fplan];
-      if (fresnel_fused_ok(fp)) {  // psi built in D's column pass, |.|^2 taken in its row pass
-        if ((rc = fresnel_intensity(fp, B.p, m->k, m->beta_ratio, B.psi, out[t], T.ntheta, 0, bs))) return rc;
-      } else if ((rc = mm_psi_from_proj(B.p, B.psi, T.count, m->k, m->beta_ratio, bs)) ||
-                 (rc = mm_fresnel_fwd(fp, B.psi, B.psi, T.ntheta, 0, bs)) ||
-                 (rc = mm_abs2(B.psi, out[t], T.count, bs))) {
-        return rc;
-      }
+      if (fresnel_fused_ok(fp))  // psi built in D's column pass, |.|^2 taken in its row pass
+        rc = fresnel_intensity(fp, B.p, m->k, m->beta_ratio, B.psi, out[t], T.ntheta, 0, bs);
+      else if (!(rc = mm_psi_from_proj(B.p, B.psi, T.count, m->k, m->beta_ratio, bs)) &&
+               !(rc = mm_fresnel_fwd(fp, B.psi, B.psi, T.ntheta, 0, bs)))
+        rc = mm_abs2(B.psi, out[t], T.count, bs);
     }
   }
+  // the branches are joined back even after an error, so that no stream is left forked off a capture
   for (size_t b = 1; b < m->branches.size(); ++b) {
     MM_HIP_TRY(hipEventRecord(m->branches[b].ev_join, m->branches[b].st));
     MM_HIP_TRY(hipStreamWaitEvent(st, m->branches[b].ev_join, 0));
   }
-  return MM_OK;
+  return rc;
 }
 
 extern "C" {
