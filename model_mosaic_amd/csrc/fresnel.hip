@@ -176,12 +176,15 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(cons
 #define MM_FRESNEL_DCT_ROW_BATCH 4096
 #endif
 #ifndef MM_FRESNEL_DCT_COL_BATCH
-#define MM_FRESNEL_DCT_COL_BATCH kRowBatch
+#define MM_FRESNEL_DCT_COL_BATCH kBatchElems
 #endif
-// Short column transforms (fewer than 16 threads each) put consecutive lanes on consecutive columns of one
-// row: a wave's load or store is one contiguous run of 64 columns instead of 8-row pieces of 8 columns.
-// With lanes of one wave on different transforms at stride S in LDS, an odd S keeps every 32-lane 8-byte
-// LDS access on distinct bank pairs.
+// Column transforms of fewer than 16 threads, and of 2048 points and more, put consecutive lanes on
+// consecutive columns of one row: a wave's load or store is one contiguous run of its workgroup's columns
+// (64 of them for short transforms, 8 = 64 bytes for 2048-point ones) instead of 8-byte pieces 16 lanes
+// apart.  With lanes of one wave on different transforms at stride S in LDS, an odd S keeps every 32-lane
+// 8-byte LDS access on distinct bank pairs.  (2048-point columns: 8 per workgroup in this mapping,
+// 6.13 -> 5.62 ms per D of 128 frames of 2048^2; 4 per workgroup in the interleaved one was the best
+// before.)
 template <int L, int BATCH>
 struct ColFastShape : RegShape<L, BATCH> {
   using Base = RegShape<L, BATCH>;
@@ -193,7 +196,8 @@ struct ColFastShape : RegShape<L, BATCH> {
   }
 };
 template <int L, int BATCH = (L >= 2048 ? MM_FRESNEL_DCT_COL_BATCH : kBatchElems)>
-using DctColShape = std::conditional_t<(RegShape<L, BATCH>::TPC < 16), ColFastShape<L, BATCH>, RegShape<L, BATCH>>;
+using DctColShape = std::conditional_t<(RegShape<L, BATCH>::TPC < 16 || L >= 2048), ColFastShape<L, BATCH>,
+                                       RegShape<L, BATCH>>;
 template <int L>
 using DctRowShape = RegShape<L, MM_FRESNEL_DCT_ROW_BATCH, 0, false>;
 
