@@ -62,8 +62,9 @@ __device__ __forceinline__ int frame_row(int m, int t) {
 // Column-pass shape: long transforms take 8192 points per workgroup (two
 // columns of a 4096-point transform, 70 KB of LDS), so two workgroups share a
 // CU and one's loads and stores overlap the other's two transforms (4 %
-// faster D than four columns per workgroup, which holds a CU alone); short
-// ones keep 16384 points (many columns per workgroup).
+// faster D than four columns per workgroup, which holds a CU alone; the
+// column-fastest mapping of the DCT passes with four columns was 2 % slower
+// here); short ones keep 16384 points (many columns per workgroup).
 template <int L>
 using ColShape = RegShape<L, (L >= 2048 ? kRowBatch : kBatchElems)>;
 
@@ -178,26 +179,11 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void row_pass(cons
 #ifndef MM_FRESNEL_DCT_COL_BATCH
 #define MM_FRESNEL_DCT_COL_BATCH kBatchElems
 #endif
-// Column transforms of fewer than 16 threads, and of 2048 points and more, put consecutive lanes on
-// consecutive columns of one row: a wave's load or store is one contiguous run of its workgroup's columns
-// (64 of them for short transforms, 8 = 64 bytes for 2048-point ones) instead of 8-byte pieces 16 lanes
-// apart.  With lanes of one wave on different transforms at stride S in LDS, an odd S keeps every 32-lane
-// 8-byte LDS access on distinct bank pairs.  (2048-point columns: 8 per workgroup in this mapping,
-// 6.13 -> 5.62 ms per D of 128 frames of 2048^2; 4 per workgroup in the interleaved one was the best
-// before.)
-template <int L, int BATCH>
-struct ColFastShape : RegShape<L, BATCH> {
-  using Base = RegShape<L, BATCH>;
-  static constexpr int S = Base::S0 | 1;
-  static constexpr int LDS = Base::C * S;
-  __device__ static __forceinline__ void map(int tid, int& c, int& t) {
-    c = tid % Base::C;
-    t = tid / Base::C;
-  }
-};
+// Column transforms of fewer than 16 threads, and of 2048 points and more, take the column-fastest lane
+// mapping (ColFastShape): 8 columns per workgroup for 2048-point columns, 6.13 -> 5.62 ms per D of 128
+// frames of 2048^2 (4 per workgroup in the interleaved mapping was the best before).
 template <int L, int BATCH = (L >= 2048 ? MM_FRESNEL_DCT_COL_BATCH : kBatchElems)>
-using DctColShape = std::conditional_t<(RegShape<L, BATCH>::TPC < 16 || L >= 2048), ColFastShape<L, BATCH>,
-                                       RegShape<L, BATCH>>;
+using DctColShape = ColumnShape<L, BATCH>;
 template <int L>
 using DctRowShape = RegShape<L, MM_FRESNEL_DCT_ROW_BATCH, 0, false>;
 

@@ -5,6 +5,8 @@
 // arithmetic only.
 #pragma once
 
+#include <type_traits>
+
 #include "mm_common.h"
 
 namespace mm {
@@ -210,6 +212,27 @@ __device__ __forceinline__ int opaque_index(int t) {
   asm volatile("" : "+v"(t));
   return t;
 }
+
+// Column-fastest lane mapping for column transforms (columns of a row-major array, row stride = the array
+// width): consecutive lanes on consecutive columns of one row, so a wave's load or store is one contiguous
+// run of its workgroup's C columns (8-byte pieces 16 lanes apart, on 16 different rows, in the interleaved
+// mapping of RegShape).  With the lanes of one wave on different transforms at stride S in LDS, an odd S
+// keeps every 32-lane 8-byte LDS access on distinct bank pairs.
+template <int L, int BATCH>
+struct ColFastShape : RegShape<L, BATCH> {
+  using Base = RegShape<L, BATCH>;
+  static constexpr int S = Base::S0 | 1;
+  static constexpr int LDS = Base::C * S;
+  __device__ static __forceinline__ void map(int tid, int& c, int& t) {
+    c = tid % Base::C;
+    t = tid / Base::C;
+  }
+};
+// Column transforms of fewer than 16 threads or of 2048 points and more run column-fastest; the others keep
+// the interleaved mapping.
+template <int L, int BATCH>
+using ColumnShape = std::conditional_t<(RegShape<L, BATCH>::TPC < 16 || L >= 2048), ColFastShape<L, BATCH>,
+                                       RegShape<L, BATCH>>;
 
 // Row-pass shape of the register FFT: 8192 points per workgroup, plain mapping.
 template <int L, int XTRA = 0>
