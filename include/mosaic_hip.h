@@ -55,13 +55,18 @@ int mm_version(void);
  *     MM_RADON_FP64_TILES  scatter tiles accumulate in FP64;
  *     MM_RADON_KAHAN_TILES scatter tiles accumulate in FP32 with Kahan
  *                          compensation (not together with FP64_TILES);
+ *     MM_RADON_LIST_TILES  scatter tiles accumulate in FP32 on the vector
+ *                          ALUs from per-point tap lists;
  *     MM_RADON_GATHER_ZP1  gather tiles of width <= 32 stage one z slice per
  *                          round (default: two slices interleaved per patch
  *                          element; wider tiles always stage one).
+ *   Without one of the three tile flags the scatter runs as dense 4 x 4-patch
+ *   products on the matrix cores (FP32 in, FP32 accumulate).
  */
 #define MM_RADON_FP64_TILES 1u
 #define MM_RADON_GATHER_ZP1 2u
 #define MM_RADON_KAHAN_TILES 4u
+#define MM_RADON_LIST_TILES 8u
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags);
 
 /* R: obj (nz, n, n) float32 -> sino (ntheta, nz, n) float32, 1 <= nz <= nz_max.

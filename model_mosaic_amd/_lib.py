@@ -19,6 +19,7 @@ MM_ERR_UNSUPPORTED = -4
 MM_RADON_FP64_TILES = 1
 MM_RADON_GATHER_ZP1 = 2
 MM_RADON_KAHAN_TILES = 4
+MM_RADON_LIST_TILES = 8
 MM_RADON_TUNE_GATHER_XCD_GROUP = 1
 MM_RADON_TUNE_SCATTER_XCD_GROUP = 2
 MM_RADON_TUNE_COL_SPLIT = 3

@@ -29,13 +29,15 @@
 //    LDS (applying the conj reflection while loading) and each lane sums its
 //    81 taps from LDS; per-sample weights stay in registers across the z loop.
 //  * The scatter is owner-computes, without atomics: a workgroup owns a 16 x 16
-//    tile of the output half plane, each lane one point of it, and walks the
-//    point's plan-time list of (entry, tap) slots in registers, reading the
-//    entries' values from LDS; accumulation is FP32 (FP64 with
-//    MM_RADON_FP64_TILES, compensated FP32 with MM_RADON_KAHAN_TILES) in a
-//    fixed order, so RT is deterministic.  Tiles whose lists exceed one work
-//    item (near the origin, where every angle passes) are split into partial
-//    tiles that scatter_reduce sums.
+//    tile of the output half plane and reads the values of the entries touching
+//    it from LDS.  By default (scatter_mfma) each 4 x 4 patch of the tile is a
+//    dense product of the entries' values and Gaussian weights on the matrix
+//    cores (FP32); with MM_RADON_LIST_TILES (FP32), MM_RADON_FP64_TILES (FP64)
+//    or MM_RADON_KAHAN_TILES (compensated FP32) each lane owns one point and
+//    walks its plan-time list of (entry, tap) slots (scatter_lists).  The
+//    summation order is fixed either way, so RT is deterministic.  Tiles with
+//    more entries than one work item holds (near the origin, where every angle
+//    passes) are split into partial tiles that scatter_reduce sums.
 //  * FFTs are the register-resident Stockham transforms of mm_fft.h (up to
 //    16384 points per workgroup); the 2n-point passes of n > 8192 run as two
 //    n-point halves (r1_rows_split, col_fft_split, rt4_rows_c2r_split).
@@ -856,6 +858,12 @@ struct SplitTile {
   int x0, y0, slot0, nslots;
 };
 
+// Tile point (row-major, 16 x 16) of index p in 4 x 4 patch order.
+__host__ __device__ __forceinline__ int patch_major_point(int p) {
+  const int P = p >> 4, q = p & 15;
+  return (4 * (P >> 2) + (q >> 2)) * STS + 4 * (P & 3) + (q & 3);
+}
+
 typedef unsigned u32x32 __attribute__((ext_vector_type(SL_PAIRS_A)));
 typedef unsigned u32x16 __attribute__((ext_vector_type(SL_PAIRS_B)));  // power-of-two sizes stay in VGPRs
 
@@ -1015,8 +1023,9 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(MM_SCATTER_
         }
       }
     } else {
+      const int q = patch_major_point(p);  // partial tiles are stored in 4 x 4 patch order
       for (int zz = 0; zz < nzz; ++zz) {
-        const int i = zz * SNP + p;
+        const int i = zz * SNP + q;
         partial[((long long)it.slot * nz + zc + zz) * SNP + p] = R[i / SZC].v[i % SZC];
       }
     }
@@ -1024,14 +1033,16 @@ __global__ __launch_bounds__(SNP) __attribute__((amdgpu_waves_per_eu(MM_SCATTER_
   }
 }
 
-// Sum of the partial tiles of split tiles.
+// Sum of the partial tiles of split tiles (stored in 4 x 4 patch order: index
+// p = 16 patch + point, patch = 4 patch row + patch column, point = 4 row + column).
 __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__ partial, float2* __restrict__ Hb,
                                                       const SplitTile* __restrict__ splits, int n, int nz,
                                                       long long hzs) {
   const SplitTile st = splits[blockIdx.x];
   const int z = blockIdx.y;
   const int p = threadIdx.x;
-  const int l0 = st.x0 + (p % STS), l1 = st.y0 + (p / STS);
+  const int q = patch_major_point(p);
+  const int l0 = st.x0 + (q % STS), l1 = st.y0 + (q / STS);
   if (l0 > n) return;
   float2 acc = make_float2(0.f, 0.f);
 #pragma unroll 8  // eight loads in flight; the summation order stays slot order
@@ -1041,6 +1052,229 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
     acc.y += v.y;
   }
   Hb[(long long)z * hzs + hs_idx(l1 < 0 ? l1 + 2 * n : l1, l0, 2 * n)] = acc;
+}
+
+// ---- adjoint scatter on the matrix cores (the default FP32 path) ------------
+// The 16 x 16 output tile is sixteen 4 x 4 patches, aligned to the 4-column
+// blocks of the half plane (a patch of one slice is 16 consecutive complex
+// values).  An entry's 9 x 9 window covers exactly 3 x 3 patches (9 = 2 * 4 + 1)
+// wherever it sits, so the scatter into one patch is the dense product
+//   C[zc][q] = sum_e V[zc][e] W[e][q],   zc = 2 slice + (re | im), q = patch point,
+// over the entries e touching the patch, with W[e][q] the Gaussian weight of
+// the tap q - (window base of e) and 0 off the window: 144 (entry, point)
+// pairs per entry for its 81 taps (1.78x zero fill).  One
+// v_mfma_f32_16x16x4_f32 takes 4 entries x 16 zc x 16 points; a staging round
+// holds MZ = 32 slices (four 16-row zc blocks), so each weight is evaluated once
+// per 16 slices and each staged value is read once per patch.  A workgroup of
+// 8 waves owns the tile, two patches per wave.
+// At plan time the item's entries are sorted by window anchor class
+// (floor(bx / 4) + 2, floor(by / 4) + 2) in [0, 6)^2, bx, by = window base
+// relative to the tile: the entries touching patch (px, py) are then three
+// contiguous runs (anchor rows py .. py + 2, anchor columns px .. px + 2 each),
+// so no per-point list exists.  The fold needs no test: a tap counts iff its
+// raw column is <= n, i.e. iff the output point exists; points past column n
+// are computed and not written.
+// Lane l of a wave: k = l / 16 picks the chunk's entry, q = l % 16 the patch
+// point (B operand W[k][q]) and the zc row (A operand V[zc = q][k]); the result
+// C[zc][q] lands in register r of lane l with zc = 4 (l / 16) + r, q = l % 16.
+// Accumulation order is fixed by the plan: RT stays bitwise deterministic.
+constexpr int MZ = 16;            // z slices per staging round (two 16-row zc blocks)
+constexpr int MZB = MZ / 8;       // zc blocks of 16
+constexpr int MROW = 2 * MZ + 4;  // floats per LDS entry row: values, then bx + M, by + M, fx, fy
+constexpr int ME_MAX = 480;       // entries per work item (78 KB of LDS: two workgroups per CU)
+constexpr int MNT = 512;          // threads: 8 waves, two patches each
+constexpr int MCLS = 36;          // anchor classes
+constexpr int MNULL = ME_MAX;     // LDS row of zeros outside every window: pads the patch lists
+// patch lists: <= 9 patches per entry, <= 3 padding slots per patch, and room for the read-ahead of a
+// patch's last chunk (4 slots past the last list)
+constexpr int MLIST = 9 * ME_MAX + 16 * 3 + 4;
+static_assert(ME_MAX <= MNT, "one staging thread per entry");
+static_assert(MROW % 4 == 0 && (MROW / 4) % 2 == 1, "16-byte rows at an odd 16-byte stride: conflict-free staging");
+static_assert(MROW * (ME_MAX + 1) < 65536, "row offsets in 16 bits");
+
+struct MItem {
+  int eoff, ne;            // entries[eoff, eoff + ne), sorted by anchor class
+  int x0, y0;              // tile origin (l0, l1), centred coordinates
+  int slot;                // -1: the item owns the tile; >= 0: partial slot
+  int pad_[3];
+  unsigned short cls[40];  // class c: entries [cls[c], cls[c + 1]), c < MCLS; cls[MCLS] = ne
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Entries of patch (px, py) in run order (three runs of anchor rows py .. py + 2)
+// padded with MNULL to a multiple of 4; returns the padded length.
+__device__ __forceinline__ int patch_list_len(const unsigned short* cls, int px, int py) {
+  int k = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) k += cls[(py + r) * 6 + px + 3] - cls[(py + r) * 6 + px];
+  return (k + 3) & ~3;
+}
+
+// OFF64: byte offsets of the representatives in 64 bits (their buffer spans 4 GiB or more); otherwise
+// 32-bit offsets from a wave-uniform base.
+template <bool OFF64>
+__global__ __launch_bounds__(MNT) void scatter_mfma(const float2* __restrict__ yh, float2* __restrict__ Hb,
+                                                    float2* __restrict__ partial, const MItem* __restrict__ items,
+                                                    const int* __restrict__ entries,
+                                                    const unsigned short* __restrict__ rows,
+                                                    const double2* __restrict__ dirs, int n, int H, int nz,
+                                                    long long hzs, int zchunk, float a2, int xg) {
+  __shared__ __align__(16) float R[(ME_MAX + 1) * MROW];
+  __shared__ unsigned short LST[MLIST];
+  __shared__ unsigned short CLS[40];
+  const unsigned lw = xcd_group(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y, xg);
+  const unsigned item = lw % gridDim.x;
+  const int z0 = (int)(lw / gridDim.x) * zchunk, z1 = min(nz, z0 + zchunk);
+  const MItem* it = items + item;
+  const int ne = it->ne, x0 = it->x0, y0 = it->y0, slot = it->slot;
+  const int t = threadIdx.x;
+  if (t < 40) CLS[t] = it->cls[t];
+  for (int j = t; j < MLIST; j += MNT) LST[j] = (unsigned short)(MNULL * MROW);  // read-ahead slots: valid rows
+  // Staging: thread j takes entry j (item entries are in sample order, so neighbouring lanes stage
+  // neighbouring samples) and writes its window base relative to the tile and its fractional
+  // offset (mirror: negated) into the entry's LDS row (its place in anchor-class order).
+  using Off = typename std::conditional<OFF64, long long, unsigned>::type;
+  Off src = 0;  // byte offset of the entry's (theta, k) at z = 0
+  bool cj = false;
+  int myrow = MNULL;
+  if (t < ne) {
+    const int code = entries[it->eoff + t];
+    myrow = rows[it->eoff + t];
+    const SampleGeom g = sample_geom(code >> 1, H, dirs);
+    cj = (code & 1) != 0;
+    const int twon = 2 * n;
+    const int cx = cj ? -g.rx : g.rx, cy = cj ? -g.ry : g.ry;
+    const int bx = ((cx - M - x0 + 8) % twon + twon) % twon - 8;
+    const int by = ((cy - M - y0 + 8) % twon + twon) % twon - 8;
+    *reinterpret_cast<float4*>(&R[myrow * MROW + 2 * MZ]) =
+        make_float4((float)(bx + M), (float)(by + M), cj ? -g.fx : g.fx, cj ? -g.fy : g.fy);
+    src = (Off)(((long long)g.th * nz * H + g.k) * (long long)sizeof(float2));
+  }
+  if (t == MNULL) {
+    float4* r = reinterpret_cast<float4*>(&R[MNULL * MROW]);
+#pragma unroll
+    for (int j = 0; j < MROW / 4 - 1; ++j) r[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    r[MROW / 4 - 1] = make_float4(1e4f, 1e4f, 0.f, 0.f);
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l = t & 63, kk = l >> 4, q = l & 15;
+  const int py = w >> 1;  // this wave's patch row; patches px = 2 (w & 1) + {0, 1}
+  const float fqy = (float)(4 * py + (q >> 2));
+  // patch lists: patch P = 4 py + px at LST[off(P)], off = the padded lengths of the patches before
+  // it; each wave writes its own two
+  int off[2], len[2];
+  {
+    int o = 0;
+    for (int P = 0; P < 4 * py + 2 * (w & 1); ++P) o += patch_list_len(CLS, P & 3, P >> 2);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int px = 2 * (w & 1) + i;
+      off[i] = __builtin_amdgcn_readfirstlane(o);
+      len[i] = __builtin_amdgcn_readfirstlane(patch_list_len(CLS, px, py));
+      o += len[i];
+      const int s0 = CLS[py * 6 + px], L0 = CLS[py * 6 + px + 3] - s0;
+      const int s1 = CLS[(py + 1) * 6 + px], L1 = CLS[(py + 1) * 6 + px + 3] - s1;
+      const int s2 = CLS[(py + 2) * 6 + px];
+      const int L01 = L0 + L1, K = L01 + CLS[(py + 2) * 6 + px + 3] - s2;
+      for (int j = l; j < len[i]; j += 64) {
+        const int e = j < L0 ? s0 + j : (j < L01 ? s1 + j - L0 : (j < K ? s2 + j - L01 : MNULL));
+        LST[off[i] + j] = (unsigned short)(e * MROW);  // the row's first float
+      }
+    }
+  }
+  // Staging at the start of each round.  Every lane issues all its loads from valid addresses
+  // (the round's last slice and sample 0 stand in for missing ones) and zeroes what it does not
+  // use, so the loads stay in flight together.  (Loading round r + 1 into registers during round
+  // r measured slower: the second workgroup on the CU already overlaps one's staging with the
+  // other's products.)
+  float2 v[MZ];
+  // (the representatives span less than 4 GiB unless OFF64: byte offsets fit the buffer form, whose
+  // range check would return zeros rather than fault)
+  const __amdgpu_buffer_rsrc_t ybuf = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(yh), (short)0, -1, 0x00020000);
+  auto load_round = [&](int zr) {
+    const int nzr = min(MZ, z1 - zr);
+#pragma unroll
+    for (int zz = 0; zz < MZ; ++zz) {
+      const long long zoff = (long long)(zr + min(zz, nzr - 1)) * H * (long long)sizeof(float2);
+      if constexpr (OFF64) {
+        v[zz] = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(yh) + zoff + src);
+      } else {  // buffer load: per-lane 32-bit offset plus the slice's (uniform) offset
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(ybuf, (int)src, (int)zoff, 0);
+        v[zz] = make_float2(__uint_as_float(u.x), __uint_as_float(u.y));
+      }
+    }
+#pragma unroll
+    for (int zz = 0; zz < MZ; ++zz)
+      if (t >= ne || zz >= nzr) v[zz] = make_float2(0.f, 0.f);
+  };
+  for (int zc = z0; zc < z1; zc += MZ) {
+    const int nzz = min(MZ, z1 - zc);
+    load_round(zc);
+    if (t < ne) {
+      float4* dst = reinterpret_cast<float4*>(&R[myrow * MROW]);
+      const float sy = cj ? -1.f : 1.f;
+#pragma unroll
+      for (int j = 0; j < MZ / 2; ++j)
+        dst[j] = make_float4(v[2 * j].x, sy * v[2 * j].y, v[2 * j + 1].x, sy * v[2 * j + 1].y);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < 2; ++i) {
+      const int px = 2 * (w & 1) + i;
+      const float fqx = (float)(4 * px + (q & 3));
+      const unsigned short* lp = LST + off[i] + kk;
+      // one chunk of 4 entries per step (MZB independent accumulators); the next chunk's row
+      // offset is read at the top of the step, off the dependent path
+      f32x4 a[MZB];
+#pragma unroll
+      for (int j = 0; j < MZB; ++j) a[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      int ea = lp[0];
+      for (int c = 0; c < len[i]; c += 4) {
+        const int na = lp[c + 4];
+        __builtin_amdgcn_sched_barrier(0);  // issue that read first: it is waited for at the step's end
+        const float* ra = R + ea;
+        const float4 ma = *reinterpret_cast<const float4*>(ra + 2 * MZ);
+        float va[MZB];
+#pragma unroll
+        for (int j = 0; j < MZB; ++j) va[j] = ra[16 * j + q];
+        // tap offset from the window centre (exact), then the offset from the sample
+        const float tx = fqx - ma.x, ty = fqy - ma.y;
+        const float dx = tx - ma.z, dy = ty - ma.w;
+        float wgt = __builtin_amdgcn_exp2f(a2 * (dx * dx + dy * dy));
+        wgt = fmaxf(fabsf(tx), fabsf(ty)) <= (float)M ? wgt : 0.f;
+#pragma unroll
+        for (int j = 0; j < MZB; ++j) a[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(va[j], wgt, a[j], 0, 0, 0);
+        ea = na;
+        asm volatile("" : "+v"(ea));  // keeps the offset read a step ahead (not re-read here)
+      }
+      // lane (kk, q) holds slices 8 j + 2 kk and 8 j + 2 kk + 1 of point q in block j
+      const int col = x0 + 4 * px + (q & 3), lrow = y0 + 4 * py + (q >> 2);
+      const float sgn = ((col + lrow) & 1) ? -1.f : 1.f;
+      float2* dst;
+      long long zs;
+      bool live = true;
+      if (slot < 0) {
+        dst = Hb + (long long)zc * hzs + hs_idx(lrow < 0 ? lrow + 2 * n : lrow, col, 2 * n);
+        zs = hzs;
+        live = col <= n;
+      } else {
+        dst = partial + ((long long)slot * nz + zc) * SNP + (4 * py + px) * 16 + q;
+        zs = SNP;
+      }
+      if (live) {
+#pragma unroll
+        for (int j = 0; j < MZB; ++j) {
+          const int s = 8 * j + 2 * kk;
+          if (s < nzz) dst[(long long)s * zs] = make_float2(sgn * a[j][0], sgn * a[j][1]);
+          if (s + 1 < nzz) dst[(long long)(s + 1) * zs] = make_float2(sgn * a[j][2], sgn * a[j][3]);
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Row pass of RT: C2R inverse transform of the half-spectrum rows n/2 + y
@@ -1767,7 +2001,9 @@ static void scatter_tile_items(const HostGeom& G, int tx, int ty, const std::vec
   }
 }
 
-static void build_scatter_lists(const HostGeom& G, SBinning& B) {
+// Entry codes ((sample id << 1) | mirror) of every 16 x 16 output tile that
+// one of the entry's counted taps (raw column <= n) lands in.
+static std::vector<std::vector<int>> scatter_tile_codes(const HostGeom& G) {
   const int n = G.n;
   const int ntx = n / STS + 1, nty = 2 * n / STS;
   std::vector<std::vector<int>> tiles((size_t)ntx * nty);
@@ -1794,6 +2030,12 @@ static void build_scatter_lists(const HostGeom& G, SBinning& B) {
           for (int tx = r.c0 / STS; tx <= r.c1 / STS; ++tx) tiles[(size_t)ty * ntx + tx].push_back((id << 1) | mir);
     }
   }
+  return tiles;
+}
+
+static void build_scatter_lists(const HostGeom& G, SBinning& B) {
+  const int ntx = G.n / STS + 1;
+  const std::vector<std::vector<int>> tiles = scatter_tile_codes(G);
   const size_t ntiles = tiles.size();
   unsigned nw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   std::vector<SBinning> part(nw);
@@ -1821,6 +2063,123 @@ static void build_scatter_lists(const HostGeom& G, SBinning& B) {
     B.entries.insert(B.entries.end(), P.entries.begin(), P.entries.end());
     B.perm.insert(B.perm.end(), P.perm.begin(), P.perm.end());
     B.lists.insert(B.lists.end(), P.lists.begin(), P.lists.end());
+    B.nslots += P.nslots;
+  }
+}
+
+// ---- scatter patch runs (host, at plan time) --------------------------------
+
+struct MBinning {
+  std::vector<MItem> items;
+  std::vector<int> entries;            // per item, in sample order
+  std::vector<unsigned short> rows;    // per entry: its LDS row (place in anchor-class order)
+  std::vector<SplitTile> splits;
+  int nslots = 0;
+};
+
+// Window base (bx, by) of an entry relative to the tile origin (x0, y0): the
+// tap (i0, i1) lands on tile point (bx + i0, by + i1) when that lies in the
+// tile (columns and centred rows wrap mod 2n; the representative in
+// [-8, 2n - 8) is the one that can reach the tile).
+static inline void window_base(const HostGeom& G, int code, int x0, int y0, int& bx, int& by) {
+  const int twon = 2 * G.n;
+  int rx, ry;
+  G.centre(code >> 1, rx, ry);
+  const int cx = (code & 1) ? -rx : rx, cy = (code & 1) ? -ry : ry;
+  bx = pmod(cx - M - x0 + 8, twon) - 8;
+  by = pmod(cy - M - y0 + 8, twon) - 8;
+}
+
+// Items of one 16 x 16 output tile for scatter_mfma: the LDS rows of the
+// entries in anchor-class order ((by + 8) / 4) * 6 + (bx + 8) / 4; more than
+// ME_MAX entries are split into near-equal runs of consecutive rows whose
+// partial tiles scatter_reduce sums.  Each item lists its entries in sample
+// order with their rows, so the staging loads of neighbouring lanes hit
+// neighbouring samples.
+static void scatter_tile_items_mfma(const HostGeom& G, int tx, int ty, const std::vector<int>& codes, MBinning& B) {
+  const int x0 = tx * STS, y0 = ty * STS - G.n;
+  MItem base{};
+  base.x0 = x0;
+  base.y0 = y0;
+  base.slot = -1;
+  base.eoff = (int)B.entries.size();
+  std::vector<std::pair<int, int>> ce;  // (class, code)
+  ce.reserve(codes.size());
+  for (int code : codes) {
+    int bx, by;
+    window_base(G, code, x0, y0, bx, by);
+    if (bx < -8 || bx >= STS || by < -8 || by >= STS) continue;  // no tap in the tile
+    ce.push_back({((by + 8) >> 2) * 6 + ((bx + 8) >> 2), code});
+  }
+  const int ne = (int)ce.size();
+  if (ne == 0) {  // no sample touches the tile: the item writes zeros
+    B.items.push_back(base);
+    return;
+  }
+  std::stable_sort(ce.begin(), ce.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    return a.first < b.first;
+  });
+  const int nch = (ne + ME_MAX - 1) / ME_MAX;
+  int slot0 = -1;
+  if (nch > 1) {
+    slot0 = B.nslots;
+    B.nslots += nch;
+    B.splits.push_back({x0, y0, slot0, nch});
+  }
+  for (int c = 0; c < nch; ++c) {
+    const int a = (int)((long long)ne * c / nch), b = (int)((long long)ne * (c + 1) / nch);
+    MItem it = base;
+    it.eoff = (int)B.entries.size();
+    it.ne = b - a;
+    it.slot = nch > 1 ? slot0 + c : -1;
+    int cnt[MCLS] = {};
+    std::vector<std::pair<int, int>> order;  // (code, row): staged in sample order
+    for (int e = a; e < b; ++e) {
+      ++cnt[ce[e].first];
+      order.push_back({ce[e].second, e - a});
+    }
+    std::sort(order.begin(), order.end());
+    for (const auto& o : order) {
+      B.entries.push_back(o.first);
+      B.rows.push_back((unsigned short)o.second);
+    }
+    int acc = 0;
+    for (int k = 0; k < MCLS; ++k) {
+      it.cls[k] = (unsigned short)acc;
+      acc += cnt[k];
+    }
+    for (int k = MCLS; k < 40; ++k) it.cls[k] = (unsigned short)acc;
+    B.items.push_back(it);
+  }
+}
+
+static void build_scatter_patches(const HostGeom& G, MBinning& B) {
+  const int ntx = G.n / STS + 1;
+  const std::vector<std::vector<int>> tiles = scatter_tile_codes(G);
+  const size_t ntiles = tiles.size();
+  unsigned nw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<MBinning> part(nw);
+  std::vector<std::thread> pool;
+  for (unsigned w = 0; w < nw; ++w)
+    pool.emplace_back([&, w]() {
+      const size_t t0 = ntiles * w / nw, t1 = ntiles * (w + 1) / nw;
+      for (size_t t = t0; t < t1; ++t)
+        scatter_tile_items_mfma(G, (int)(t % ntx), (int)(t / ntx), tiles[t], part[w]);
+    });
+  for (auto& th : pool) th.join();
+  for (auto& P : part) {
+    const int eoff = (int)B.entries.size(), soff = B.nslots;
+    for (MItem it : P.items) {
+      it.eoff += eoff;
+      if (it.slot >= 0) it.slot += soff;
+      B.items.push_back(it);
+    }
+    for (SplitTile s : P.splits) {
+      s.slot0 += soff;
+      B.splits.push_back(s);
+    }
+    B.entries.insert(B.entries.end(), P.entries.begin(), P.entries.end());
+    B.rows.insert(B.rows.end(), P.rows.begin(), P.rows.end());
     B.nslots += P.nslots;
   }
 }
@@ -1869,27 +2228,29 @@ static std::vector<int2> gather_support(const Binning& B, int n) {
 // Rows the scatter writes.  Per tile column, the tiles between the lowest and
 // the highest tile that has entries stay work items (empty ones write zeros);
 // empty tiles outside that range are dropped, and the column pass reads the
-// range only.  Returns the support; compacts B.items and B.perm.
-static std::vector<int2> trim_scatter(SBinning& B, int n) {
+// range only.  Returns the support; compacts the items (and the lane
+// permutations of list items).
+template <class ItemT>
+static std::vector<int2> trim_scatter_items(std::vector<ItemT>& all, std::vector<unsigned char>* perm, int n) {
   const int ntx = n / STS + 1;
   std::vector<int> tmin(ntx, INT_MAX), tmax(ntx, INT_MIN);
-  for (const SItem& it : B.items)
+  for (const ItemT& it : all)
     if (it.ne > 0) {
       const int tx = it.x0 / STS, ty = (it.y0 + n) / STS;
       tmin[tx] = std::min(tmin[tx], ty);
       tmax[tx] = std::max(tmax[tx], ty);
     }
-  std::vector<SItem> items;
-  std::vector<unsigned char> perm;
-  for (size_t i = 0; i < B.items.size(); ++i) {
-    const SItem& it = B.items[i];
+  std::vector<ItemT> items;
+  std::vector<unsigned char> kept;
+  for (size_t i = 0; i < all.size(); ++i) {
+    const ItemT& it = all[i];
     const int tx = it.x0 / STS, ty = (it.y0 + n) / STS;
     if (it.ne == 0 && (ty < tmin[tx] || ty > tmax[tx])) continue;
     items.push_back(it);
-    perm.insert(perm.end(), B.perm.begin() + (long long)i * SNP, B.perm.begin() + (long long)(i + 1) * SNP);
+    if (perm) kept.insert(kept.end(), perm->begin() + (long long)i * SNP, perm->begin() + (long long)(i + 1) * SNP);
   }
-  B.items.swap(items);
-  B.perm.swap(perm);
+  all.swap(items);
+  if (perm) perm->swap(kept);
   std::vector<int2> s((size_t)(n / 4 + 1), make_int2(0, 0));
   for (int tx = 0; tx < ntx; ++tx)
     if (tmin[tx] <= tmax[tx])
@@ -1897,6 +2258,8 @@ static std::vector<int2> trim_scatter(SBinning& B, int n) {
         support_add(s, c, tmin[tx] * STS - n, (tmax[tx] + 1) * STS - n);
   return s;
 }
+static std::vector<int2> trim_scatter(SBinning& B, int n) { return trim_scatter_items(B.items, &B.perm, n); }
+static std::vector<int2> trim_scatter(MBinning& B, int n) { return trim_scatter_items(B.items, nullptr, n); }
 
 template <typename T>
 static int upload(T** dst, const std::vector<T>& src, size_t& bytes) {
@@ -1943,6 +2306,12 @@ struct mm_radon {
   bool split = false;        // 2n > kMaxFFT
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
   int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
+  // adjoint scatter: patch runs on the matrix cores (FP32, default) or per-point lists (FP64 / Kahan tiles)
+  bool mfma = true;
+  MItem* mitems = nullptr;
+  int* mentries = nullptr;
+  unsigned short* mrows = nullptr;
+  int n_mitems = 0;
   SItem* sitems = nullptr;
   unsigned char* sperm = nullptr;
   int* sentries = nullptr;
@@ -1975,7 +2344,7 @@ static void radon_free(mm_radon* p) {
   if (!p->own_ws) p->fde = p->gh = nullptr;
   void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
                   p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
-                  p->splits, p->gsupp, p->ssupp, p->hsplit};
+                  p->mitems, p->mentries, p->mrows, p->splits, p->gsupp, p->ssupp, p->hsplit};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -2147,12 +2516,19 @@ static int launch_gather(const mm_radon* p, int nz, hipStream_t st) {
 }
 
 static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int nz, hipStream_t st) {
-  const int zsplit = zsplit_for(p->n_sitems, nz);
+  const int zsplit = zsplit_for(p->mfma ? p->n_mitems : p->n_sitems, nz);
   const int zchunk = (nz + zsplit - 1) / zsplit;
   const int zblocks = (nz + zchunk - 1) / zchunk;
   const long long hzs = p->zs;
   const int xg = p->tune[MM_RADON_TUNE_SCATTER_XCD_GROUP];
-  if (p->flags & MM_RADON_FP64_TILES)
+  if (p->mfma) {
+    const bool off64 = (unsigned long long)p->ntheta * nz * p->H * sizeof(float2) >= (1ULL << 32);
+    hipLaunchKernelGGL(off64 ? scatter_mfma<true> : scatter_mfma<false>,
+                       dim3((unsigned)p->n_mitems, (unsigned)zblocks), dim3(MNT), 0, st, yh, Hb, p->partial,
+                       p->mitems, p->mentries, p->mrows, p->dirs, p->n, p->H, nz, hzs, zchunk,
+                       (float)(-(double)p->a * M_LOG2E), xg);
+  }
+  else if (p->flags & MM_RADON_FP64_TILES)
     hipLaunchKernelGGL(scatter_lists<double>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
                        (float)(-(double)p->a * M_LOG2E), xg);
@@ -2164,7 +2540,7 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
     hipLaunchKernelGGL(scatter_lists<float>, dim3((unsigned)p->n_sitems, (unsigned)zblocks), dim3(SNP), 0, st, yh, Hb,
                        p->partial, p->sitems, p->sperm, p->sentries, p->slists, p->dirs, p->n, p->H, nz, hzs, zchunk,
                        (float)(-(double)p->a * M_LOG2E), xg);
-  int rc = check_launch("scatter_lists");
+  int rc = check_launch(p->mfma ? "scatter_mfma" : "scatter_lists");
   if (rc) return rc;
   if (p->n_splits > 0) {
     hipLaunchKernelGGL(scatter_reduce, dim3((unsigned)p->n_splits, (unsigned)nz), dim3(SNP), 0, st, p->partial, Hb,
@@ -2207,10 +2583,10 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
-  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES)) == 0,
+  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0,
                "mm_radon_create: unknown flags");
-  MM_ARG_CHECK(!((flags & MM_RADON_FP64_TILES) && (flags & MM_RADON_KAHAN_TILES)),
-               "mm_radon_create: MM_RADON_FP64_TILES and MM_RADON_KAHAN_TILES exclude each other");
+  MM_ARG_CHECK(__builtin_popcount(flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) <= 1,
+               "mm_radon_create: at most one of MM_RADON_FP64_TILES, MM_RADON_KAHAN_TILES, MM_RADON_LIST_TILES");
   const double eps = 1e-3;
   const double mu = -std::log(eps) / (2.0 * n * n);
   const int m = (int)std::ceil(2.0 * n / M_PI * std::sqrt(-mu * std::log(eps) + (mu * n) * (mu * n) / 4.0));
@@ -2267,14 +2643,21 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   BG.zp = (flags & MM_RADON_GATHER_ZP1) ? 1 : 2;
   p->gzp = BG.zp;
   SBinning BS;
+  MBinning BM;
   const int root = std::min(ROOT, n);
   build_gather(G, root, BG);
   p->fwd_only = fwd_only;
+  p->mfma = (flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0;
   const std::vector<int2> gsupp = gather_support(BG, n);
   std::vector<int2> ssupp;
   if (!fwd_only) {
-    build_scatter_lists(G, BS);
-    ssupp = trim_scatter(BS, n);
+    if (p->mfma) {
+      build_scatter_patches(G, BM);
+      ssupp = trim_scatter(BM, n);
+    } else {
+      build_scatter_lists(G, BS);
+      ssupp = trim_scatter(BS, n);
+    }
   }
 
   size_t& bytes = p->bytes;
@@ -2283,7 +2666,9 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
       (rc = upload(&p->gsamples, BG.samples, bytes)) || (rc = upload(&p->gorder, BG.order, bytes)) ||
       (rc = upload(&p->sitems, BS.items, bytes)) || (rc = upload(&p->sperm, BS.perm, bytes)) ||
       (rc = upload(&p->sentries, BS.entries, bytes)) || (rc = upload(&p->slists, BS.lists, bytes)) ||
-      (rc = upload(&p->splits, BS.splits, bytes)) || (rc = upload(&p->gsupp, gsupp, bytes)) ||
+      (rc = upload(&p->mitems, BM.items, bytes)) || (rc = upload(&p->mentries, BM.entries, bytes)) ||
+      (rc = upload(&p->mrows, BM.rows, bytes)) ||
+      (rc = upload(&p->splits, p->mfma ? BM.splits : BS.splits, bytes)) || (rc = upload(&p->gsupp, gsupp, bytes)) ||
       (rc = upload(&p->ssupp, ssupp, bytes)))
     return bail(rc);
   if (p->split) {
@@ -2297,8 +2682,9 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   p->n_gitems = (int)BG.items.size();
   p->n_groot = BG.n_root;
   p->n_sitems = (int)BS.items.size();
-  p->n_splits = (int)BS.splits.size();
-  p->n_slots = BS.nslots;
+  p->n_mitems = (int)BM.items.size();
+  p->n_splits = (int)(p->mfma ? BM.splits.size() : BS.splits.size());
+  p->n_slots = p->mfma ? BM.nslots : BS.nslots;
   if (fde_ext && gh_ext) {
     p->own_ws = false;
     p->fde = fde_ext;
@@ -2733,6 +3119,83 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
         std::sort(want[pt].begin(), want[pt].end());
         std::sort(got[pt].begin(), got[pt].end());
         if (want[pt] != got[pt]) return bad("scatter taps of a point differ from its entries' taps");
+      }
+    }
+  }
+  {  // patch runs of scatter_mfma: every tile point past column n aside, the (entry, tap) pairs the
+     // kernel's three runs and window test give a point equal a direct enumeration of the taps
+    MBinning B;
+    build_scatter_patches(G, B);
+    const std::vector<int2> ms = trim_scatter(B, n);
+    if (ms.size() != ss.size()) return bad("patch-run support size");
+    for (size_t c = 0; c < ms.size(); ++c)
+      if (ms[c].x != ss[c].x || ms[c].y != ss[c].y) return bad("patch-run support differs from the list support");
+    const int ntx = n / STS + 1, nty = 2 * n / STS, twon = 2 * n;
+    std::vector<int> wr((size_t)ntx * nty, 0);
+    for (const MItem& it : B.items)
+      if (it.slot < 0) ++wr[(size_t)((it.y0 + n) / STS) * ntx + it.x0 / STS];
+    for (const SplitTile& t : B.splits) ++wr[(size_t)((t.y0 + n) / STS) * ntx + t.x0 / STS];
+    for (int c = 0; c <= n; ++c)
+      for (int l = ms[c >> 2].x; l < ms[c >> 2].y; ++l)
+        if (wr[(size_t)((l + n) / STS) * ntx + c / STS] != 1) return bad("support row not written by the patch runs");
+    std::vector<std::vector<int>> want(SNP), got(SNP);
+    for (const MItem& it : B.items) {
+      if (it.ne > ME_MAX) return bad("patch-run entries per item");
+      if (it.eoff < 0 || it.eoff + it.ne > (int)B.entries.size() || B.rows.size() != B.entries.size())
+        return bad("patch-run entry offset");
+      if (it.cls[0] != 0 || it.cls[MCLS] != it.ne) return bad("anchor class offsets");
+      for (int k = 0; k < MCLS; ++k)
+        if (it.cls[k] > it.cls[k + 1]) return bad("anchor class offsets not ascending");
+      for (auto& v : want) v.clear();
+      for (auto& v : got) v.clear();
+      std::vector<int> code_of(it.ne, -1);  // LDS row -> entry code
+      for (int j = 0; j < it.ne; ++j) {
+        const int r = B.rows[it.eoff + j];
+        if (r >= it.ne || code_of[r] >= 0) return bad("patch-run row map");
+        if (j > 0 && B.entries[it.eoff + j] <= B.entries[it.eoff + j - 1])
+          return bad("patch-run staging order");
+        code_of[r] = B.entries[it.eoff + j];
+      }
+      for (int e = 0; e < it.ne; ++e) {
+        const int code = code_of[e];
+        int rx, ry;
+        G.centre(code >> 1, rx, ry);
+        const int cx = (code & 1) ? -rx : rx, cy = (code & 1) ? -ry : ry;
+        for (int i1 = 0; i1 < TAPS; ++i1) {
+          const int lr = wrapc(cy - M + i1, n) - it.y0;
+          if (lr < 0 || lr >= STS) continue;
+          for (int i0 = 0; i0 < TAPS; ++i0) {
+            const int raw = pmod(cx - M + i0, twon), lc = raw - it.x0;
+            if (raw <= n && lc >= 0 && lc < STS) want[lr * STS + lc].push_back((e * TAPS + i1) * TAPS + i0);
+          }
+        }
+        int bx, by;
+        window_base(G, code, it.x0, it.y0, bx, by);
+        const int cls = ((by + 8) >> 2) * 6 + ((bx + 8) >> 2);
+        if (bx < -8 || bx >= STS || by < -8 || by >= STS || e < it.cls[cls] || e >= it.cls[cls + 1])
+          return bad("entry outside its anchor class");
+      }
+      // the kernel: patch (px, py), runs [cls[ay 6 + px], cls[ay 6 + px + 3]) for ay = py .. py + 2,
+      // tap (tx, ty) = point - (bx + M, by + M) counted iff max(|tx|, |ty|) <= M
+      for (int P = 0; P < 16; ++P) {
+        const int px = P & 3, py = P >> 2;
+        for (int ay = py; ay < py + 3; ++ay)
+          for (int e = it.cls[ay * 6 + px]; e < it.cls[ay * 6 + px + 3]; ++e) {
+            int bx, by;
+            window_base(G, code_of[e], it.x0, it.y0, bx, by);
+            for (int q = 0; q < 16; ++q) {
+              const int qx = 4 * px + (q & 3), qy = 4 * py + (q >> 2);
+              const int tx = qx - (bx + M), ty = qy - (by + M);
+              if (std::max(std::abs(tx), std::abs(ty)) <= M)
+                got[qy * STS + qx].push_back((e * TAPS + (ty + M)) * TAPS + (tx + M));
+            }
+          }
+      }
+      for (int pt = 0; pt < SNP; ++pt) {
+        if (it.x0 + pt % STS > n) continue;  // never written
+        std::sort(want[pt].begin(), want[pt].end());
+        std::sort(got[pt].begin(), got[pt].end());
+        if (want[pt] != got[pt]) return bad("patch-run taps of a point differ from its entries' taps");
       }
     }
   }

@@ -23,14 +23,16 @@ class TomoReal:
     Parameters mirror processing/tomo.py:30: ``n`` detector width (= object
     width), ``nz`` the maximum number of slices per call, ``theta`` the
     projection angles in radians (stored as float32, like the reference).
-    ``fp64_tiles`` accumulates the adjoint scatter tiles in FP64, ``kahan_tiles`` in FP32 with Kahan
-    compensation (at most one of the two).
+    The adjoint scatter runs as dense 4 x 4-patch products on the matrix cores (FP32) unless one of
+    ``fp64_tiles`` (tiles accumulate in FP64), ``kahan_tiles`` (FP32 with Kahan compensation) or
+    ``list_tiles`` (FP32 on the vector ALUs from per-point tap lists) is set (at most one of them).
     ``gather_zp`` (1 or 2) is the number of z slices the forward gather stages
     per patch element for tiles of width <= 32 (2, the default, is the faster
     variant on gfx950; wider tiles always stage one slice).
     """
 
-    def __init__(self, n, nz, theta, fp64_tiles: bool = False, gather_zp: int = 2, kahan_tiles: bool = False):
+    def __init__(self, n, nz, theta, fp64_tiles: bool = False, gather_zp: int = 2, kahan_tiles: bool = False,
+                 list_tiles: bool = False):
         require_gpu()
         self._lib = _lib.load()
         self.n = int(n)
@@ -42,8 +44,10 @@ class TomoReal:
         plan = C.c_void_p()
         if gather_zp not in (1, 2):
             raise ValueError(f"TomoReal: gather_zp must be 1 or 2, got {gather_zp}")
+        if int(fp64_tiles) + int(kahan_tiles) + int(list_tiles) > 1:
+            raise ValueError("TomoReal: at most one of fp64_tiles, kahan_tiles, list_tiles")
         flags = ((_lib.MM_RADON_FP64_TILES if fp64_tiles else 0) | (_lib.MM_RADON_GATHER_ZP1 if gather_zp == 1 else 0)
-                 | (_lib.MM_RADON_KAHAN_TILES if kahan_tiles else 0))
+                 | (_lib.MM_RADON_KAHAN_TILES if kahan_tiles else 0) | (_lib.MM_RADON_LIST_TILES if list_tiles else 0))
         _lib.check(self._lib.mm_radon_create(C.byref(plan), self.n, self._nz, self.ntheta,
                                              th.ctypes.data_as(C.POINTER(C.c_double)), flags),
                    "TomoReal")

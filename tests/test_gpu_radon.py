@@ -255,19 +255,27 @@ def test_large_streams_z_slabs_bitwise(tomo_cls, cplx):
     assert np.array_equal(got, t.RT(torch.from_numpy(sino).cuda()).cpu().numpy())
 
 
-@pytest.mark.parametrize("n,nt,nz", [(256, 384, 2), (2048, 256, 1)])
+@pytest.mark.parametrize("n,nt,nz", [(256, 384, 2), (2048, 256, 1), (64, 24, 17), (96, 200, 33)])
 def test_scatter_accumulators_vs_fp64(tomo_cls, n, nt, nz):
-    """RT with the scatter tiles accumulated in FP32, compensated FP32 (Kahan) and FP64, each vs the FP64 oracle:
-    all within the 1e-5 bar, the compensated and FP64 sums no worse than plain FP32 (beyond rounding noise)."""
+    """RT with the scatter as dense patch products on the matrix cores (default), as FP32 per-point lists on the
+    vector ALUs, as compensated FP32 (Kahan) lists and as FP64 lists, each vs the FP64 oracle: all within the 1e-5
+    bar, the compensated and FP64 sums no worse than plain FP32 lists (beyond rounding noise), the matrix-core
+    result within FP32 rounding of the list result (nz = 17, 33: partial staging rounds of 16 slices)."""
     th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
     y = np.random.default_rng(n + 1).standard_normal((nt, nz, n)).astype(np.float32)
     ref = usfft.tomo_RT(y, th, dtype=np.float64)
-    errs = {}
-    for name, kw in (("fp32", {}), ("kahan", {"kahan_tiles": True}), ("fp64", {"fp64_tiles": True})):
-        got = tomo_cls(n, nz, th, **kw).RT(torch.from_numpy(y).cuda()).cpu().numpy()
-        errs[name] = rel(got, ref)
-    print(f"RT n={n} ntheta={nt}: rel-L2 vs FP64 " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    errs, outs = {}, {}
+    for name, kw in (("mfma", {}), ("lists", {"list_tiles": True}), ("kahan", {"kahan_tiles": True}),
+                     ("fp64", {"fp64_tiles": True})):
+        outs[name] = tomo_cls(n, nz, th, **kw).RT(torch.from_numpy(y).cuda()).cpu().numpy()
+        errs[name] = rel(outs[name], ref)
+    d = rel(outs["mfma"], outs["lists"])
+    print(f"RT n={n} ntheta={nt} nz={nz}: rel-L2 vs FP64 " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items())
+          + f"; matrix cores vs lists {d:.2e}")
     assert max(errs.values()) < 1e-5
-    assert errs["kahan"] <= 1.05 * errs["fp32"] and errs["fp64"] <= 1.05 * errs["fp32"]
-    with pytest.raises(ValueError):
-        tomo_cls(64, 1, th[:8], fp64_tiles=True, kahan_tiles=True)
+    assert errs["kahan"] <= 1.05 * errs["lists"] and errs["fp64"] <= 1.05 * errs["lists"]
+    assert d < 1e-6
+    for bad in ({"fp64_tiles": True, "kahan_tiles": True}, {"list_tiles": True, "kahan_tiles": True},
+                {"list_tiles": True, "fp64_tiles": True}):
+        with pytest.raises(ValueError):
+            tomo_cls(64, 1, th[:8], **bad)

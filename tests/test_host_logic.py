@@ -55,7 +55,8 @@ def test_radon_plan_tables_match_kernel_indexing(n):
     """Host-only: every gather staging offset lies inside a half-spectrum plane (tiles at the right
     edge of small grids reach past 2n and must wrap), every patch element is staged once, every tap
     read stays inside the LDS patch, every scatter slot names a staged entry, every written scatter point gets exactly
-    the taps of its item's entries that land on it (lists decoded as the kernel does, against a direct enumeration),
+    the taps of its item's entries that land on it (lists decoded as the kernel does, against a direct enumeration;
+    likewise the matrix-core scatter's anchor-class runs and window test, with its row map and sample-order staging),
     the column passes cover every row the gather reads and read no row the scatter leaves unwritten (dense and sparse
     angle sets)."""
     import ctypes as C
