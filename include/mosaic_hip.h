@@ -46,9 +46,11 @@ int mm_version(void);
  * (R :344, RT :582).  eps = 1e-3, mu = -ln(eps)/(2 n^2), m = 4 taps each side
  * (processing/tomo.py:39-42), 2x oversampled Fourier grid.
  *
- * mm_radon_create: n = 2^k in [16, 16384] or 3 * 2^k in [96, 12288], nz_max >= 1,
+ * mm_radon_create: n = 2^k in [16, 32768] or 3 * 2^k in [96, 24576], nz_max >= 1,
  *   (above 8192 the 2n-point row and column passes run as two n-point halves,
- *   split by frequency parity forward and by output parity inverse),
+ *   split by frequency parity forward and by output parity inverse; above 16384
+ *   as four n/2-point quarters and the n-point lines as two halves, through a
+ *   second buffer of the n object rows; mm_radon_adj_fbp is not built there),
  *   ntheta >= 1 angles in radians (theta_host, host memory, ntheta doubles;
  *   the reference stores theta as float32 -- pass those values).
  *   flags: 0 or an OR of

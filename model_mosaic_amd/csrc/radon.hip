@@ -326,7 +326,9 @@ __host__ __device__ __forceinline__ int wrap_mod(int v, int m) {
   const int r = v % m;
   return r < 0 ? r + m : r;
 }
-__host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int idx, int n, int pe, int& soff,
+// soff: element offset inside one z plane of the half spectrum (unsigned: a plane of n = 32768 holds more
+// than 2^31 elements).
+__host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int idx, int n, int pe, unsigned& soff,
                                                            int& sdst) {
   const int pw = it.w + 2 * M, ph = it.h + 2 * M;
   const int pp = patch_pitch(it.w);
@@ -345,9 +347,9 @@ __host__ __device__ __forceinline__ void gather_stage_desc(const Item& it, int i
   const int k0 = wrap_mod(l0, twon), k1 = wrap_mod(py0 + pr, twon);
   int fl = 0;
   if (k0 <= n) {
-    soff = (int)hs_idx(k1, k0, twon);
+    soff = (unsigned)hs_idx(k1, k0, twon);
   } else {
-    soff = (int)hs_idx(k1 == 0 ? 0 : twon - k1, twon - k0, twon);
+    soff = (unsigned)hs_idx(k1 == 0 ? 0 : twon - k1, twon - k0, twon);
     fl = 1;
   }
   sdst = ((pr * pp + pc) << 1) | fl;
@@ -369,7 +371,8 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
   const int pp = patch_pitch(it.w);  // LDS row pitch >= w + 2M
   // staging descriptors: source offset in the z plane; LDS slot << 1 | conj.
   // Idle slots read offset 0 and write the sink, so the loads issue together.
-  int soff[STG], sdst[STG];
+  unsigned soff[STG];
+  int sdst[STG];
 #pragma unroll
   for (int e = 0; e < STG; ++e) gather_stage_desc(it, threadIdx.x + e * NT, n, PE, soff[e], sdst[e]);
   // Lane threadIdx.x computes samples s_j = threadIdx.x + j NT (j < SPT).
@@ -1636,6 +1639,353 @@ __global__ __launch_bounds__(SplitShape<L>::NT) void rt4_rows_c2r_split(const fl
 }
 
 // ---------------------------------------------------------------------------
+// Detector widths above 16384 (n = 2 Q, Q <= 16384): every transform on Q points
+// ---------------------------------------------------------------------------
+// The 2n = 4Q-point rows and columns split four ways by decimation in frequency
+// (forward) or by output residue (inverse); the n = 2Q-point lines split two
+// ways, as the halves above.  With w = exp(-i pi / n) (the hsplit table,
+// w^j for j < n):
+//   forward, x centred in the 2n-point line (x_s at s + Q, s < n):
+//     Z[4m + r] = FFT_Q(y)[m],  y_s = w^(r s) ((-i)^r x_s + (-1)^r x_(s+Q)),  s < Q;
+//   inverse, kept points p = 4t + r in [Q, 3Q), i.e. t in [Q/4, 3Q/4):
+//     z[4t + r] = IFFT_Q(u)[t],  u_k = conj(w^(r k)) sum_q i^(r q) Z[k + q Q],  k < Q;
+//   lines, split by frequency parity e (forward) or output parity e (inverse):
+//     V[2m + e] = FFT_Q(w^(2 e s) (x_s + (-1)^e x_(s+Q)))[m],
+//     y[2t + e] = IFFT_Q(conj(w^(2 e k)) (Z[k] + (-1)^e Z[k+Q]))[t].
+// One workgroup per (line, residue): each reads its whole line, so the
+// residues of one column cannot share it in place -- R's row pass writes, and
+// RT's inverse column pass writes, a second half-spectrum buffer B holding only
+// the n object rows (B row s = padded row s + Q, block layout of n rows).  Rows
+// are transformed one real row at a time (the residues r and 4 - r of a packed
+// pair would need each other).  These passes read each input four times
+// (columns, rows) or twice (lines): they serve the size range, not speed.
+__device__ __forceinline__ float2 big_w(const float2* __restrict__ h, int j, int n) {  // w^j, 0 <= j < 2n
+  return j < n ? h[j] : make_float2(-h[j - n].x, -h[j - n].y);
+}
+__device__ __forceinline__ float2 mul_negi_pow(float2 x, int r) {  // (-i)^r x
+  switch (r & 3) {
+    case 0: return x;
+    case 1: return make_float2(x.y, -x.x);
+    case 2: return make_float2(-x.x, -x.y);
+    default: return make_float2(-x.y, x.x);
+  }
+}
+__device__ __forceinline__ float2 mul_i_pow(float2 x, int r) { return mul_negi_pow(x, (4 - (r & 3)) & 3); }  // i^r x
+__host__ __device__ __forceinline__ long long big_idx(int row, int col, int n) {  // B: blocks of 4 columns x n rows
+  return ((long long)(col >> 2) * n + row) * 4 + (col & 3);
+}
+
+// R's row pass: workgroup ((z, y), r) -> B row y, columns k = 4m + r <= n.
+template <int Q>
+__global__ __launch_bounds__(SplitShape<Q>::NT) void big_r1_rows(const float* __restrict__ obj, float2* __restrict__ B,
+                                                               const float* __restrict__ phx,
+                                                               const float* __restrict__ phy,
+                                                               const float2* __restrict__ h, long long bzs,
+                                                               const float2* __restrict__ tw) {
+  using Sh = SplitShape<Q>;
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int n = 2 * Q;
+  const long long u = blockIdx.x;  // the grid is exactly nz * n * 4 workgroups
+  const int r = (int)(u & 3);
+  const long long zy = u >> 2;
+  const int z = (int)(zy / n), y = (int)(zy - (long long)z * n);
+  const float* row = obj + ((long long)z * n + y) * n;
+  const float py = phy[y];
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int j = t + m * Sh::TPC;
+    const float x0 = row[j] * py * phx[j], x1 = row[j + Q] * py * phx[j + Q];
+    const float2 a = mul_negi_pow(make_float2(x0, 0.f), r);
+    const float2 yv = make_float2(a.x + ((r & 1) ? -x1 : x1), a.y);
+    v[m] = r ? cmul(yv, big_w(h, r * j, n)) : yv;
+  }
+  fft_reg<Q, Sh::TPC, false>(v, lds, t, tw);
+  float2* b = B + (long long)z * bzs;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int k = 4 * (t + m * Sh::TPC) + r;
+    if (k <= n) b[big_idx(y, k, n)] = v[m];
+  }
+}
+
+// Column passes.  Forward: B column col (the n object rows) -> the rows of the
+// disc (supp) of fde column col, raw frequency order.  Inverse: the disc's rows
+// of fde column col -> B rows s = p - Q, unnormalised (the row pass applies
+// both 1/(2n)).  Workgroup (col, r), blockIdx.y = z.
+template <int Q, bool INV>
+__global__ __launch_bounds__(SplitShape<Q>::NT) void big_col(float2* __restrict__ fde, float2* __restrict__ B,
+                                                           int ncols, long long zs, long long bzs,
+                                                           const int2* __restrict__ supp,
+                                                           const float2* __restrict__ h,
+                                                           const float2* __restrict__ tw) {
+  using Sh = SplitShape<Q>;
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int n = 2 * Q, L2 = 4 * Q;
+  const int col = (int)(blockIdx.x >> 2), r = (int)(blockIdx.x & 3);
+  if (col >= ncols) return;  // the whole workgroup, before any barrier
+  float2* fb = fde + (long long)blockIdx.y * zs + (long long)(col >> 2) * (4LL * L2) + (col & 3);  // row k: fb[4 k]
+  float2* bb = B + (long long)blockIdx.y * bzs + (long long)(col >> 2) * (4LL * n) + (col & 3);    // row s: bb[4 s]
+  const int2 sp = supp[col >> 2];
+  auto carries = [&](int k) {
+    const int l = k < n ? k : k - L2;
+    return l >= sp.x && l < sp.y;
+  };
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+  if constexpr (!INV) {
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int j = t + m * Sh::TPC;
+      const float2 x0 = bb[4LL * j], x1 = bb[4LL * (j + Q)];
+      const float2 a = mul_negi_pow(x0, r);
+      const float2 yv = (r & 1) ? make_float2(a.x - x1.x, a.y - x1.y) : make_float2(a.x + x1.x, a.y + x1.y);
+      v[m] = r ? cmul(yv, big_w(h, r * j, n)) : yv;
+    }
+    fft_reg<Q, Sh::TPC, false>(v, lds, t, tw);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int k = 4 * (t + m * Sh::TPC) + r;
+      if (carries(k)) fb[4LL * k] = v[m];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int k = t + m * Sh::TPC;
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int kk = k + qq * Q;
+        if (carries(kk)) {
+          const float2 zq = mul_i_pow(fb[4LL * kk], r * qq);
+          acc.x += zq.x;
+          acc.y += zq.y;
+        }
+      }
+      v[m] = r ? cmul(acc, cconj(big_w(h, r * k, n))) : acc;
+    }
+    constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4;  // t in [Q/4, 3Q/4): p = 4t + r in [Q, 3Q)
+    static_assert(Sh::PT % 4 == 0, "Q/4 is a multiple of TPC");
+    fft_reg<Q, Sh::TPC, true>(v, lds, t, tw);
+#pragma unroll
+    for (int m = M0; m < M1; ++m) bb[4LL * (4 * (t + m * Sh::TPC) + r - Q)] = v[m];
+  }
+}
+
+// RT's row pass: C2R of the 2n-point rows from B (Hermitian half rows k <= n,
+// imaginary parts dropped at k = 0 and k = n), kept points p = 4t + r in
+// [Q, 3Q) to columns p - Q, times phi.  A real output packs rows y, y + 1 as
+// Z = A + i B; a complex one takes Z = Hre + i Him of row y.
+template <int Q, bool CPLX>
+__global__ __launch_bounds__(SplitShape<Q>::NT) void big_rows_c2r(const float2* __restrict__ Bre,
+                                                                const float2* __restrict__ Bim,
+                                                                void* __restrict__ out_v,
+                                                                const float* __restrict__ phx,
+                                                                const float* __restrict__ phy,
+                                                                const float2* __restrict__ h, long long bzs,
+                                                                float scale, const float2* __restrict__ tw) {
+  using Sh = SplitShape<Q>;
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int n = 2 * Q, L2 = 4 * Q;
+  const long long u = blockIdx.x >> 2;  // the grid is exactly units * 4 workgroups
+  const int r = (int)(blockIdx.x & 3);
+  int z, y;
+  if (CPLX) {
+    z = (int)(u / n);
+    y = (int)(u - (long long)z * n);
+  } else {
+    z = (int)(u / Q);
+    y = 2 * (int)(u - (long long)z * Q);
+  }
+  const float2* pa = Bre + (long long)z * bzs + 4LL * y;
+  const float2* pb = CPLX ? Bim + (long long)z * bzs + 4LL * y : pa + 4;
+  auto at = [&](const float2* row, int col) { return row[(long long)(col >> 2) * (4LL * n) + (col & 3)]; };
+  auto spec = [&](const float2* row, int k) {  // Hermitian extension of the half row to k in [0, 2n)
+    if (k == 0 || k == n) return make_float2(at(row, k).x, 0.f);
+    return k < n ? at(row, k) : cconj(at(row, L2 - k));
+  };
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int k = t + m * Sh::TPC;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const float2 a = spec(pa, k + qq * Q), b = spec(pb, k + qq * Q);
+      const float2 zq = mul_i_pow(make_float2(a.x - b.y, a.y + b.x), r * qq);  // i^(r q) (a + i b)
+      acc.x += zq.x;
+      acc.y += zq.y;
+    }
+    v[m] = r ? cmul(acc, cconj(big_w(h, r * k, n))) : acc;
+  }
+  fft_reg<Q, Sh::TPC, true>(v, lds, t, tw);
+  constexpr int M0 = Sh::PT / 4, M1 = 3 * Sh::PT / 4;
+  const float pya = phy[y] * scale, pyb = CPLX ? 0.f : phy[y + 1] * scale;
+#pragma unroll
+  for (int m = M0; m < M1; ++m) {
+    const int xx = 4 * (t + m * Sh::TPC) + r - Q;
+    const float px = phx[xx];
+    if (CPLX) {
+      const float pw = pya * px;
+      ((float2*)out_v)[((long long)z * n + y) * n + xx] = make_float2(v[m].x * pw, v[m].y * pw);
+    } else {
+      float* o = (float*)out_v + ((long long)z * n + y) * n;
+      o[xx] = v[m].x * (pya * px);
+      o[n + xx] = v[m].y * (pyb * px);
+    }
+  }
+}
+
+// R's line pass on n = 2Q points: rows (theta, z), (theta + 1, z) as one complex
+// transform (r4_line_ifft), output parity e = blockIdx.x & 1.
+template <int Q>
+__global__ __launch_bounds__(SplitShape<Q>::NT) void big_line_ifft(const float2* __restrict__ gh, float* __restrict__ sino,
+                                                                 const int* __restrict__ sig, int H, long long nrows,
+                                                                 int nz, float scale, const float2* __restrict__ h,
+                                                                 const float2* __restrict__ tw) {
+  using Sh = SplitShape<Q>;
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int L = 2 * Q, half = Q;
+  const int e = (int)(blockIdx.x & 1);
+  const long long q = blockIdx.x >> 1, th2 = q / nz;
+  const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
+  const bool va = ra < nrows, vb = rb < nrows;
+  const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+  const float2* srca = gh + (va ? ra : 0) * H;
+  const float2* srcb = gh + (vb ? rb : 0) * H;
+  auto rebuild = [&](const float2* src, int sg, bool valid, int r) {
+    const int d = r - half;
+    int kk;
+    bool cj;
+    if (sg > 0) {
+      kk = d >= 0 ? d : -d;
+      cj = d < 0;
+    } else {
+      kk = d <= 0 ? -d : d;
+      cj = d > 0;
+    }
+    float2 w = valid ? src[kk] : make_float2(0.f, 0.f);
+    if (cj) w.y = -w.y;
+    if (r == 0 || r == half) w.y = 0.f;  // Hermitian part
+    return w;
+  };
+  auto line = [&](int r) {  // c1d . (a + i b) at r
+    const float2 a = rebuild(srca, sga, va, r), b = rebuild(srcb, sgb, vb, r);
+    const float2 w = make_float2(a.x - b.y, a.y + b.x);
+    return (r & 1) ? w : make_float2(-w.x, -w.y);
+  };
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int k = t + m * Sh::TPC;
+    const float2 lo = line(k), hi = line(k + Q);
+    const float2 sum = e ? make_float2(lo.x - hi.x, lo.y - hi.y) : make_float2(lo.x + hi.x, lo.y + hi.y);
+    v[m] = e ? cmul(sum, cconj(h[2 * k])) : sum;
+  }
+  fft_reg<Q, Sh::TPC, true>(v, lds, t, tw);
+  float* dsta = sino + (va ? ra : 0) * L;
+  float* dstb = sino + (vb ? rb : 0) * L;
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int s = 2 * (t + m * Sh::TPC) + e;
+    const float sc = (s & 1) ? scale : -scale;
+    if (va) dsta[s] = v[m].x * sc;
+    if (vb) dstb[s] = v[m].y * sc;
+  }
+}
+
+// RT's line pass on n = 2Q points (rt1_line_fft), frequency parity e =
+// blockIdx.x & 1: the representatives k of parity e use V[r], V[L - r] with
+// r = L/2 +- k of the same parity, all in this workgroup's half.
+template <int Q, bool CPLX>
+__global__ __launch_bounds__(SplitShape<Q>::NT) void big_line_fft(const void* __restrict__ sino_v,
+                                                                float2* __restrict__ yre, float2* __restrict__ yim,
+                                                                const int* __restrict__ sig, int H, long long nrows,
+                                                                int nz, float scale, const float2* __restrict__ h,
+                                                                const float2* __restrict__ tw) {
+  using Sh = SplitShape<Q>;
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int L = 2 * Q, half = Q;
+  const int e = (int)(blockIdx.x & 1);
+  const long long q = blockIdx.x >> 1;
+  long long ra, rb;
+  bool va, vb;
+  if (CPLX) {
+    ra = q;
+    rb = 0;
+    va = ra < nrows;
+    vb = false;
+  } else {
+    const long long th2 = q / nz;
+    ra = th2 * 2 * nz + (q - th2 * nz);
+    rb = ra + nz;
+    va = ra < nrows;
+    vb = rb < nrows;
+  }
+  const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+  auto line = [&](int j) {  // c1d . y at j (real rows: a + i b)
+    float2 w;
+    if (CPLX) {
+      w = va ? ((const float2*)sino_v)[ra * L + j] : make_float2(0.f, 0.f);
+    } else {
+      const float* sv = (const float*)sino_v;
+      w = make_float2(va ? sv[ra * L + j] : 0.f, vb ? sv[rb * L + j] : 0.f);
+    }
+    return (j & 1) ? w : make_float2(-w.x, -w.y);
+  };
+  const int t = threadIdx.x;
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int j = t + m * Sh::TPC;
+    const float2 lo = line(j), hi = line(j + Q);
+    const float2 sum = e ? make_float2(lo.x - hi.x, lo.y - hi.y) : make_float2(lo.x + hi.x, lo.y + hi.y);
+    v[m] = e ? cmul(sum, h[2 * j]) : sum;
+  }
+  fft_reg<Q, Sh::TPC, false>(v, lds, t, tw);
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) lds[lpad(t + m * Sh::TPC)] = v[m];  // V[2 m' + e] at m'
+  __syncthreads();
+  if (!va) return;  // no barrier below
+  auto yhat = [&](int r) {  // c1d . V at r (r of parity e)
+    const float2 w = lds[lpad(r >> 1)];
+    return (r & 1) ? w : make_float2(-w.x, -w.y);
+  };
+  for (int k = 2 * t + e; k < H; k += 2 * Sh::TPC) {
+    const float f = ((k == 0 || k == half) ? 0.5f : 1.0f) * scale;
+    if (k == half) {  // from r = 0 for either sign (half is even: e = 0)
+      const float2 v0 = yhat(0);
+      if (CPLX) {
+        yre[ra * H + k] = cscale(sga > 0 ? cconj(v0) : v0, f);
+        yim[ra * H + k] = cscale(sga > 0 ? make_float2(v0.y, v0.x) : make_float2(v0.y, -v0.x), f);
+      } else {
+        yre[ra * H + k] = make_float2(v0.x * f, 0.f);
+        if (vb) yre[rb * H + k] = make_float2(v0.y * f, 0.f);
+      }
+      continue;
+    }
+    {
+      const int r = sga > 0 ? half + k : half - k;
+      const float2 p = yhat(r), pm = yhat(L - r);
+      if (CPLX) {
+        yre[ra * H + k] = cscale(make_float2(0.5f * (p.x + pm.x), 0.5f * (p.y - pm.y)), f);
+        yim[ra * H + k] = cscale(make_float2(0.5f * (p.y + pm.y), -0.5f * (p.x - pm.x)), f);
+      } else {
+        yre[ra * H + k] = make_float2(0.5f * (p.x + pm.x) * f, 0.5f * (p.y - pm.y) * f);
+      }
+    }
+    if (!CPLX && vb) {
+      const int r = sgb > 0 ? half + k : half - k;
+      const float2 p = yhat(r), pm = yhat(L - r);
+      yre[rb * H + k] = make_float2(0.5f * (p.y + pm.y) * f, -0.5f * (p.x - pm.x) * f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host side: geometry, binning, launches
 // ---------------------------------------------------------------------------
 
@@ -2261,6 +2611,13 @@ static std::vector<int2> trim_scatter_items(std::vector<ItemT>& all, std::vector
 static std::vector<int2> trim_scatter(SBinning& B, int n) { return trim_scatter_items(B.items, &B.perm, n); }
 static std::vector<int2> trim_scatter(MBinning& B, int n) { return trim_scatter_items(B.items, nullptr, n); }
 
+// Detector widths: 2^k in [16, 32768] or 3 * 2^k in [96, 24576].  Up to 8192 the 2n-point passes take one
+// workgroup per transform, up to 16384 two n-point halves, above that four n/2-point quarters (big_*).
+static bool radon_width_ok(int n) {
+  if (!((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96))) return false;
+  return n <= kMaxFFT || (n <= 2 * kMaxFFT && fft_len_ok(n / 2) && pt_of(n / 2) % 4 == 0);
+}
+
 template <typename T>
 static int upload(T** dst, const std::vector<T>& src, size_t& bytes) {
   *dst = nullptr;
@@ -2304,6 +2661,10 @@ struct mm_radon {
   int2* ssupp = nullptr;  // per 4-column block: centred rows the scatter writes
   float2* hsplit = nullptr;  // exp(-i pi j / n), j < n: the 2n-point passes run as two n-point halves (split)
   bool split = false;        // 2n > kMaxFFT
+  bool big = false;          // n > kMaxFFT: four-way 2n passes through bbuf (big_*)
+  float2* bbuf = nullptr;    // nz_max x bzs: the n object rows of the half spectrum (big)
+  float2* bbuf2 = nullptr;   // second channel (complex sinograms, allocated on first use)
+  long long bzs = 0;         // elements per z plane of bbuf
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
   int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   // adjoint scatter: patch runs on the matrix cores (FP32, default) or per-point lists (FP64 / Kahan tiles)
@@ -2342,7 +2703,7 @@ static void radon_free(mm_radon* p) {
     for (auto& x : e)
       if (x) (void)hipEventDestroy(x);
   if (!p->own_ws) p->fde = p->gh = nullptr;
-  void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->phx, p->phy, p->dirs, p->sig,
+  void* ptrs[] = {p->fde, p->fde2, p->gh, p->gh2, p->partial, p->bbuf, p->bbuf2, p->phx, p->phy, p->dirs, p->sig,
                   p->gitems, p->gsamples, p->gorder, p->sitems, p->sperm, p->sentries, p->slists,
                   p->mitems, p->mentries, p->mrows, p->splits, p->gsupp, p->ssupp, p->hsplit};
   for (void* q : ptrs)
@@ -2481,6 +2842,54 @@ static void launch_rt4_split(const mm_radon* p, void* out, bool cplx, int nz, hi
     default: return ::mm::fail(MM_ERR_UNSUPPORTED, "split 2n transforms not built for n=" + std::to_string(N_RUNTIME)); \
   }
 
+// Runs CALL with constexpr int QQ = n / 2 for the detector widths above 16384 (big_*).
+#define MM_DISPATCH_BIG(N_RUNTIME, CALL)                                                              \
+  switch ((N_RUNTIME) / 2) {                                                                          \
+    case 12288: { constexpr int QQ = 12288; CALL; } break;                                            \
+    case 16384: { constexpr int QQ = 16384; CALL; } break;                                            \
+    default: return ::mm::fail(MM_ERR_UNSUPPORTED, "four-way transforms not built for n=" + std::to_string(N_RUNTIME)); \
+  }
+
+template <int Q>
+static void launch_big_r1(const mm_radon* p, const float* obj, int nz, hipStream_t st) {
+  hipLaunchKernelGGL(big_r1_rows<Q>, dim3((unsigned)(4LL * nz * 2 * Q)), dim3(SplitShape<Q>::NT), 0, st, obj, p->bbuf,
+                     p->phx, p->phy, p->hsplit, p->bzs, p->tw + tw_offset(Q));
+}
+template <int Q, bool INV>
+static void launch_big_col(const mm_radon* p, float2* fde, float2* B, int nz, hipStream_t st) {
+  const int ncols = 2 * Q + 1;
+  hipLaunchKernelGGL((big_col<Q, INV>), dim3((unsigned)(4 * ncols), (unsigned)nz), dim3(SplitShape<Q>::NT), 0, st, fde,
+                     B, ncols, p->zs, p->bzs, INV ? p->ssupp : p->gsupp, p->hsplit, p->tw + tw_offset(Q));
+}
+template <int Q>
+static void launch_big_c2r(const mm_radon* p, void* out, bool cplx, int nz, hipStream_t st) {
+  const long long units = cplx ? (long long)nz * 2 * Q : (long long)nz * Q;
+  const double n = 2.0 * Q;
+  const float scale = (float)(1.0 / (4.0 * n * n));  // column and row inverse passes, 1/(2n) each
+  if (cplx)
+    hipLaunchKernelGGL((big_rows_c2r<Q, true>), dim3((unsigned)(4 * units)), dim3(SplitShape<Q>::NT), 0, st, p->bbuf,
+                       p->bbuf2, out, p->phx, p->phy, p->hsplit, p->bzs, scale, p->tw + tw_offset(Q));
+  else
+    hipLaunchKernelGGL((big_rows_c2r<Q, false>), dim3((unsigned)(4 * units)), dim3(SplitShape<Q>::NT), 0, st, p->bbuf,
+                       p->bbuf2, out, p->phx, p->phy, p->hsplit, p->bzs, scale, p->tw + tw_offset(Q));
+}
+template <int Q>
+static void launch_big_r4(const mm_radon* p, float* sino, int nz, hipStream_t st) {
+  const long long rows = (long long)p->ntheta * nz, pairs = (long long)((p->ntheta + 1) / 2) * nz;
+  hipLaunchKernelGGL(big_line_ifft<Q>, dim3((unsigned)(2 * pairs)), dim3(SplitShape<Q>::NT), 0, st, p->gh, sino, p->sig,
+                     p->H, rows, nz, 1.0f / (4.0f * p->n), p->hsplit, p->tw + tw_offset(Q));
+}
+template <int Q>
+static void launch_big_rt1(const mm_radon* p, const void* sino, bool cplx, int nz, hipStream_t st) {
+  const long long rows = (long long)p->ntheta * nz, pairs = (long long)((p->ntheta + 1) / 2) * nz;
+  if (cplx)
+    hipLaunchKernelGGL((big_line_fft<Q, true>), dim3((unsigned)(2 * rows)), dim3(SplitShape<Q>::NT), 0, st, sino, p->gh,
+                       p->gh2, p->sig, p->H, rows, nz, p->C0s, p->hsplit, p->tw + tw_offset(Q));
+  else
+    hipLaunchKernelGGL((big_line_fft<Q, false>), dim3((unsigned)(2 * pairs)), dim3(SplitShape<Q>::NT), 0, st, sino,
+                       p->gh, p->gh2, p->sig, p->H, rows, nz, p->C0s, p->hsplit, p->tw + tw_offset(Q));
+}
+
 static int zsplit_for(int items, int nz) {
   if (items <= 0) return 1;
   int zs = (2048 + items - 1) / items;
@@ -2578,11 +2987,12 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
                       float2* fde_ext, float2* gh_ext, bool fwd_only) {
   MM_ARG_CHECK(plan != nullptr, "mm_radon_create: plan is NULL");
   *plan = nullptr;
-  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && n <= kMaxFFT,
-               "mm_radon_create: n must be 2^k in [16, 16384] or 3 * 2^k in [96, 12288], got " + std::to_string(n));
+  MM_ARG_CHECK(radon_width_ok(n),
+               "mm_radon_create: n must be 2^k in [16, 32768] or 3 * 2^k in [96, 24576], got " + std::to_string(n));
   MM_ARG_CHECK(nz_max >= 1, "mm_radon_create: nz_max must be >= 1");
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
-  MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 31), "mm_radon_create: ntheta * (n/2+1) overflows int32");
+  MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 30),
+               "mm_radon_create: ntheta * (n/2+1) must stay below 2^30 (entry codes carry it shifted by one bit)");
   MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0,
                "mm_radon_create: unknown flags");
   MM_ARG_CHECK(__builtin_popcount(flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) <= 1,
@@ -2612,6 +3022,10 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   p->C0g = (float)(M_PI / mu / n);
   p->C0s = (float)(M_PI / mu);
   p->split = 2 * n > kMaxFFT;
+  p->big = n > kMaxFFT;
+  p->bzs = (long long)p->NCB * n * 4;
+  if (p->big && (fde_ext || gh_ext))
+    return bail(fail(MM_ERR_UNSUPPORTED, "mm_radon_create: shared workspaces are not built for n > 16384"));
 
   // separable phi (processing/tomo.py:46-49,59): phi(y,x) = phy[y] * phx[x]
   std::vector<float> phx(n), phy(n);
@@ -2693,6 +3107,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
              (rc = dmalloc((void**)&p->gh, sizeof(float2) * (size_t)ntheta * nz_max * p->H, bytes, "representatives"))) {
     return bail(rc);
   }
+  if (p->big && (rc = dmalloc((void**)&p->bbuf, sizeof(float2) * (size_t)nz_max * p->bzs, bytes, "object rows")))
+    return bail(rc);
   if (p->n_slots > 0 &&
       (rc = dmalloc((void**)&p->partial, sizeof(float2) * (size_t)p->n_slots * nz_max * SNP, bytes, "partial tiles")))
     return bail(rc);
@@ -2708,20 +3124,28 @@ static int radon_fwd_chain(mm_radon* p, const float* obj, float* sino, int nz, h
   const int L2 = 2 * p->n, L1 = p->n;
   int rc;
   if ((rc = mark(p, 0, 0, st))) return rc;
-  if (p->split) {
+  if (p->big) {
+    MM_DISPATCH_BIG(L1, (launch_big_r1<QQ>(p, obj, nz, st)));
+  } else if (p->split) {
     MM_DISPATCH_SPLIT(L1, (launch_r1_split<LL>(p, obj, nz, st)));
   } else {
     MM_DISPATCH_FFT(L2, (launch_r1<LL>(p, obj, nz, st)));
   }
   if ((rc = check_launch("r1_rows_rfft")) || (rc = mark(p, 0, 1, st)) || (rc = mark(p, 1, 0, st))) return rc;
-  if (p->split) {
+  if (p->big) {
+    MM_DISPATCH_BIG(L1, (launch_big_col<QQ, false>(p, p->fde, p->bbuf, nz, st)));
+  } else if (p->split) {
     MM_DISPATCH_SPLIT(L1, (launch_col_split<LL, false>(p, p->fde, nz, st)));
   } else {
     MM_DISPATCH_FFT(L2, (launch_col<LL, false>(p, p->fde, nz, st)));
   }
   if ((rc = check_launch("col_fft (R)")) || (rc = mark(p, 1, 1, st)) || (rc = mark(p, 2, 0, st))) return rc;
   if ((rc = launch_gather(p, nz, st)) || (rc = mark(p, 2, 1, st)) || (rc = mark(p, 3, 0, st))) return rc;
-  MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
+  if (p->big) {
+    MM_DISPATCH_BIG(L1, (launch_big_r4<QQ>(p, sino, nz, st)));
+  } else {
+    MM_DISPATCH_FFT(L1, (launch_r4<LL>(p, sino, nz, st)));
+  }
   if ((rc = check_launch("r4_line_ifft"))) return rc;
   return mark(p, 3, 1, st);
 }
@@ -2827,11 +3251,20 @@ int mm_radon_adj(mm_radon* p, const void* sino, int sino_is_c64, void* obj, int 
         (rc = dmalloc((void**)&p->gh2, sizeof(float2) * (size_t)p->ntheta * p->nz_max * p->H, p->bytes, "representatives (im)")))
       return rc;
   }
+  if (cplx && p->big && !p->bbuf2) {
+    int rc;
+    if ((rc = dmalloc((void**)&p->bbuf2, sizeof(float2) * (size_t)p->nz_max * p->bzs, p->bytes, "object rows (im)")))
+      return rc;
+  }
   hipStream_t st = (hipStream_t)stream;
   const int L1 = p->n;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
-  MM_DISPATCH_FFT(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
+  if (p->big) {
+    MM_DISPATCH_BIG(L1, (launch_big_rt1<QQ>(p, sino, cplx, nz, st)));
+  } else {
+    MM_DISPATCH_FFT(L1, (launch_rt1<LL>(p, sino, cplx, nz, st)));
+  }
   if ((rc = check_launch("rt1_line_fft")) || (rc = mark(p, 4, 1, st))) return rc;
   return radon_adj_tail(p, cplx, obj, nz, st);
 }
@@ -2846,7 +3279,9 @@ static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t
   if ((rc = mark(p, 5, 1, st)) || (rc = mark(p, 6, 0, st))) return rc;
   for (int ch = 0; ch < (cplx ? 2 : 1); ++ch) {
     float2* buf = ch ? p->fde2 : p->fde;
-    if (p->split) {
+    if (p->big) {
+      MM_DISPATCH_BIG(L1, (launch_big_col<QQ, true>(p, buf, ch ? p->bbuf2 : p->bbuf, nz, st)));
+    } else if (p->split) {
       MM_DISPATCH_SPLIT(L1, (launch_col_split<LL, true>(p, buf, nz, st)));
     } else {
       MM_DISPATCH_FFT(L2, (launch_col<LL, true>(p, buf, nz, st)));
@@ -2854,7 +3289,9 @@ static int radon_adj_tail(mm_radon* p, bool cplx, void* obj, int nz, hipStream_t
     if ((rc = check_launch(ch ? "col_fft (RT im)" : "col_fft (RT)"))) return rc;
   }
   if ((rc = mark(p, 6, 1, st)) || (rc = mark(p, 7, 0, st))) return rc;
-  if (p->split) {
+  if (p->big) {
+    MM_DISPATCH_BIG(L1, (launch_big_c2r<QQ>(p, obj, cplx, nz, st)));
+  } else if (p->split) {
     MM_DISPATCH_SPLIT(L1, (launch_rt4_split<LL>(p, obj, cplx, nz, st)));
   } else {
     MM_DISPATCH_FFT(L2, (launch_rt4<LL>(p, obj, cplx, nz, st)));
@@ -2870,6 +3307,7 @@ int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float
                "mm_radon_adj_fbp: nz=" + std::to_string(nz) + " outside [1, " + std::to_string(p->nz_max) + "]");
   MM_ARG_CHECK(sino && G && A && B && obj, "mm_radon_adj_fbp: null pointer");
   if (p->fwd_only) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: forward-only plan (no scatter tables)");
+  if (p->big) return fail(MM_ERR_UNSUPPORTED, "mm_radon_adj_fbp: the fused filter is not built for n > 16384");
   hipStream_t st = (hipStream_t)stream;
   int rc;
   if ((rc = mark(p, 4, 0, st))) return rc;
@@ -2990,8 +3428,7 @@ int mm_radon_stage_ms(mm_radon* p, float* ms, int max_stages) {
 // every list slot naming a staged entry row).  Builds the tables like
 // mm_radon_create, touches no device.
 int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
-  MM_ARG_CHECK(((is_pow2(n) && n >= 16) || (is_fft_len(n) && n >= 96)) && n <= kMaxFFT,
-               "mm_radon_plan_check: unsupported n " + std::to_string(n));
+  MM_ARG_CHECK(radon_width_ok(n), "mm_radon_plan_check: unsupported n " + std::to_string(n));
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_plan_check: need ntheta >= 1 angles");
   HostGeom G;
   G.n = n;
@@ -3022,9 +3459,10 @@ int mm_radon_plan_check(int n, int ntheta, const double* theta_host) {
       if (pp < pw || ph * pp > PE) return bad("patch pitch");
       std::vector<int> cover(PE + 1, 0);
       for (int idx = 0; idx < nslots; ++idx) {
-        int so, sd;
+        unsigned so;
+        int sd;
         gather_stage_desc(it, idx, n, PE, so, sd);
-        if (so < 0 || so >= zs || (sd >> 1) > PE)
+        if ((long long)so >= zs || (sd >> 1) > PE)
           return bad("staging offset " + std::to_string(so) + " / slot " + std::to_string(sd >> 1) + " out of range");
         if ((sd >> 1) < PE) {  // a patch element: its row is one the forward column pass writes
           const int blk = (int)(so / (8LL * n)), row = (int)((so / 4) % (2 * n));

@@ -68,10 +68,11 @@ class TomoLargeReal:
     # ---------- sizing ------------------------------------------------------
     def _slice_bytes(self, cplx: bool) -> int:
         n, nt = self.n, self.ntheta
-        per = 2 * n * (n + 2) * 8 + nt * (n // 2 + 1) * 8   # plan workspace per slice
+        rows = n * (n + 4) * 8 if n > 16384 else 0           # widths above 16384: the buffer of object rows
+        per = 2 * n * (n + 2) * 8 + rows + nt * (n // 2 + 1) * 8   # plan workspace per slice
         per += 2 * (4 * n * n + 4 * nt * n)                  # obj + sino slabs, two of each when streamed
         if cplx:
-            per += 2 * n * (n + 2) * 8 + nt * (n // 2 + 1) * 8 + 2 * (4 * n * n + 4 * nt * n)
+            per += 2 * n * (n + 2) * 8 + rows + nt * (n // 2 + 1) * 8 + 2 * (4 * n * n + 4 * nt * n)
         return per
 
     def _zbatch(self, nz: int, cplx: bool = False) -> int:

@@ -57,7 +57,7 @@ def test_null_plan_is_an_argument_error(lib):
 def test_bad_sizes_rejected_before_device_work(lib):
     p = C.c_void_p()
     th = (C.c_double * 4)(0, 0.1, 0.2, 0.3)
-    for n in (100, 48, 160, 24576):   # not 2^k / 3 * 2^k, 3 * 2^k below 96, 5 * 2^k, n above the largest FFT
+    for n in (100, 48, 160, 49152, 65536):   # not 2^k / 3 * 2^k, 3 * 2^k below 96, 5 * 2^k, n above the largest widths
         rc = lib.mm_radon_create(C.byref(p), n, 1, 4, th, 0)
         assert rc == _lib.MM_ERR_ARG and not p, n
     for n, nz in ((64, 40), (20, 64), (64, 12)):   # 5 * 2^3, 5 * 2^2, 3 * 2^2 below 24

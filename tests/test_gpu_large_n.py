@@ -185,3 +185,56 @@ def test_rt_filtered_above_8192():
     t.free()
     print(f"RT_filtered n={n}: rel-L2 vs filter + RT {e:.2e}", flush=True)
     assert e < 1e-6
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("n", [24576, 32768])
+def test_radon_quarter_split_widths(n):
+    """Widths above 16384 run their 2n-point row and column passes as four n/2-point quarters (decimation in
+    frequency forward, output residue inverse) and their n-point lines as two halves, out of place through the
+    plan's buffer of object rows.  Checks on one slice and four angles (0, pi/4, pi/2, 3pi/4): the FP64 adjoint
+    identity, R at theta = 0 against the column sums and at theta = pi/2 against the reversed row sums (O(eps),
+    the window's design error), the complex RT's linearity and real part; at n = 24576 also R and RT vs the FP64
+    oracle (its padded 2n x 2n planes take ~60 GB of host memory; n = 32768 would take ~140 GB)."""
+    from model_mosaic_amd.processing.tomo import TomoReal
+    nt = 4
+    print(f"[large n] Radon n={n} ntheta={nt} nz=1 (four-way split)", flush=True)
+    th = _angles(nt)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((1, n, n), dtype=np.float32)
+    y = rng.standard_normal((nt, 1, n), dtype=np.float32)
+    t = TomoReal(n, 1, th)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    rd = t.R(xd)
+    bd = t.RT(yd)
+    torch.cuda.synchronize()
+    lhs, rhs = dot64(rd, yd), dot64(xd, bd)
+    cs = abs(lhs - rhs) / (float(rd.double().norm()) * float(yd.double().norm()))
+    colsum = xd[0].double().sum(dim=0) / math.sqrt(n * nt)
+    ec = float((rd[0, 0].double() - colsum).norm() / colsum.norm())
+    rowsum = xd[0].double().sum(dim=1) / math.sqrt(n * nt)
+    idx = (n - torch.arange(n, device="cuda")) % n
+    er2 = float((rd[2, 0].double() - rowsum[idx]).norm() / rowsum.norm())
+    g = torch.Generator(device="cuda").manual_seed(n)
+    a = torch.randn((nt, 1, n), generator=g, device="cuda")
+    b = torch.randn((nt, 1, n), generator=g, device="cuda")
+    zero = torch.zeros_like(a)
+    c = t.RT(torch.complex(a, b))
+    ca, cb = t.RT(torch.complex(a, zero)), t.RT(torch.complex(b, zero))
+    ra = t.RT(a)
+    ref = ca + 1j * cb
+    el = float((c - ref).abs().double().norm() / ref.abs().double().norm())
+    erp = float((ca.real - ra).double().norm() / ra.double().norm())
+    r, bb = rd.cpu().numpy(), bd.cpu().numpy()
+    t.free()
+    del rd, bd, xd, yd, a, b, c, ca, cb, ra, ref
+    torch.cuda.empty_cache()
+    print(f"n={n}: adjoint {cs:.2e}, theta=0 column sums {ec:.2e}, theta=pi/2 row sums {er2:.2e}, "
+          f"complex RT linearity {el:.2e}, real part {erp:.2e}", flush=True)
+    assert cs < 1e-6 and ec < 1.5e-3 and er2 < 1.5e-3 and el < 1e-6 and erp < 1e-6
+    if n <= 24576:
+        er = rel(r, usfft.tomo_R(x, th, np.float64))
+        del x
+        eb = rel(bb, usfft.tomo_RT(y, th, dtype=np.float64))
+        print(f"n={n}: R rel-L2 vs FP64 {er:.2e}, RT {eb:.2e}", flush=True)
+        assert er < 1e-5 and eb < 1e-5
