@@ -154,9 +154,10 @@ void mm_radon_destroy(mm_radon* plan);
  * and processing/propagation_large.py:35 PropagationLarge.D (:202); DT is the
  * exact adjoint of D (the reference has no DT, propagation_large.py:15).
  *
- * mm_fresnel_create: frame (nz, n), nz and n each 2^k in [8, 16384] or 3 * 2^k in
- *   [24, 12288]; a side above 8192 needs pad_mode 0 and both sides 2^k >= 16 or
- *   3 * 2^k >= 48 (it runs only on the frame-length DCT form);
+ * mm_fresnel_create: frame (nz, n), nz 2^k in [8, 16384] or 3 * 2^k in [24, 12288],
+ *   n 2^k in [8, 32768] or 3 * 2^k in [24, 24576]; a side above 8192 needs pad_mode 0
+ *   and both sides 2^k >= 16 or 3 * 2^k >= 48 (it runs only on the frame-length DCT
+ *   form; rows above 16384 as two n/2-point halves, plain D / DT only);
  *   ndist distances (dist_host, metres), wavelength and voxelsize in metres.
  *   pad_mode 0 = symmetric mirror padding (processing/kernels.py:530-550, the
  *   reference), 1 = zero padding.  The chirp exp(-i pi lambda z f^2) is built in

@@ -379,6 +379,78 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctRowShape<L>::NT) void dct_row_
   if constexpr (Store::kLoss) block_sum<Sh::NT>(acc, lds, partial, blockIdx.x);
 }
 
+// Rows longer than one workgroup's transform (n = 2Q, Q <= 16384) on the DCT
+// form: the n-point FFT of the reordered row v (v_s = x_(2s), v_(s+Q) =
+// x_(n-1-2s), s < Q) splits by frequency parity, V[2m + e] =
+// FFT_Q(w^(2es) (v_s + (-1)^e v_(s+Q)))[m] with w = exp(-i pi / n) (table h);
+// the partner V[n - k] of V[k] has k's parity, so each half takes its own
+// step U = A V + B V(n - .) through LDS; the inverse n-point transform splits by
+// input parity, E = IFFT_Q(U[2m]), O = IFFT_Q(U[2m + 1]), y_p = E_p + conj(w^(2p)) O_p,
+// y_(p+Q) = E_p - conj(w^(2p)) O_p, and y_j goes back to x position dct_src(j).
+// One row per workgroup, in place: both halves' inputs are in registers before
+// the first barrier.
+template <int Q>
+using RowSplitShape = RegShape<Q, Q>;  // one Q-point transform per workgroup
+template <int Q, bool CONJ_K>
+__global__ __launch_bounds__(RowSplitShape<Q>::NT) void dct_row_split(
+    float2* __restrict__ buf, long long nrows, const float2* __restrict__ M, const float2* __restrict__ h,
+    const float2* __restrict__ tw) {
+  using Sh = RowSplitShape<Q>;
+  static_assert(Sh::C == 1 && Sh::NT == Sh::TPC, "one transform per workgroup");
+  __shared__ float2 lds[Sh::LDS];
+  constexpr int N = 2 * Q;
+  const long long row = blockIdx.x;
+  if (row >= nrows) return;  // the whole workgroup, before any barrier
+  float2* x = buf + row * N;
+  const int t = threadIdx.x;
+  // Thread t reads and writes only the row positions 2s and N - 1 - 2s of its own s = t + m TPC, so it
+  // parks the odd half's input and then the even half's result there: one half is live in registers.
+  float2 v[Sh::PT];
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int s = t + m * Sh::TPC;
+    const float2 lo = x[2 * s], hi = x[N - 1 - 2 * s];
+    v[m] = make_float2(lo.x + hi.x, lo.y + hi.y);
+    x[N - 1 - 2 * s] = cmul(make_float2(lo.x - hi.x, lo.y - hi.y), h[2 * s]);
+  }
+  // one half: forward transform, the step between the transforms, inverse transform (in place in v)
+  auto half = [&](int e, int tt, const float2* twp) {
+    fft_reg<Q, Sh::TPC, false, 1, false>(v, lds, tt, twp);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) lds[lpad(tt + m * Sh::TPC)] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) {
+      const int mm = tt + m * Sh::TPC, k = 2 * mm + e;
+      const int mr = e ? Q - 1 - mm : (mm == 0 ? 0 : Q - mm);  // V[N - k] (k = 0: V[0])
+      const float2 vr = lds[lpad(mr)];
+      float2 a = M[k];
+      if (CONJ_K) a.y = -a.y;
+      const float2 b = M[(CONJ_K ? 2 * N : N) + k];
+      const float2 p = cmul(a, v[m]), q = cmul(b, vr);
+      v[m] = make_float2(p.x + q.x, p.y + q.y);
+    }
+    __syncthreads();  // the inverse transform's first pass rewrites lds
+    fft_reg<Q, Sh::TPC, true, 1, false>(v, lds, tt, twp);
+  };
+  half(0, t, tw);
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {  // E parked at 2p, the odd input taken back
+    const int s = t + m * Sh::TPC;
+    const float2 odd = x[N - 1 - 2 * s];
+    x[2 * s] = v[m];
+    v[m] = odd;
+  }
+  half(1, opaque_index(t), opaque_twiddles(tw));
+#pragma unroll
+  for (int m = 0; m < Sh::PT; ++m) {
+    const int p = t + m * Sh::TPC;
+    const float2 e = x[2 * p], o = cmul(v[m], cconj(h[2 * p]));
+    x[2 * p] = make_float2(e.x + o.x, e.y + o.y);          // y_p at dct_src(p) = 2p
+    x[N - 1 - 2 * p] = make_float2(e.x - o.x, e.y - o.y);  // y_(p+Q) at dct_src(p+Q)
+  }
+}
+
 }  // namespace fresnel
 }  // namespace mm
 
@@ -393,6 +465,7 @@ struct mm_fresnel {
   float2* Ky = nullptr;  // ndist x 2nz
   float2* Mx = nullptr;  // DCT form: ndist x [A | B | B of DT] x n (see dct_mid)
   float2* My = nullptr;  // ndist x 3 x nz
+  float2* hx = nullptr;  // rows above 16384 (n = 2Q): exp(-i pi j / n), j < n (dct_row_split)
   const float2* tw = nullptr;
   size_t bytes = 0;
 };
@@ -403,6 +476,7 @@ static void fresnel_free(mm_fresnel* p) {
   if (p->Ky) (void)hipFree(p->Ky);
   if (p->Mx) (void)hipFree(p->Mx);
   if (p->My) (void)hipFree(p->My);
+  if (p->hx) (void)hipFree(p->hx);
   delete p;
 }
 
@@ -424,6 +498,15 @@ static void launch_row_pass(const mm_fresnel* p, const float2* in, float2* out, 
 
 // The DCT form needs the length-N register FFT (N >= 16, 3 * 2^k >= 48).
 static bool dct_len_ok(int N) { return ((is_pow2(N) && N >= 16) || (is_fft_len(N) && N >= 48)) && N <= kMaxFFT; }
+// Rows may also be twice that (dct_row_split: two N/2-point halves).
+static bool dct_row_len_ok(int N) { return dct_len_ok(N) || (N > kMaxFFT && N <= 2 * kMaxFFT && dct_len_ok(N / 2)); }
+
+template <int Q, bool CONJ_K>
+static void launch_dct_row_split(const mm_fresnel* p, float2* buf, int nframes, int j, hipStream_t st) {
+  const long long rows = (long long)nframes * p->nz;
+  hipLaunchKernelGGL((dct_row_split<Q, CONJ_K>), dim3((unsigned)rows), dim3(RowSplitShape<Q>::NT), 0, st, buf, rows,
+                     p->Mx + (size_t)j * 3 * 2 * Q, p->hx, p->tw + tw_offset(Q));
+}
 
 // [A | B | B of DT] (dct_mid) of a DCT side of N points for every distance, in FP64 from the chirp of the
 // 2N-point padded transform (the same phase as chirp_table, K carrying 1/(2N)).
@@ -492,6 +575,18 @@ static int fresnel_apply(mm_fresnel* p, const void* in, void* out, int nframes, 
     }
     int rc = check_launch("fresnel dct_col_pass");
     if (rc) return rc;
+    if (p->n > kMaxFFT) {  // n = 24576 or 32768 (dct_row_len_ok)
+      if (p->n == 24576) {
+        if (adj) launch_dct_row_split<12288, true>(p, dst, nframes, j, st);
+        else launch_dct_row_split<12288, false>(p, dst, nframes, j, st);
+      } else if (p->n == 32768) {
+        if (adj) launch_dct_row_split<16384, true>(p, dst, nframes, j, st);
+        else launch_dct_row_split<16384, false>(p, dst, nframes, j, st);
+      } else {
+        return fail(MM_ERR_UNSUPPORTED, "fresnel: split rows not built for n=" + std::to_string(p->n));
+      }
+      return check_launch("fresnel dct_row_split");
+    }
     if (adj) {
       MM_DISPATCH_FFT(p->n, (launch_dct_row<LL, true>(p, ld_out, sto, nframes, j, nullptr, st)));
     } else {
@@ -588,14 +683,15 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
   MM_ARG_CHECK(plan != nullptr, "mm_fresnel_create: plan is NULL");
   *plan = nullptr;
   auto size_ok = [](int v) { return ((is_pow2(v) && v >= 8) || (is_fft_len(v) && v >= 24)) && v <= kMaxFFT; };
-  MM_ARG_CHECK(size_ok(n), "mm_fresnel_create: n must be 2^k in [8, 16384] or 3 * 2^k in [24, 12288], got " +
-                               std::to_string(n));
+  MM_ARG_CHECK(size_ok(n) || (pad_mode == 0 && n > kMaxFFT && dct_row_len_ok(n)),
+               "mm_fresnel_create: n must be 2^k in [8, 32768] or 3 * 2^k in [24, 24576] (above 16384 with symmetric "
+               "padding), got " + std::to_string(n));
   MM_ARG_CHECK(size_ok(nz), "mm_fresnel_create: nz must be 2^k in [8, 16384] or 3 * 2^k in [24, 12288], got " +
                                 std::to_string(nz));
   MM_ARG_CHECK(ndist >= 1 && dist_host != nullptr, "mm_fresnel_create: need ndist >= 1 distances");
   MM_ARG_CHECK(pad_mode == 0 || pad_mode == 1, "mm_fresnel_create: pad_mode must be 0 (symmetric) or 1 (zero)");
   // a side past 8192 runs only on the frame-length DCT form (the padded form's transforms are 2N long)
-  MM_ARG_CHECK((2 * n <= kMaxFFT && 2 * nz <= kMaxFFT) || (pad_mode == 0 && dct_len_ok(n) && dct_len_ok(nz)),
+  MM_ARG_CHECK((2 * n <= kMaxFFT && 2 * nz <= kMaxFFT) || (pad_mode == 0 && dct_row_len_ok(n) && dct_len_ok(nz)),
                "mm_fresnel_create: sides above 8192 need symmetric padding and both sides 2^k >= 16 or 3 * 2^k >= 48");
   MM_ARG_CHECK(voxelsize > 0 && wavelength >= 0, "mm_fresnel_create: bad wavelength / voxelsize");
   mm_fresnel* p = new mm_fresnel();
@@ -610,9 +706,22 @@ int mm_fresnel_create(mm_fresnel** plan, int n, int nz, int ndist, double wavele
   p->nz = nz;
   p->ndist = ndist;
   p->pad_mode = pad_mode;
-  p->dct = pad_mode == 0 && dct_len_ok(n) && dct_len_ok(nz);
+  p->dct = pad_mode == 0 && dct_row_len_ok(n) && dct_len_ok(nz);
   double* ddist = nullptr;
   if (hipMalloc(&ddist, sizeof(double) * ndist) != hipSuccess) return bail(fail(MM_ERR_OOM, "fresnel distances"));
+  if (n > kMaxFFT) {  // the split rows' half-angle table, FP64-built
+    std::vector<float2> hh(n);
+    for (int j = 0; j < n; ++j) {
+      const double a = -M_PI * (double)j / n;
+      hh[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if (hipMalloc((void**)&p->hx, sizeof(float2) * n) != hipSuccess ||
+        hipMemcpy(p->hx, hh.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(ddist);
+      return bail(fail(MM_ERR_OOM, "fresnel split-row table"));
+    }
+    p->bytes += sizeof(float2) * n;
+  }
   if (hipMalloc(&p->Kx, sizeof(float2) * (size_t)ndist * 2 * n) != hipSuccess ||
       hipMalloc(&p->Ky, sizeof(float2) * (size_t)ndist * 2 * nz) != hipSuccess) {
     (void)hipFree(ddist);

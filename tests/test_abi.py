@@ -97,8 +97,11 @@ def test_header_size_limits_match_the_library(lib):
     for n in (a // 2, 2 * b, c // 2, 2 * d):
         assert lib.mm_radon_plan_check(n, 2, th) == _lib.MM_ERR_ARG, n
     p = C.c_void_p()
-    fa, fb, fc, fd = _header_ranges("mm_fresnel_create", "nz and n")
+    za, zb, zc, zd = _header_ranges("mm_fresnel_create", "nz ")
+    fa, fb, fc, fd = _header_ranges("mm_fresnel_create", ", n ")
+    assert (za, zb, zc, zd) == (8, 16384, 24, 12288) and (fa, fb, fc, fd) == (8, 32768, 24, 24576)
     dist = (C.c_double * 1)(0.1)
     for n in (fa // 2, 2 * fb, fc // 2, 2 * fd):
-        assert lib.mm_fresnel_create(C.byref(p), n, fa, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, n
-        assert lib.mm_fresnel_create(C.byref(p), fa, n, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, n
+        assert lib.mm_fresnel_create(C.byref(p), n, 16, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, n
+    for nz in (za // 2, 2 * zb, zc // 2, 2 * zd):
+        assert lib.mm_fresnel_create(C.byref(p), 16, nz, 1, 1e-10, 1e-6, dist, 0) == _lib.MM_ERR_ARG and not p, nz

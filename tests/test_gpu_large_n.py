@@ -125,7 +125,7 @@ def test_radon_split_complex_rt(n):
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("nz,n", [(16, 16384), (48, 12288), (12288, 48), (16384, 16)])
+@pytest.mark.parametrize("nz,n", [(16, 16384), (48, 12288), (12288, 48), (16384, 16), (16, 24576), (16, 32768)])
 def test_fresnel_long_sides_vs_fp64_and_adjoint(nz, n):
     from model_mosaic_amd.processing.propagation import Propagation
     b, vox, dist, wl = 2, 0.69e-6, [0.2, 0.8], 1.24e-9 / 30.0
