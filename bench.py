@@ -449,8 +449,10 @@ def run_mosaic(torch, reps=2):
            "algorithmic_GB": round(alg / 1e9, 3), "hbm_frac": round(alg / dt / 1e9 / PEAK_HBM_GBS, 4),
            "plan_s": round(plan_s, 2), "radon_plans": info["radon_plans"], "fresnel_plans": info["fresnel_plans"],
            "graph_captures": info["graph_captures"], "branches": info["branches"],
-           "tuned_per_plan": sorted({(t["n"], t["ntheta"], g, c) for t, g, c in
-                                     zip(spec, info["gather_xcd_group"], info["col_split"])}),
+           "tuned_per_plan": sorted({(t["n"], t["ntheta"], g, c, r, z) for t, g, c, r, z in
+                                     zip(spec, info["gather_xcd_group"], info["col_split"], info["gather_root"],
+                                         info["gather_zp"])}),
+           "tuned_fields": ["n", "ntheta", "gather_xcd_group", "col_split", "gather_root", "gather_zp"],
            "workspace_GB": round(info["workspace_bytes"] / 1e9, 2)}
     mb.free()
     del deltas, outs

@@ -61,7 +61,10 @@ int mm_version(void);
  *                          ALUs from per-point tap lists;
  *     MM_RADON_GATHER_ZP1  gather tiles of width <= 32 stage one z slice per
  *                          round (default: two slices interleaved per patch
- *                          element; wider tiles always stage one).
+ *                          element; wider tiles always stage one);
+ *     MM_RADON_GATHER_ROOT32 gather quadtree tiles of at most 32 x 32 (default
+ *                          64 x 64 roots).  The gather flags change the speed,
+ *                          not the result.
  *   Without one of the three tile flags the scatter runs as dense 4 x 4-patch
  *   products on the matrix cores (FP32 in, FP32 accumulate).
  */
@@ -69,6 +72,7 @@ int mm_version(void);
 #define MM_RADON_GATHER_ZP1 2u
 #define MM_RADON_KAHAN_TILES 4u
 #define MM_RADON_LIST_TILES 8u
+#define MM_RADON_GATHER_ROOT32 16u
 int mm_radon_create(mm_radon** plan, int n, int nz_max, int ntheta, const double* theta_host, unsigned flags);
 
 /* R: obj (nz, n, n) float32 -> sino (ntheta, nz, n) float32, 1 <= nz <= nz_max.
@@ -120,16 +124,24 @@ int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
  * MM_RADON_TUNE_COL_SPLIT: 0 = column FFTs four columns per workgroup (one
  * per CU), 1 = two columns per workgroup (two per CU, block halves paired on
  * one XCD).
+ * MM_RADON_TUNE_GATHER_ROOT (64 or 32) and MM_RADON_TUNE_GATHER_ZP (2 or 1):
+ * the gather's quadtree root tile and z slices per patch element, as the
+ * MM_RADON_GATHER_* flags; setting one rebuilds the gather tables (host work,
+ * device allocations and a device synchronisation: not while launches of the
+ * plan are being captured into a graph).
  * Results do not depend on the knobs, only the speed does. */
 #define MM_RADON_TUNE_GATHER_XCD_GROUP 1
 #define MM_RADON_TUNE_SCATTER_XCD_GROUP 2
 #define MM_RADON_TUNE_COL_SPLIT 3
-#define MM_RADON_TUNE_NKNOBS 4
+#define MM_RADON_TUNE_GATHER_ROOT 4
+#define MM_RADON_TUNE_GATHER_ZP 5
+#define MM_RADON_TUNE_NKNOBS 6
 int mm_radon_tune(mm_radon* plan, int knob, int value);
 /* Current value of a knob (or MM_ERR_ARG). */
 int mm_radon_get_tune(const mm_radon* plan, int knob);
-/* Tile autotuner: times R (obj -> sino) for every gather XCD group size in
- * {1, 2, 4, 8, 16} and keeps the fastest; for n = 2048 (the only width whose
+/* Tile autotuner: times R (obj -> sino) for every gather launch shape (root
+ * tile 64 / 32, two / one z slices per patch element) with every gather XCD
+ * group size in {1, 2, 4, 8, 16} and keeps the fastest pair; for n = 2048 (the only width whose
  * column transforms the split knob reshapes) then R (and RT) with the column
  * split off and on; with back != NULL last RT (sino -> back) for the scatter
  * knob.  Synchronises the stream (not for use under graph capture).  obj is

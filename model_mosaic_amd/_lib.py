@@ -20,9 +20,12 @@ MM_RADON_FP64_TILES = 1
 MM_RADON_GATHER_ZP1 = 2
 MM_RADON_KAHAN_TILES = 4
 MM_RADON_LIST_TILES = 8
+MM_RADON_GATHER_ROOT32 = 16
 MM_RADON_TUNE_GATHER_XCD_GROUP = 1
 MM_RADON_TUNE_SCATTER_XCD_GROUP = 2
 MM_RADON_TUNE_COL_SPLIT = 3
+MM_RADON_TUNE_GATHER_ROOT = 4
+MM_RADON_TUNE_GATHER_ZP = 5
 MM_MOSAIC_NO_GRAPH = 1
 MM_MOSAIC_AUTOTUNE = 2
 MM_MOSAIC_INFO_RADON_PLANS = 0

@@ -253,8 +253,10 @@ int mm_mosaic_run(mm_mosaic* m, const float* const* delta, float* const* out, vo
   if ((m->flags & MM_MOSAIC_AUTOTUNE) && !m->tuned) {
     for (size_t r = 0; r < m->rplans.size(); ++r) {
       const int t = m->rplan_first[r];
+      const size_t before = mm_radon_workspace_bytes(m->rplans[r]);  // the gather tables may be rebuilt
       if ((rc = mm_radon_autotune(m->rplans[r], delta[t], m->branches[0].p, nullptr, m->tiles[t].nz, m->st)))
         return rc;
+      m->bytes = m->bytes - before + mm_radon_workspace_bytes(m->rplans[r]);
     }
     m->tuned = true;
   }

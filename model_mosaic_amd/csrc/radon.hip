@@ -2666,7 +2666,10 @@ struct mm_radon {
   float2* bbuf2 = nullptr;   // second channel (complex sinograms, allocated on first use)
   long long bzs = 0;         // elements per z plane of bbuf
   int gzp = 2;  // z slices per gather patch element (the lane colouring depends on it)
-  int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0};  // launch knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
+  int groot = ROOT;                  // largest gather quadtree tile (64, or 32 with MM_RADON_GATHER_ROOT32)
+  std::vector<double> theta_h;       // the angles, for rebuilding the gather tables (radon_regather)
+  size_t gbytes = 0;                 // device bytes of the gather tables (part of bytes)
+  int tune[MM_RADON_TUNE_NKNOBS] = {0, 8, 4, 0, ROOT, 2};  // knobs (mm_radon_tune): XCD groups of 8 gather, 4 scatter items
   // adjoint scatter: patch runs on the matrix cores (FP32, default) or per-point lists (FP64 / Kahan tiles)
   bool mfma = true;
   MItem* mitems = nullptr;
@@ -2959,6 +2962,60 @@ static int launch_scatter(const mm_radon* p, const float2* yh, float2* Hb, int n
   return rc;
 }
 
+// Rebuilds the gather tables (items, sample lists, lane order, column-pass support) for another
+// quadtree root tile and z slices per patch element: same result, other launch shape.  Host work and
+// device allocations; the device is synchronised before the old tables are freed.
+static int radon_regather(mm_radon* p, int root, int zp) {
+  if (root == p->groot && zp == p->gzp) return MM_OK;
+  HostGeom G;
+  G.n = p->n;
+  G.H = p->H;
+  G.ntheta = p->ntheta;
+  G.dx.resize(p->ntheta);
+  G.dy.resize(p->ntheta);
+  for (int t = 0; t < p->ntheta; ++t) {
+    const double c = std::cos(p->theta_h[t]), sn = std::sin(p->theta_h[t]);
+    G.dx[t] = std::fabs(c);
+    G.dy[t] = -(c >= 0 ? 1 : -1) * sn;
+  }
+  Binning BG;
+  BG.zp = zp;
+  build_gather(G, std::min(root, p->n), BG);
+  const std::vector<int2> gsupp = gather_support(BG, p->n);
+  Item* gi = nullptr;
+  int* gs = nullptr;
+  unsigned short* go = nullptr;
+  int2* gp = nullptr;
+  size_t nb = 0;
+  int rc;
+  if ((rc = upload(&gi, BG.items, nb)) || (rc = upload(&gs, BG.samples, nb)) || (rc = upload(&go, BG.order, nb)) ||
+      (rc = upload(&gp, gsupp, nb))) {
+    for (void* q : {(void*)gi, (void*)gs, (void*)go, (void*)gp})
+      if (q) (void)hipFree(q);
+    return rc;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {  // no launch may still read the old tables
+    for (void* q : {(void*)gi, (void*)gs, (void*)go, (void*)gp})
+      if (q) (void)hipFree(q);
+    return fail(MM_ERR_HIP, "mm_radon_tune: device synchronisation before the gather tables are rebuilt");
+  }
+  for (void* q : {(void*)p->gitems, (void*)p->gsamples, (void*)p->gorder, (void*)p->gsupp})
+    if (q) (void)hipFree(q);
+  p->gitems = gi;
+  p->gsamples = gs;
+  p->gorder = go;
+  p->gsupp = gp;
+  p->bytes = p->bytes - p->gbytes + nb;
+  p->gbytes = nb;
+  p->n_gitems = (int)BG.items.size();
+  p->n_groot = BG.n_root;
+  p->gzp = zp;
+  p->groot = root;
+  p->tune[MM_RADON_TUNE_GATHER_ROOT] = root;
+  p->tune[MM_RADON_TUNE_GATHER_ZP] = zp;
+  return MM_OK;
+}
+
 namespace mm {
 void radon_dims(const mm_radon* p, int* n, int* ntheta, int* nz_max) {
   *n = p->n;
@@ -2993,7 +3050,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   MM_ARG_CHECK(ntheta >= 1 && theta_host != nullptr, "mm_radon_create: need ntheta >= 1 angles");
   MM_ARG_CHECK((long long)ntheta * (n / 2 + 1) < (1LL << 30),
                "mm_radon_create: ntheta * (n/2+1) must stay below 2^30 (entry codes carry it shifted by one bit)");
-  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0,
+  MM_ARG_CHECK((flags & ~(MM_RADON_FP64_TILES | MM_RADON_GATHER_ZP1 | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES |
+                           MM_RADON_GATHER_ROOT32)) == 0,
                "mm_radon_create: unknown flags");
   MM_ARG_CHECK(__builtin_popcount(flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) <= 1,
                "mm_radon_create: at most one of MM_RADON_FP64_TILES, MM_RADON_KAHAN_TILES, MM_RADON_LIST_TILES");
@@ -3058,8 +3116,12 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   p->gzp = BG.zp;
   SBinning BS;
   MBinning BM;
-  const int root = std::min(ROOT, n);
+  const int root = std::min((flags & MM_RADON_GATHER_ROOT32) ? ROOT / 2 : ROOT, n);
   build_gather(G, root, BG);
+  p->groot = (flags & MM_RADON_GATHER_ROOT32) ? ROOT / 2 : ROOT;
+  p->tune[MM_RADON_TUNE_GATHER_ROOT] = p->groot;
+  p->tune[MM_RADON_TUNE_GATHER_ZP] = BG.zp;
+  p->theta_h.assign(theta_host, theta_host + ntheta);
   p->fwd_only = fwd_only;
   p->mfma = (flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0;
   const std::vector<int2> gsupp = gather_support(BG, n);
@@ -3095,6 +3157,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   }
   p->n_gitems = (int)BG.items.size();
   p->n_groot = BG.n_root;
+  p->gbytes = BG.items.size() * sizeof(Item) + BG.samples.size() * sizeof(int) +
+              BG.order.size() * sizeof(unsigned short) + gsupp.size() * sizeof(int2);
   p->n_sitems = (int)BS.items.size();
   p->n_mitems = (int)BM.items.size();
   p->n_splits = (int)(p->mfma ? BM.splits.size() : BS.splits.size());
@@ -3335,6 +3399,14 @@ int mm_radon_adj_fbp(mm_radon* p, const float* sino, const float* G, const float
 int mm_radon_tune(mm_radon* p, int knob, int value) {
   MM_ARG_CHECK(p != nullptr, "mm_radon_tune: plan is NULL");
   MM_ARG_CHECK(knob >= 1 && knob < MM_RADON_TUNE_NKNOBS, "mm_radon_tune: unknown knob " + std::to_string(knob));
+  if (knob == MM_RADON_TUNE_GATHER_ROOT) {
+    MM_ARG_CHECK(value == 32 || value == 64, "mm_radon_tune: gather root tile must be 32 or 64");
+    return radon_regather(p, value, p->gzp);
+  }
+  if (knob == MM_RADON_TUNE_GATHER_ZP) {
+    MM_ARG_CHECK(value == 1 || value == 2, "mm_radon_tune: gather slices per patch element must be 1 or 2");
+    return radon_regather(p, p->groot, value);
+  }
   if (knob == MM_RADON_TUNE_COL_SPLIT)
     MM_ARG_CHECK(value == 0 || value == 1, "mm_radon_tune: column split must be 0 or 1");
   else
@@ -3378,7 +3450,38 @@ int mm_radon_autotune(mm_radon* p, const float* obj, float* sino, float* back, i
     }
     p->tune[knob] = best;
   };
-  sweep(MM_RADON_TUNE_GATHER_XCD_GROUP, {1, 2, 4, 8, 16}, [&]() { return mm_radon_fwd(p, obj, sino, nz, stream); });
+  // gather: every launch shape (root tile 64 / 32, one / two slices per patch element; bitwise the same
+  // result) with every XCD group, the fastest pair kept
+  {
+    float best_ms = 1e30f;
+    int best_root = p->groot, best_zp = p->gzp, best_xg = p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP];
+    const int roots[2] = {ROOT, ROOT / 2};
+    for (int ri = 0; ri < (p->n > ROOT / 2 ? 2 : 1) && !rc; ++ri)
+      for (int zp = 2; zp >= 1 && !rc; --zp) {
+        if ((rc = radon_regather(p, roots[ri], zp))) break;
+        if ((rc = mm_radon_fwd(p, obj, sino, nz, stream))) break;  // warm-up
+        for (int xg : {1, 2, 4, 8, 16}) {
+          p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP] = xg;
+          for (int r = 0; r < 2 && !rc; ++r) {
+            if (hipEventRecord(e0, st) != hipSuccess || (rc = mm_radon_fwd(p, obj, sino, nz, stream)) ||
+                hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+              if (!rc) rc = fail(MM_ERR_HIP, "mm_radon_autotune: event timing");
+              break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+              best_ms = ms;
+              best_root = roots[ri];
+              best_zp = zp;
+              best_xg = xg;
+            }
+          }
+        }
+      }
+    if (!rc) rc = radon_regather(p, best_root, best_zp);
+    p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP] = best_xg;
+  }
   // the column split changes the workgroup shape of 4096-point column transforms only (n = 2048)
   if (!rc && 2 * p->n == 4096) {
     auto both = [&]() {

@@ -70,6 +70,10 @@ class MosaicBatch:
                                      for t in range(len(self.tiles))],
                 "col_split": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_COL_SPLIT)
                               for t in range(len(self.tiles))],
+                "gather_root": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_GATHER_ROOT)
+                                for t in range(len(self.tiles))],
+                "gather_zp": [self._lib.mm_mosaic_tile_knob(self._plan, t, _lib.MM_RADON_TUNE_GATHER_ZP)
+                              for t in range(len(self.tiles))],
                 "workspace_bytes": int(self._lib.mm_mosaic_workspace_bytes(self._plan))}
 
     def out_buffers(self):

@@ -28,11 +28,12 @@ class TomoReal:
     ``list_tiles`` (FP32 on the vector ALUs from per-point tap lists) is set (at most one of them).
     ``gather_zp`` (1 or 2) is the number of z slices the forward gather stages
     per patch element for tiles of width <= 32 (2, the default, is the faster
-    variant on gfx950; wider tiles always stage one slice).
+    variant on gfx950; wider tiles always stage one slice); ``gather_root``
+    (64 or 32) is the largest quadtree tile.  Neither changes the result.
     """
 
     def __init__(self, n, nz, theta, fp64_tiles: bool = False, gather_zp: int = 2, kahan_tiles: bool = False,
-                 list_tiles: bool = False):
+                 list_tiles: bool = False, gather_root: int = 64):
         require_gpu()
         self._lib = _lib.load()
         self.n = int(n)
@@ -44,10 +45,13 @@ class TomoReal:
         plan = C.c_void_p()
         if gather_zp not in (1, 2):
             raise ValueError(f"TomoReal: gather_zp must be 1 or 2, got {gather_zp}")
+        if gather_root not in (32, 64):
+            raise ValueError(f"TomoReal: gather_root must be 32 or 64, got {gather_root}")
         if int(fp64_tiles) + int(kahan_tiles) + int(list_tiles) > 1:
             raise ValueError("TomoReal: at most one of fp64_tiles, kahan_tiles, list_tiles")
         flags = ((_lib.MM_RADON_FP64_TILES if fp64_tiles else 0) | (_lib.MM_RADON_GATHER_ZP1 if gather_zp == 1 else 0)
-                 | (_lib.MM_RADON_KAHAN_TILES if kahan_tiles else 0) | (_lib.MM_RADON_LIST_TILES if list_tiles else 0))
+                 | (_lib.MM_RADON_KAHAN_TILES if kahan_tiles else 0) | (_lib.MM_RADON_LIST_TILES if list_tiles else 0)
+                 | (_lib.MM_RADON_GATHER_ROOT32 if gather_root == 32 else 0))
         _lib.check(self._lib.mm_radon_create(C.byref(plan), self.n, self._nz, self.ntheta,
                                              th.ctypes.data_as(C.POINTER(C.c_double)), flags),
                    "TomoReal")

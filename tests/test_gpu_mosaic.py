@@ -81,7 +81,9 @@ def test_no_graph_and_autotune_same_results():
     torch.cuda.synchronize()
     for x, y in zip(a, b):
         assert torch.equal(x, y)
-    assert all(g in (1, 2, 4, 8, 16) for g in mb.info()["gather_xcd_group"])
+    info = mb.info()
+    assert all(g in (1, 2, 4, 8, 16) for g in info["gather_xcd_group"])
+    assert all(r in (32, 64) for r in info["gather_root"]) and all(z in (1, 2) for z in info["gather_zp"])
 
 
 def test_branches_same_results():

@@ -279,3 +279,38 @@ def test_scatter_accumulators_vs_fp64(tomo_cls, n, nt, nz):
                 {"list_tiles": True, "fp64_tiles": True}):
         with pytest.raises(ValueError):
             tomo_cls(64, 1, th[:8], **bad)
+
+
+@pytest.mark.parametrize("n,nt,nz", [(256, 180, 3), (2048, 64, 2)])
+def test_gather_shape_knobs_bitwise(tomo_cls, n, nt, nz):
+    """Rebuilding the gather tables for root tile 64 / 32 and one / two slices per patch element (the knobs the
+    autotuner sweeps) leaves R bitwise unchanged, keeps the plan's workspace byte count equal to a plan created
+    with the same shape, and rejects values outside {32, 64} / {1, 2}; the autotuner keeps R's bits too."""
+    import ctypes as C
+    from model_mosaic_amd import _lib
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    x = torch.from_numpy(np.random.default_rng(n).standard_normal((nz, n, n)).astype(np.float32)).cuda()
+    t = tomo_cls(n, nz, th)
+    ref = t.R(x).clone()
+    lib = t._lib
+    for root in (32, 64):
+        for zp in (1, 2):
+            _lib.check(lib.mm_radon_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ROOT, root), "tune")
+            _lib.check(lib.mm_radon_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ZP, zp), "tune")
+            assert lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ROOT) == root
+            assert lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ZP) == zp
+            assert torch.equal(t.R(x), ref), (root, zp)
+            fresh = tomo_cls(n, nz, th, gather_root=root, gather_zp=zp)
+            assert t.workspace_bytes() == fresh.workspace_bytes(), (root, zp)
+            fresh.free()
+    for knob, bad in ((_lib.MM_RADON_TUNE_GATHER_ROOT, 16), (_lib.MM_RADON_TUNE_GATHER_ROOT, 128),
+                      (_lib.MM_RADON_TUNE_GATHER_ZP, 0), (_lib.MM_RADON_TUNE_GATHER_ZP, 4)):
+        assert lib.mm_radon_tune(t._plan, knob, bad) == _lib.MM_ERR_ARG
+    sino = torch.empty_like(ref)
+    back = torch.empty_like(x)
+    _lib.check(lib.mm_radon_autotune(t._plan, C.c_void_p(x.data_ptr()), C.c_void_p(sino.data_ptr()),
+                                     C.c_void_p(back.data_ptr()), nz, _lib.stream_handle()), "autotune")
+    assert lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ROOT) in (32, 64)
+    assert lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ZP) in (1, 2)
+    assert torch.equal(t.R(x), ref)
+    t.free()
