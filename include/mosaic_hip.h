@@ -66,7 +66,9 @@ int mm_version(void);
  *                          64 x 64 roots).  The gather flags change the speed,
  *                          not the result.
  *   Without one of the three tile flags the scatter runs as dense 4 x 4-patch
- *   products on the matrix cores (FP32 in, FP32 accumulate).
+ *   products on the matrix cores (FP32 in, FP32 accumulate) when nz_max >= 16;
+ *   those products take 16 slices at a time, so plans with nz_max < 16 use the
+ *   FP32 per-point lists instead (as MM_RADON_LIST_TILES).
  */
 #define MM_RADON_FP64_TILES 1u
 #define MM_RADON_GATHER_ZP1 2u

@@ -3123,7 +3123,8 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   p->tune[MM_RADON_TUNE_GATHER_ZP] = BG.zp;
   p->theta_h.assign(theta_host, theta_host + ntheta);
   p->fwd_only = fwd_only;
-  p->mfma = (flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0;
+  // the matrix-core products pad every call to MZ slices: a plan that never sees MZ slices takes the lists
+  p->mfma = (flags & (MM_RADON_FP64_TILES | MM_RADON_KAHAN_TILES | MM_RADON_LIST_TILES)) == 0 && nz_max >= MZ;
   const std::vector<int2> gsupp = gather_support(BG, n);
   std::vector<int2> ssupp;
   if (!fwd_only) {

@@ -23,7 +23,8 @@ class TomoReal:
     Parameters mirror processing/tomo.py:30: ``n`` detector width (= object
     width), ``nz`` the maximum number of slices per call, ``theta`` the
     projection angles in radians (stored as float32, like the reference).
-    The adjoint scatter runs as dense 4 x 4-patch products on the matrix cores (FP32) unless one of
+    The adjoint scatter runs as dense 4 x 4-patch products on the matrix cores (FP32, 16 slices per
+    product, so only for ``nz >= 16``; smaller plans take the per-point lists) unless one of
     ``fp64_tiles`` (tiles accumulate in FP64), ``kahan_tiles`` (FP32 with Kahan compensation) or
     ``list_tiles`` (FP32 on the vector ALUs from per-point tap lists) is set (at most one of them).
     ``gather_zp`` (1 or 2) is the number of z slices the forward gather stages

@@ -260,7 +260,8 @@ def test_scatter_accumulators_vs_fp64(tomo_cls, n, nt, nz):
     """RT with the scatter as dense patch products on the matrix cores (default), as FP32 per-point lists on the
     vector ALUs, as compensated FP32 (Kahan) lists and as FP64 lists, each vs the FP64 oracle: all within the 1e-5
     bar, the compensated and FP64 sums no worse than plain FP32 lists (beyond rounding noise), the matrix-core
-    result within FP32 rounding of the list result (nz = 17, 33: partial staging rounds of 16 slices)."""
+    result within FP32 rounding of the list result (nz = 17, 33: partial staging rounds of 16 slices; plans with
+    nz < 16 take the lists by default, so there "mfma" and "lists" are the same path)."""
     th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
     y = np.random.default_rng(n + 1).standard_normal((nt, nz, n)).astype(np.float32)
     ref = usfft.tomo_RT(y, th, dtype=np.float64)
