@@ -240,65 +240,92 @@ __device__ __forceinline__ void dct_mid(float2 (&v)[Sh::PT], float2* __restrict_
 // forward model builds psi = exp(-k b) e^{i k p} from the two float32
 // sinograms while loading, and its backward forms 4 (|u|^2 - d) u while
 // loading and the chain rule while storing, so none of these element-wise
-// steps is a pass of its own.
+// steps is a pass of its own.  Each policy splits into the raw global reads
+// (fetch / pre) and the arithmetic on them (make / the store): the passes
+// issue every point's reads first and compute afterwards, so the reads stay in
+// flight together (reads interleaved with the libm calls of psi were issued
+// one or two at a time).
+struct NoAux {};
 struct LoadC64 {
   const float2* __restrict__ in;
-  __device__ __forceinline__ float2 operator()(long long i) const { return in[i]; }
+  static constexpr bool kSplit = false;  // plain reads: kept predicated (the split form costs registers)
+  using Raw = float2;
+  __device__ __forceinline__ Raw fetch(long long i) const { return in[i]; }
+  __device__ __forceinline__ float2 make(Raw r) const { return r; }
 };
 struct LoadPsi {  // psi of step3_propagation.py:94-103 with separate phase / attenuation sinograms
   const float* __restrict__ p;
   const float* __restrict__ b;
   float k;
-  __device__ __forceinline__ float2 operator()(long long i) const {
+  static constexpr bool kSplit = true;
+  using Raw = float2;  // (p, b)
+  __device__ __forceinline__ Raw fetch(long long i) const { return make_float2(p[i], b[i]); }
+  __device__ __forceinline__ float2 make(Raw r) const {
     float sn, cs;
-    sincosf(k * p[i], &sn, &cs);
-    const float at = expf(-k * b[i]);
+    sincosf(k * r.x, &sn, &cs);
+    const float at = expf(-k * r.y);
     return make_float2(at * cs, at * sn);
   }
 };
 struct LoadPsiRatio {  // psi of one projection sinogram (step3_propagation.py:94-103)
   const float* __restrict__ p;
   float inv_norm, inv_beta_ratio;
-  __device__ __forceinline__ float2 operator()(long long i) const { return psi_of(p[i], inv_norm, inv_beta_ratio); }
+  static constexpr bool kSplit = true;
+  using Raw = float;
+  __device__ __forceinline__ Raw fetch(long long i) const { return p[i]; }
+  __device__ __forceinline__ float2 make(Raw r) const { return psi_of(r, inv_norm, inv_beta_ratio); }
 };
 struct LoadResidual {  // 4 (|u|^2 - d) u
   const float2* __restrict__ u;
   const float* __restrict__ d;
-  __device__ __forceinline__ float2 operator()(long long i) const {
+  static constexpr bool kSplit = true;
+  using Raw = float3;  // (u, d)
+  __device__ __forceinline__ Raw fetch(long long i) const {
     const float2 v = u[i];
-    const float r = 4.0f * (v.x * v.x + v.y * v.y - d[i]);
-    return make_float2(r * v.x, r * v.y);
+    return make_float3(v.x, v.y, d[i]);
+  }
+  __device__ __forceinline__ float2 make(Raw r) const {
+    const float q = 4.0f * (r.x * r.x + r.y * r.y - r.z);
+    return make_float2(q * r.x, q * r.y);
   }
 };
 struct StoreC64 {
   float2* __restrict__ out;
   static constexpr bool kLoss = false;
-  __device__ __forceinline__ void operator()(long long i, float2 v, double&) const { out[i] = v; }
+  using Aux = NoAux;
+  __device__ __forceinline__ Aux pre(long long) const { return {}; }
+  __device__ __forceinline__ void operator()(long long i, float2 v, double&, Aux) const { out[i] = v; }
 };
 struct StoreLoss {  // u kept for the backward; (|u|^2 - d)^2 summed in FP64
   float2* __restrict__ out;
   const float* __restrict__ d;
   static constexpr bool kLoss = true;
-  __device__ __forceinline__ void operator()(long long i, float2 v, double& acc) const {
+  using Aux = float;  // d
+  __device__ __forceinline__ Aux pre(long long i) const { return d[i]; }
+  __device__ __forceinline__ void operator()(long long i, float2 v, double& acc, Aux dd) const {
     out[i] = v;
-    const double r = (double)(v.x * v.x + v.y * v.y) - (double)d[i];
+    const double r = (double)(v.x * v.x + v.y * v.y) - (double)dd;
     acc += r * r;
   }
 };
 struct StoreAbs2 {  // intensity |u|^2 (step3_propagation.py:105-110)
   float* __restrict__ out;
   static constexpr bool kLoss = false;
-  __device__ __forceinline__ void operator()(long long i, float2 v, double&) const { out[i] = abs2_of(v); }
+  using Aux = NoAux;
+  __device__ __forceinline__ Aux pre(long long) const { return {}; }
+  __device__ __forceinline__ void operator()(long long i, float2 v, double&, Aux) const { out[i] = abs2_of(v); }
 };
 struct StoreChain {  // dL/dp = -k Im(conj(G) psi), dL/db = -k Re(conj(G) psi), times gscale, over p and b
   float* __restrict__ p;
   float* __restrict__ b;
   float k, gscale;
   static constexpr bool kLoss = false;
-  __device__ __forceinline__ void operator()(long long i, float2 g, double&) const {
+  using Aux = float2;  // (p, b) before they are overwritten
+  __device__ __forceinline__ Aux pre(long long i) const { return make_float2(p[i], b[i]); }
+  __device__ __forceinline__ void operator()(long long i, float2 g, double&, Aux pb) const {
     float sn, cs;
-    sincosf(k * p[i], &sn, &cs);
-    const float at = expf(-k * b[i]);
+    sincosf(k * pb.x, &sn, &cs);
+    const float at = expf(-k * pb.y);
     const float2 psi = make_float2(at * cs, at * sn);
     const float re = g.x * psi.x + g.y * psi.y;  // Re(conj(g) psi)
     const float im = g.x * psi.y - g.y * psi.x;  // Im(conj(g) psi)
@@ -336,18 +363,31 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctColShape<L>::NT) void dct_col_
   const int bx = (int)(lw % gridDim.x), frame = (int)(lw / gridDim.x);
   const int col = bx * Sh::C + c;
   const bool live = col < n;
-  const long long fo = (long long)frame * L * n + col;
+  // split reads of a column past the frame width (last block) come from column n - 1 and are dropped
+  const long long fo = (long long)frame * L * n + (Load::kSplit && !live ? n - 1 : col);
   float2* x = lds + c * Sh::S;
   float2 v[Sh::PT];
+  if constexpr (Load::kSplit) {
+    typename Load::Raw r[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) v[m] = live ? ld(fo + (long long)dct_src<Sh>(m, t) * n) : make_float2(0.f, 0.f);
+    for (int m = 0; m < Sh::PT; ++m) r[m] = ld.fetch(fo + (long long)dct_src<Sh>(m, t) * n);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) v[m] = live ? ld.make(r[m]) : make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m)
+      v[m] = live ? ld.make(ld.fetch(fo + (long long)dct_src<Sh>(m, t) * n)) : make_float2(0.f, 0.f);
+  }
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
   dct_mid<Sh, CONJ_K>(v, x, t, M);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   double acc = 0.0;
   if (live) {
+    typename Store::Aux a[Sh::PT];
 #pragma unroll
-    for (int m = 0; m < Sh::PT; ++m) st(fo + (long long)dct_src<Sh>(m, t) * n, v[m], acc);
+    for (int m = 0; m < Sh::PT; ++m) a[m] = st.pre(fo + (long long)dct_src<Sh>(m, t) * n);
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) st(fo + (long long)dct_src<Sh>(m, t) * n, v[m], acc, a[m]);
   }
   if constexpr (Store::kLoss) block_sum<Sh::NT>(acc, lds, partial, blockIdx.x + gridDim.x * blockIdx.y);
 }
@@ -366,15 +406,26 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(DctRowShape<L>::NT) void dct_row_
   const long long ro = (valid ? row : 0) * L;
   float2* x = lds + c * Sh::S;
   float2 v[Sh::PT];
+  if constexpr (Load::kSplit) {
+    typename Load::Raw r[Sh::PT];
 #pragma unroll
-  for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? ld(ro + dct_src<Sh>(m, t)) : make_float2(0.f, 0.f);
+    for (int m = 0; m < Sh::PT; ++m) r[m] = ld.fetch(ro + dct_src<Sh>(m, t));
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? ld.make(r[m]) : make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) v[m] = valid ? ld.make(ld.fetch(ro + dct_src<Sh>(m, t))) : make_float2(0.f, 0.f);
+  }
   fft_reg<L, Sh::TPC, false, 1, false>(v, x, t, tw);
   dct_mid<Sh, CONJ_K>(v, x, t, M);
   fft_reg<L, Sh::TPC, true, 1, false>(v, x, t, tw);
   double acc = 0.0;
   if (valid) {
+    typename Store::Aux a[Sh::PT];
 #pragma unroll
-    for (int m = 0; m < Sh::PT; ++m) st(ro + dct_src<Sh>(m, t), v[m], acc);
+    for (int m = 0; m < Sh::PT; ++m) a[m] = st.pre(ro + dct_src<Sh>(m, t));
+#pragma unroll
+    for (int m = 0; m < Sh::PT; ++m) st(ro + dct_src<Sh>(m, t), v[m], acc, a[m]);
   }
   if constexpr (Store::kLoss) block_sum<Sh::NT>(acc, lds, partial, blockIdx.x);
 }
