@@ -302,6 +302,20 @@ struct SampleGeom {
   float fx, fy;
 };
 
+// (direction d = dirs[id / H] already read)
+__device__ __forceinline__ SampleGeom sample_geom_d(int id, int H, double2 d) {
+  SampleGeom g;
+  g.th = id / H;
+  g.k = id - g.th * H;
+  const double X = (2.0 * g.k) * d.x, Y = (2.0 * g.k) * d.y;
+  const double rx = rint(X), ry = rint(Y);
+  g.rx = (int)rx;
+  g.ry = (int)ry;
+  g.fx = (float)(X - rx);
+  g.fy = (float)(Y - ry);
+  return g;
+}
+
 __device__ __forceinline__ SampleGeom sample_geom(int id, int H, const double2* __restrict__ dirs) {
   SampleGeom g;
   g.th = id / H;
@@ -400,14 +414,40 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
   // output of sorted sample t_j = threadIdx.x + j NT
   int src_lane[SPT];
   long long out0[SPT];
+  // Root shape (two samples per lane): the sample reads of both go out together, in two dependent rounds
+  // (ids and lane order; then directions and the ids in lane order), from clamped indices; one sample at a
+  // time behind the s < count predicate they were issued one by one, each waited on.  The leaf shape (one
+  // sample per lane, 64 VGPRs) keeps the reads in the per-sample loop: holding them spills.
+  constexpr bool kBatchReads = SPT > 1;
+  int idj[SPT], idl[SPT];
+  double2 dj[SPT];
+  if (kBatchReads && it.count > 0) {
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      const int sc = it.start + min((int)threadIdx.x + j * NT, it.count - 1);
+      idj[j] = samples[sc];
+      src_lane[j] = order[sc];
+    }
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      dj[j] = dirs[idj[j] / H];
+      idl[j] = samples[it.start + src_lane[j]];
+    }
+  }
 #pragma unroll
   for (int j = 0; j < SPT; ++j) {
     const int s = threadIdx.x + j * NT;
     base[j] = 0;
-    src_lane[j] = 0;
     out0[j] = 0;
-    if (s >= it.count) continue;
-    const SampleGeom g = sample_geom(samples[it.start + s], H, dirs);
+    if (s >= it.count) {
+      src_lane[j] = 0;
+      continue;
+    }
+    if constexpr (!kBatchReads) {
+      idj[j] = samples[it.start + s];
+      dj[j] = dirs[idj[j] / H];
+    }
+    const SampleGeom g = sample_geom_d(idj[j], H, dj[j]);
     const int b = (g.ry - it.y0) * pp + (g.rx - it.x0);
     const int par = ZP == 1 ? (b & 1) : 0;
     base[j] = b - par;
@@ -425,8 +465,11 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
       const float dy = (float)(i - M) - g.fy;
       ey[j][i] = ((i & 1) ? -sy : sy) * __expf(-a * dy * dy);
     }
-    src_lane[j] = order[it.start + s];
-    const int id = samples[it.start + src_lane[j]];
+    if constexpr (!kBatchReads) {
+      src_lane[j] = order[it.start + s];
+      idl[j] = samples[it.start + src_lane[j]];
+    }
+    const int id = idl[j];
     const int th = id / H;
     out0[j] = (long long)th * nz * H + (id - th * H);
   }
