@@ -123,9 +123,10 @@ int mm_radon_stage_ms(mm_radon* plan, float* ms_out, int max_stages);
  * MM_RADON_TUNE_GATHER_XCD_GROUP / MM_RADON_TUNE_SCATTER_XCD_GROUP: number G
  * of neighbouring gather / scatter work items that run on one XCD (one L2)
  * before the next XCD takes the next G; 1 = plain round-robin order.
- * MM_RADON_TUNE_COL_SPLIT: 0 = column FFTs four columns per workgroup (one
- * per CU), 1 = two columns per workgroup (two per CU, block halves paired on
- * one XCD).
+ * MM_RADON_TUNE_COL_SPLIT: 0 = the 2n-point column FFTs at 16384 points per
+ * workgroup (4 columns at n = 2048: one workgroup per CU), 1 = half as many
+ * columns per workgroup (two workgroups per CU where the LDS allows; the
+ * workgroups of one 4-column block kept on one XCD).
  * MM_RADON_TUNE_GATHER_ROOT (64 or 32) and MM_RADON_TUNE_GATHER_ZP (2 or 1):
  * the gather's quadtree root tile and z slices per patch element, as the
  * MM_RADON_GATHER_* flags; setting one rebuilds the gather tables (host work,
@@ -143,10 +144,9 @@ int mm_radon_tune(mm_radon* plan, int knob, int value);
 int mm_radon_get_tune(const mm_radon* plan, int knob);
 /* Tile autotuner: times R (obj -> sino) for every gather launch shape (root
  * tile 64 / 32, two / one z slices per patch element) with every gather XCD
- * group size in {1, 2, 4, 8, 16} and keeps the fastest pair; for n = 2048 (the only width whose
- * column transforms the split knob reshapes) then R (and RT) with the column
- * split off and on; with back != NULL last RT (sino -> back) for the scatter
- * knob.  Synchronises the stream (not for use under graph capture).  obj is
+ * group size in {1, 2, 4, 8, 16} and keeps the fastest pair; for 512 <= n <=
+ * 8192 then R (and RT) with the column split off and on; with back != NULL
+ * last RT (sino -> back) for the scatter knob.  Synchronises the stream (not for use under graph capture).  obj is
  * only read; sino and back are overwritten. */
 int mm_radon_autotune(mm_radon* plan, const float* obj_dev, float* sino_dev, float* back_dev, int nz, void* stream);
 

@@ -2811,7 +2811,7 @@ static void launch_col_shape(const mm_radon* p, float2* buf, int nz, hipStream_t
 
 template <int L, bool INV>
 static void launch_col(const mm_radon* p, float2* buf, int nz, hipStream_t st) {
-  if constexpr (ColShapeB<L, kRowBatch>::C == 2) {
+  if constexpr (ColShapeB<L, kRowBatch>::C < ColShapeB<L, kBatchElems>::C) {
     if (p->tune[MM_RADON_TUNE_COL_SPLIT]) {
       launch_col_shape<L, INV, kRowBatch>(p, buf, nz, st);
       return;
@@ -3179,6 +3179,9 @@ int radon_create_impl(mm_radon** plan, int n, int nz_max, int ntheta, const doub
   build_gather(G, root, BG);
   p->groot = (flags & MM_RADON_GATHER_ROOT32) ? ROOT / 2 : ROOT;
   p->tune[MM_RADON_TUNE_GATHER_ROOT] = p->groot;
+  // half as many columns per column-FFT workgroup measured faster (same bits) at these widths: 15 - 17 % at
+  // n = 1024 and 1536, 9 % at 3072; neutral at 2048, slower at 4096
+  p->tune[MM_RADON_TUNE_COL_SPLIT] = (n == 1024 || n == 1536 || n == 3072) ? 1 : 0;
   p->tune[MM_RADON_TUNE_GATHER_ZP] = BG.zp;
   p->theta_h.assign(theta_host, theta_host + ntheta);
   p->fwd_only = fwd_only;
@@ -3542,8 +3545,8 @@ int mm_radon_autotune(mm_radon* p, const float* obj, float* sino, float* back, i
     if (!rc) rc = radon_regather(p, best_root, best_zp);
     p->tune[MM_RADON_TUNE_GATHER_XCD_GROUP] = best_xg;
   }
-  // the column split changes the workgroup shape of 4096-point column transforms only (n = 2048)
-  if (!rc && 2 * p->n == 4096) {
+  // the column split halves the columns per workgroup of the 2n-point column transforms (same bits)
+  if (!rc && !p->big && !p->split && p->n >= 512) {
     auto both = [&]() {
       int r = mm_radon_fwd(p, obj, sino, nz, stream);
       if (!r && back && !p->fwd_only) r = mm_radon_adj(p, sino, 0, back, nz, stream);
