@@ -123,13 +123,13 @@ __host__ __device__ __forceinline__ long long hs_idx(int row, int col, int twon)
 }
 
 // Workgroup shapes of the row and line passes.  The forward row pass takes
-// 16384 points per workgroup for power-of-two lengths from 4096 up (four
+// 16384 points per workgroup for the power-of-two lengths 4096 and 16384 (four
 // transforms = 8 rows at L = 4096: 256-byte runs per half-spectrum block, one
-// workgroup per CU; 4 % faster than two transforms); 3 * 2^k lengths and the
-// C2R rows and the line passes take 8192 (two workgroups per CU at 70 KB of
-// LDS; 16384 points made them slower).
+// workgroup per CU; 4 % faster than two transforms); L = 8192 (n = 4096),
+// 3 * 2^k lengths, the C2R rows and the line passes take 8192 (two workgroups
+// per CU at 70 KB of LDS; 16384 points made them slower: 12 % at L = 8192).
 template <int L>
-using R1Shape = RegShape<L, (pow2c(L) && L >= 4096 ? kBatchElems : kRowBatch), 0, false>;
+using R1Shape = RegShape<L, (pow2c(L) && L >= 4096 && L != 8192 ? kBatchElems : kRowBatch), 0, false>;
 template <int L>
 using LineShape = RowReg<L>;
 
