@@ -315,3 +315,23 @@ def test_gather_shape_knobs_bitwise(tomo_cls, n, nt, nz):
     assert lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_GATHER_ZP) in (1, 2)
     assert torch.equal(t.R(x), ref)
     t.free()
+
+
+@pytest.mark.parametrize("n,nt,nz", [(1024, 96, 2), (1536, 64, 2), (2048, 64, 2)])
+def test_column_split_knob_bitwise(tomo_cls, n, nt, nz):
+    """Half as many columns per column-FFT workgroup (MM_RADON_TUNE_COL_SPLIT = 1, the default at n = 1024 and
+    1536) and the full-size workgroups (0) give R and RT bit for bit the same: the per-column transforms are
+    identical, only their packing into workgroups changes."""
+    from model_mosaic_amd import _lib
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(n + nt)
+    x = torch.from_numpy(rng.standard_normal((nz, n, n)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.standard_normal((nt, nz, n)).astype(np.float32)).cuda()
+    t = tomo_cls(n, nz, th)
+    assert t._lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_COL_SPLIT) == (1 if n in (1024, 1536) else 0)
+    outs = []
+    for knob in (0, 1):
+        _lib.check(t._lib.mm_radon_tune(t._plan, _lib.MM_RADON_TUNE_COL_SPLIT, knob), "tune")
+        outs.append((t.R(x).clone(), t.RT(y).clone()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    t.free()
