@@ -94,12 +94,18 @@ __global__ MM_TWO_FFT_BUDGET __launch_bounds__(RowReg<L>::NT) void filter_rows_2
     v[m] = cscale(v[m], G[i <= L / 2 ? i : L - i]);
   }
   fft_reg<L, Sh::TPC, true, 1, false>(v, lds + c * Sh::S, t, tw);
+  // the edge tables for every point first (read pairwise next to the stores they were waited on in pairs)
+  float ai[MH], bi[MH];
+#pragma unroll
+  for (int m = 0; m < MH; ++m) {
+    ai[m] = A[t + m * Sh::TPC];
+    bi[m] = B[t + m * Sh::TPC];
+  }
 #pragma unroll
   for (int m = 0; m < MH; ++m) {
     const int i = t + m * Sh::TPC;
-    const float ai = A[i], bi = B[i];
-    if (has_a) pa[i] = fmaf(a1, bi, fmaf(a0, ai, v[m].x));
-    if (has_b) pb[i] = fmaf(b1, bi, fmaf(b0, ai, v[m].y));
+    if (has_a) pa[i] = fmaf(a1, bi[m], fmaf(a0, ai[m], v[m].x));
+    if (has_b) pb[i] = fmaf(b1, bi[m], fmaf(b0, ai[m], v[m].y));
   }
 }
 
