@@ -588,23 +588,20 @@ __global__ __launch_bounds__(LineShape<L>::NT) void r4_line_ifft(const float2* _
   const long long q = (long long)blockIdx.x * Sh::C + c, th2 = q / nz;
   const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
   const bool va = ra < nrows, vb = rb < nrows;
-  const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+  // reads from in-bounds rows (row 0 for a missing line), predicates applied afterwards: every read of
+  // the workgroup is in flight at once (the sign read and per-lane predicated reads were waited on first)
+  const int sga0 = sig[(va ? ra : 0) / nz], sgb0 = sig[(vb ? rb : 0) / nz];
+  const int sga = va ? sga0 : 1, sgb = vb ? sgb0 : 1;
   const float2* srca = gh + (va ? ra : 0) * H;
   const float2* srcb = gh + (vb ? rb : 0) * H;
   constexpr int half = L / 2;
-  // line value at r from the representatives (k = r - L/2: |k| and a conjugation by the sign)
+  // line value at r from the representatives (k = r - L/2: |k|, conjugated by the sign of the angle)
   auto rebuild = [&](const float2* src, int sg, bool valid, int r) {
     const int d = r - half;
-    int kk;
-    bool cj;
-    if (sg > 0) {
-      kk = d >= 0 ? d : -d;
-      cj = d < 0;
-    } else {
-      kk = d <= 0 ? -d : d;
-      cj = d > 0;
-    }
-    float2 w = valid ? src[kk] : make_float2(0.f, 0.f);
+    const int kk = d >= 0 ? d : -d;
+    const bool cj = sg > 0 ? d < 0 : d > 0;
+    float2 w = src[kk];
+    if (!valid) w = make_float2(0.f, 0.f);
     if (cj) w.y = -w.y;
     if (r == 0 || r == half) w.y = 0.f;  // Hermitian part
     return w;
@@ -681,14 +678,18 @@ __global__ __launch_bounds__(LineShape<L>::NT) void rt1_line_fft(const void* __r
     const long long q = (long long)blockIdx.x * Sh::C + c, th2 = q / nz;
     const long long ra = th2 * 2 * nz + (q - th2 * nz), rb = ra + nz;
     const bool va = ra < nrows, vb = rb < nrows;
-    const int sga = va ? sig[ra / nz] : 1, sgb = vb ? sig[rb / nz] : 1;
+    const int sga0 = sig[(va ? ra : 0) / nz], sgb0 = sig[(vb ? rb : 0) / nz];
+    const int sga = va ? sga0 : 1, sgb = vb ? sgb0 : 1;
     const float* pa = (const float*)sino_v + (va ? ra : 0) * L;
     const float* pb = (const float*)sino_v + (vb ? rb : 0) * L;
     float2 v[Sh::PT];
+    // reads from in-bounds rows (row 0 for a missing line), zeroed afterwards: all in flight at once
 #pragma unroll
     for (int m = 0; m < Sh::PT; ++m) {
       const int r = t + m * Sh::TPC;
-      float2 w = make_float2(va ? pa[r] : 0.f, vb ? pb[r] : 0.f);
+      float2 w = make_float2(pa[r], pb[r]);
+      if (!va) w.x = 0.f;
+      if (!vb) w.y = 0.f;
       if (!(r & 1)) w = make_float2(-w.x, -w.y);
       v[m] = w;
     }
