@@ -414,14 +414,12 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
   // output of sorted sample t_j = threadIdx.x + j NT
   int src_lane[SPT];
   long long out0[SPT];
-  // Root shape (two samples per lane): the sample reads of both go out together, in two dependent rounds
-  // (ids and lane order; then directions and the ids in lane order), from clamped indices; one sample at a
-  // time behind the s < count predicate they were issued one by one, each waited on.  The leaf shape (one
-  // sample per lane, 64 VGPRs) keeps the reads in the per-sample loop: holding them spills.
-  constexpr bool kBatchReads = SPT > 1;
+  // The sample reads of all SPT samples go out together, in two dependent rounds (ids and lane order; then
+  // directions and the ids in lane order), from clamped indices: one sample at a time behind the
+  // s < count predicate they were issued one by one, each waited on.
   int idj[SPT], idl[SPT];
   double2 dj[SPT];
-  if (kBatchReads && it.count > 0) {
+  if (it.count > 0) {
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
       const int sc = it.start + min((int)threadIdx.x + j * NT, it.count - 1);
@@ -443,10 +441,6 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
       src_lane[j] = 0;
       continue;
     }
-    if constexpr (!kBatchReads) {
-      idj[j] = samples[it.start + s];
-      dj[j] = dirs[idj[j] / H];
-    }
     const SampleGeom g = sample_geom_d(idj[j], H, dj[j]);
     const int b = (g.ry - it.y0) * pp + (g.rx - it.x0);
     const int par = ZP == 1 ? (b & 1) : 0;
@@ -464,10 +458,6 @@ void gather_tiles(const float2* __restrict__ fde, float2* __restrict__ gh, const
     for (int i = 0; i < TAPS; ++i) {
       const float dy = (float)(i - M) - g.fy;
       ey[j][i] = ((i & 1) ? -sy : sy) * __expf(-a * dy * dy);
-    }
-    if constexpr (!kBatchReads) {
-      src_lane[j] = order[it.start + s];
-      idl[j] = samples[it.start + src_lane[j]];
     }
     const int id = idl[j];
     const int th = id / H;
