@@ -1117,9 +1117,9 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
 // the tap q - (window base of e) and 0 off the window: 144 (entry, point)
 // pairs per entry for its 81 taps (1.78x zero fill).  One
 // v_mfma_f32_16x16x4_f32 takes 4 entries x 16 zc x 16 points; a staging round
-// holds MZ = 16 slices (two 16-row zc blocks: real and imaginary parts), so each
-// weight is evaluated once per 16 slices and each staged value is read once per
-// patch.  A workgroup of
+// holds MZ = 16 slices (two 16-row zc blocks of 8 slices, real and imaginary
+// parts interleaved), so each weight is evaluated once per 16 slices and each
+// staged value is read once per patch.  A workgroup of
 // 8 waves owns the tile, two patches per wave.
 // At plan time the item's entries are sorted by window anchor class
 // (floor(bx / 4) + 2, floor(by / 4) + 2) in [0, 6)^2, bx, by = window base
