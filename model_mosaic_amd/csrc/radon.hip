@@ -232,10 +232,11 @@ template <int L>
 using ColShape = RegShape<L, kBatchElems>;
 
 //
-// BATCH = kBatchElems: C = 4 columns (one 32-byte block row piece) per
-// workgroup, one workgroup per CU.  BATCH = kRowBatch: C = 2 (half a block),
-// two workgroups per CU, the two halves of a block kept on one XCD (xg = 2)
-// so that its L2 fetches each 128-byte line once.  (The column-fastest lane
+// BATCH = kBatchElems: 16384 points per workgroup (C = 4 columns, one 32-byte
+// block row piece, at L = 4096: one workgroup per CU).  BATCH = kRowBatch (the
+// column split knob): half as many columns, two workgroups per CU where the
+// LDS allows, the workgroups of a block kept on one XCD so that its L2 fetches
+// each 128-byte line once.  (The column-fastest lane
 // mapping of the Fresnel and Paganin column passes is slower here: 1.27 ->
 // 1.49 ms for the forward pass at n = 2048; a wave already reads one
 // contiguous 512-byte run of the block layout.)
