@@ -1136,7 +1136,10 @@ __global__ __launch_bounds__(SNP) void scatter_reduce(const float2* __restrict__
 constexpr int MZ = 16;            // z slices per staging round (two 16-row zc blocks)
 constexpr int MZB = MZ / 8;       // zc blocks of 16
 constexpr int MROW = 2 * MZ + 4;  // floats per LDS entry row: values, then bx + M, by + M, fx, fy
-constexpr int ME_MAX = 480;       // entries per work item (78 KB of LDS: two workgroups per CU)
+// entries per work item: 52 KB of LDS and at most 80 registers hold three workgroups per CU (480 entries,
+// 78 KB and two per CU, measured 2 - 3 % slower: fewer tiles split, but less overlap of one workgroup's
+// staging and stores with another's products)
+constexpr int ME_MAX = 320;
 constexpr int MNT = 512;          // threads: 8 waves, two patches each
 constexpr int MCLS = 36;          // anchor classes
 constexpr int MNULL = ME_MAX;     // LDS row of zeros outside every window: pads the patch lists
@@ -1146,6 +1149,7 @@ constexpr int MLIST = 9 * ME_MAX + 16 * 3 + 4;
 static_assert(ME_MAX <= MNT, "one staging thread per entry");
 static_assert(MROW % 4 == 0 && (MROW / 4) % 2 == 1, "16-byte rows at an odd 16-byte stride: conflict-free staging");
 static_assert(MROW * (ME_MAX + 1) < 65536, "row offsets in 16 bits");
+static_assert(4 * (ME_MAX + 1) * MROW + 2 * (MLIST + 40) <= 160 * 1024 / 3, "three workgroups per CU");
 
 struct MItem {
   int eoff, ne;            // entries[eoff, eoff + ne), sorted by anchor class
@@ -1170,7 +1174,7 @@ __device__ __forceinline__ int patch_list_len(const unsigned short* cls, int px,
 // OFF64: byte offsets of the representatives in 64 bits (their buffer spans 4 GiB or more); otherwise
 // 32-bit offsets from a wave-uniform base.
 template <bool OFF64>
-__global__ __launch_bounds__(MNT) void scatter_mfma(const float2* __restrict__ yh, float2* __restrict__ Hb,
+__global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(6, 8))) void scatter_mfma(const float2* __restrict__ yh, float2* __restrict__ Hb,
                                                     float2* __restrict__ partial, const MItem* __restrict__ items,
                                                     const int* __restrict__ entries,
                                                     const unsigned short* __restrict__ rows,
