@@ -146,7 +146,8 @@ int mm_radon_get_tune(const mm_radon* plan, int knob);
  * tile 64 / 32, two / one z slices per patch element) with every gather XCD
  * group size in {1, 2, 4, 8, 16} and keeps the fastest pair; for 512 <= n <=
  * 8192 then R (and RT) with the column split off and on; with back != NULL
- * last RT (sino -> back) for the scatter knob.  Synchronises the stream (not for use under graph capture).  obj is
+ * last RT (sino -> back) for the scatter XCD group size in {4, 8, 16, 32, 64}
+ * (default 64 from n = 2048 on, 32 below).  Synchronises the stream (not for use under graph capture).  obj is
  * only read; sino and back are overwritten. */
 int mm_radon_autotune(mm_radon* plan, const float* obj_dev, float* sino_dev, float* back_dev, int nz, void* stream);
 

@@ -335,3 +335,22 @@ def test_column_split_knob_bitwise(tomo_cls, n, nt, nz):
         outs.append((t.R(x).clone(), t.RT(y).clone()))
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
     t.free()
+
+
+@pytest.mark.parametrize("n,nt,nz", [(256, 384, 16), (2048, 256, 16)])
+def test_scatter_xcd_group_bitwise(tomo_cls, n, nt, nz):
+    """The scatter's XCD group size (MM_RADON_TUNE_SCATTER_XCD_GROUP: 64 by default from n = 2048 on, 32 below)
+    only reorders the work items over the chip: every tile and partial slot has one owner and scatter_reduce
+    sums the slots in a fixed order, so RT is bit for bit the same for every group size."""
+    from model_mosaic_amd import _lib
+    th = np.linspace(0, np.pi, nt, endpoint=False).astype(np.float32)
+    rng = np.random.default_rng(7 * n + nt)
+    y = torch.from_numpy(rng.standard_normal((nt, nz, n)).astype(np.float32)).cuda()
+    t = tomo_cls(n, nz, th)
+    assert t._lib.mm_radon_get_tune(t._plan, _lib.MM_RADON_TUNE_SCATTER_XCD_GROUP) == (64 if n >= 2048 else 32)
+    outs = []
+    for g in (1, 4, 32, 64):
+        _lib.check(t._lib.mm_radon_tune(t._plan, _lib.MM_RADON_TUNE_SCATTER_XCD_GROUP, g), "tune")
+        outs.append(t.RT(y).clone())
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+    t.free()
